@@ -1,0 +1,174 @@
+/* C-ABI of libsucem_hip: MI355X-native Nedelec assembly, CSR operators and Krylov
+ * solves behind the SUCEM:FEM problem API.
+ *
+ * Conventions
+ *  - Every function returns an int status (SUCEM_OK = 0); on failure
+ *    sucem_last_error() returns a thread-local message.
+ *  - Arguments suffixed _h are host pointers; _d are device pointers on the
+ *    context's device (e.g. torch.Tensor.data_ptr()).  Complex vectors are
+ *    interleaved (re, im) doubles (complex128) unless stated otherwise.
+ *  - Calls on one context are issued in order on the context's stream; calls
+ *    returning host data synchronise that stream.
+ *  - Handles own device memory; destroy them before their context.
+ *
+ * Reference paths below are relative to the reference repository root.
+ */
+#ifndef SUCEM_HIP_H
+#define SUCEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SUCEM_OK = 0,
+  SUCEM_ERR_INVALID = 1,       /* bad argument (reference: ValueError / AssertionError) */
+  SUCEM_ERR_HIP = 2,           /* HIP runtime failure */
+  SUCEM_ERR_OOM = 3,           /* device allocation failed */
+  SUCEM_ERR_OUTSIDE = 4,       /* point outside the mesh (point_source.py:79 AssertionError) */
+  SUCEM_ERR_MATERIAL = 5,      /* unknown material region (Materials.py:147-148 ValueError) */
+  SUCEM_ERR_UNSUPPORTED = 6    /* order / size outside what the library supports */
+};
+
+/* matrix selectors */
+#define SUCEM_MAT_S 1u   /* curl-curl stiffness, Forms.py:79-85 */
+#define SUCEM_MAT_T 2u   /* eps_r mass ("M" in the reference), Forms.py:71-77 */
+#define SUCEM_MAT_S0 4u  /* first-order ABC, BoundaryConditions/ABC.py:35-58 */
+
+/* solver methods / preconditioners */
+#define SUCEM_METHOD_COCG 0
+#define SUCEM_METHOD_BICGSTAB 1
+#define SUCEM_PRECOND_NONE 0
+#define SUCEM_PRECOND_JACOBI 1
+
+typedef struct sucem_ctx sucem_ctx;
+typedef struct sucem_mesh sucem_mesh;
+typedef struct sucem_space sucem_space;
+typedef struct sucem_system sucem_system;
+typedef struct sucem_csr sucem_csr;
+
+typedef struct sucem_solve_opts {
+  int method;         /* SUCEM_METHOD_* */
+  int precision;      /* 64: FP64 throughout; 32: FP32 matrix+vectors, FP64 reliable updates */
+  int precond;        /* SUCEM_PRECOND_* */
+  int max_iter;
+  int check_every;    /* host polls the residual every this many iterations */
+  int eliminate_pec;  /* 1: restrict to non-PEC DOFs (rows/cols of PEC DOFs removed) */
+  double tol;         /* relative residual ||b - A x|| / ||b|| (FP64) */
+  double ru_delta;    /* FP32: reliable update when the residual fell by this factor */
+} sucem_solve_opts;
+
+typedef struct sucem_solve_stats {
+  int status;           /* 0 converged, >0 iteration limit, <0 breakdown (scipy 'info') */
+  int iterations;
+  int reliable_updates;
+  int pad_;
+  double rel_residual;  /* final FP64 ||b - A x|| / ||b|| */
+  double elapsed_ms;
+  int64_t spmv_lo;      /* SpMVs in the working precision */
+  int64_t spmv_hi;      /* FP64 residual SpMVs */
+} sucem_solve_stats;
+
+const char* sucem_last_error(void);
+int sucem_version(void);
+
+/* Context: device and the stream all work is issued on (NULL = the null stream). */
+int sucem_ctx_create(int device, void* hip_stream, sucem_ctx** out);
+int sucem_ctx_destroy(sucem_ctx* ctx);
+int sucem_ctx_synchronize(sucem_ctx* ctx);
+
+/* ---- mesh (replaces the DOLFIN Mesh handed to EMProblem.set_mesh, EMProblem.py:49-54) ---- */
+/* Cells are sorted per row on the device (DOLFIN mesh ordering). */
+int sucem_mesh_create(sucem_ctx* ctx, const double* xyz_h, int64_t nv, const int64_t* cells_h,
+                      int64_t nc, sucem_mesh** out);
+/* DOLFIN UnitCube(nx,ny,nz) Kuhn mesh, x -> x*scale + shift per axis, generated on the
+ * device (Utilities/MeshGenerators.py:26-53 without the centring step). */
+int sucem_mesh_create_box(sucem_ctx* ctx, int64_t nx, int64_t ny, int64_t nz,
+                          const double* scale3_h, const double* shift3_h, sucem_mesh** out);
+int sucem_mesh_info(const sucem_mesh* m, int64_t* nv, int64_t* nc);
+int sucem_mesh_export(sucem_mesh* m, double* xyz_h, int64_t* cells_h);
+/* Translate every vertex by -shift3 (the centring step of get_centred_cube). */
+int sucem_mesh_translate(sucem_mesh* m, const double* shift3_h);
+/* Per-cell eps_r and 1/mu_r from a region id per cell (NULL: all region 0) and a
+ * region table (MaterialFunctionFactory.get_material_functions, Materials.py:118-150).
+ * Unknown region -> SUCEM_ERR_MATERIAL. */
+int sucem_mesh_set_materials(sucem_mesh* m, const int64_t* region_h, int64_t nregions,
+                             const int64_t* region_ids_h, const double* eps_r_h,
+                             const double* mu_r_h);
+int sucem_mesh_destroy(sucem_mesh* m);
+
+/* ---- function space (replaces dolfin.FunctionSpace(mesh, "Nedelec 1st kind H(curl)", order),
+ *      EMProblem.py:100-105): first-encounter numbering, DOF map, CSR pattern, PEC DOFs ---- */
+int sucem_space_create(sucem_ctx* ctx, sucem_mesh* m, int order, sucem_space** out);
+/* info[0..7] = ndofs, nnz, nedges, nfaces, nboundary_faces, dofs_per_cell, nnz_abc, npec */
+int sucem_space_info(const sucem_space* s, int64_t* info8_h);
+/* CSR pattern as the uBLAS export gives it (Utilities/Converters.py:27-44): indptr[N+1],
+ * indices[nnz] (sorted columns). */
+int sucem_space_export_pattern(sucem_space* s, int64_t* indptr_h, int32_t* indices_h);
+int sucem_space_export_dofmap(sucem_space* s, int64_t* cell_dofs_h);
+int sucem_space_export_entities(sucem_space* s, int64_t* edges_h, int64_t* faces_h);
+/* DOFs constrained by PEC walls on every boundary facet (essential.py:86-99). */
+int sucem_space_export_pec(sucem_space* s, uint8_t* mask_h);
+int sucem_space_destroy(sucem_space* s);
+
+/* ---- assembly (replaces SystemMatrices.calc_system_matrices, SystemMatrices.py:46-58) ---- */
+int sucem_assemble(sucem_ctx* ctx, sucem_space* s, unsigned which, sucem_system** out);
+/* Values of one matrix (SUCEM_MAT_S, _T or _S0) on the full pattern, explicit zeros kept. */
+int sucem_system_export(sucem_system* sys, unsigned which_one, double* values_h);
+int sucem_system_destroy(sucem_system* sys);
+
+/* ---- driven operator (replaces DrivenProblemABC.get_LHS_matrix, EMDrivenProblem.py:67-72):
+ *      y = (S - k0^2 T + i k0 S0) x for nvec complex vectors, precision 64 (complex128 x/y)
+ *      or 32 (complex64 x/y and FP32 matrix values). ---- */
+int sucem_system_apply(sucem_ctx* ctx, sucem_system* sys, double k0, int precision,
+                       int eliminate_pec, int64_t nvec, const void* x_d, void* y_d);
+/* General combination y = (cs S + ct T + i c0 S0) x (e.g. T x for M*x, S - sigma T for the
+ * eigen shift of EMVectorWaveEigenproblem.py:74-76). */
+int sucem_system_apply_coef(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
+                            int precision, int eliminate_pec, int64_t nvec, const void* x_d,
+                            void* y_d);
+/* Materialised A values (complex128 interleaved) on the full pattern. */
+int sucem_system_export_lhs(sucem_system* sys, double k0, double* values_h);
+
+/* ---- sources (replaces calc_pointsource_contrib, Sources/point_source.py:49-94) ---- */
+/* Lowest-index cell containing each point; SUCEM_ERR_OUTSIDE if a point is in none. */
+int sucem_locate_points(sucem_ctx* ctx, sucem_space* s, int64_t n, const double* pts_h,
+                        int64_t* cells_h);
+/* Physical basis values N_i(x) (n x dofs_per_cell x 3) and the cell DOFs (n x dofs_per_cell). */
+int sucem_basis_at_points(sucem_ctx* ctx, sucem_space* s, int64_t n, const double* pts_h,
+                          double* vals_h, int64_t* dofs_h);
+/* b[r] += scale * sum_{s: rhs(s)=r} N(x_s) . p_s  for nrhs complex vectors of length N,
+ * sources added in input order (DrivenProblemABC.get_RHS, EMDrivenProblem.py:74-80, with
+ * scale = -i k0 Z0).  pos_h: nsrc x 3, val_h: nsrc x 3 complex, rhs_h: nsrc indices. */
+int sucem_rhs_point_sources(sucem_ctx* ctx, sucem_space* s, int64_t nsrc, const double* pos_h,
+                            const double* val_h, const int64_t* rhs_h, int64_t nrhs,
+                            double scale_re, double scale_im, double* b_d);
+
+/* ---- solves (replaces LinalgSolvers.BiCGStabSolver/solve_sparse_system,
+ *      Utilities/LinalgSolvers.py:239-248,308-322, on the driven system) ---- */
+void sucem_solve_opts_default(sucem_solve_opts* o);
+int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const double* b_d,
+                       double* x_d, const sucem_solve_opts* opts, sucem_solve_stats* stats,
+                       double* res_hist_h, int64_t hist_cap);
+
+/* Solve (cs S + ct T + i c0 S0) x = b (e.g. the shifted S - sigma T of the eigen solver). */
+int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
+                       const double* b_d, double* x_d, const sucem_solve_opts* opts,
+                       sucem_solve_stats* stats, double* res_hist_h, int64_t hist_cap);
+
+/* ---- generic complex CSR (LinalgSolvers on a user scipy matrix, LinalgSolvers.py:30-322) ---- */
+int sucem_csr_create(sucem_ctx* ctx, int64_t n, int64_t nnz, const int64_t* indptr_h,
+                     const int32_t* indices_h, const double* values_h, sucem_csr** out);
+int sucem_csr_apply(sucem_ctx* ctx, sucem_csr* A, const double* x_d, double* y_d);
+int sucem_csr_solve(sucem_ctx* ctx, sucem_csr* A, const double* b_d, double* x_d,
+                    const sucem_solve_opts* opts, sucem_solve_stats* stats, double* res_hist_h,
+                    int64_t hist_cap);
+int sucem_csr_destroy(sucem_csr* A);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUCEM_HIP_H */

@@ -1,0 +1,719 @@
+"""FP64 CPU restatement of the reference's Nedelec hot path (test infrastructure only).
+
+Every function names the reference file:line whose behaviour it restates and
+how it is pinned:
+
+=====================================  ===========================================================
+function                               pinned by
+=====================================  ===========================================================
+``unit_cube`` / ``centred_cube``       DOLFIN-written cells of ``power_flux_..._mesh.xml`` (exact);
+                                       order-2 driven solution fixture (centring tie-break)
+``number_entities`` / ``dofmap``       ``LHS_matrix.npy`` (order 1), ``fillament_source_test_data``
+                                       and the three order-2 solution fixtures (order 2); brute-force
+                                       sequential first-encounter scan on small meshes
+``reference_basis``                    nodal property (dual applied to basis = identity); Whitney
+                                       closed form; RHS/filament fixtures
+``element_matrices``                   closed-form Whitney mass/stiffness (order 1); ``LHS_matrix.npy``;
+                                       invariants S.G = 0 (discrete gradient), T SPD; order-2
+                                       solution fixtures
+``face_matrices`` (ABC)                imaginary part of ``LHS_matrix.npy``; order-2 solution fixtures
+``assemble``                           all of the above; pattern = union of cell pairs
+``pec_mask``                           count of boundary entities (closed form on UnitCube)
+``point_source_contrib``               ``RHS_vector.pickle`` (exact), filament fixture
+``driven_lhs`` / ``driven_rhs``        ``LHS_matrix.npy``, ``RHS_vector.pickle``
+``solve_direct``                       order-2 solution fixtures (UMFPACK solves in the reference)
+``cavity_eigs``                        analytic cavity modes (discretisation check), dense ``eigh``
+                                       on the eliminated pencil (small), null-space dimension
+=====================================  ===========================================================
+"""
+from __future__ import division
+
+import itertools
+import math
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+# ---------------------------------------------------------------------------
+# Constants -- sucemfem/Consts.py:24-33 (evaluated in the same order)
+# ---------------------------------------------------------------------------
+mu0 = np.pi * 4 * 1e-7
+c0 = 299792458
+eps0 = 10 ** 7 / 4 / np.pi / c0 / c0
+Z0 = mu0 * c0
+
+# UFC local entities of a tetrahedron with sorted vertices
+LOCAL_EDGES = np.array([(2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1)], dtype=np.int64)
+LOCAL_FACES = np.array([(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)], dtype=np.int64)
+REF_VERTS = np.array([[0., 0., 0.], [1., 0., 0.], [0., 1., 0.], [0., 0., 1.]])
+
+
+# ---------------------------------------------------------------------------
+# Meshes
+# ---------------------------------------------------------------------------
+def unit_cube(nx, ny, nz):
+    """DOLFIN 1.0 ``UnitCube(nx, ny, nz)``: Kuhn split of each box into 6 tets.
+
+    Follows the DOLFIN mesh that ``Utilities/MeshGenerators.py:33`` builds.
+    Vertex ``v = iz*(nx+1)*(ny+1) + iy*(nx+1) + ix`` at ``(ix/nx, iy/ny, iz/nz)``;
+    boxes visited with ix fastest; each cell's vertex list sorted ascending
+    (UFC ordering).  Returns ``(coords float64[nv,3], cells int64[nc,4])``.
+    """
+    ix, iy, iz = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    ix, iy, iz = (a.transpose(2, 1, 0).ravel() for a in (ix, iy, iz))
+    coords = np.stack([ix / nx, iy / ny, iz / nz], axis=1).astype(np.float64)
+    bx, by, bz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    bx, by, bz = (a.transpose(2, 1, 0).ravel() for a in (bx, by, bz))
+    sx, sxy = nx + 1, (nx + 1) * (ny + 1)
+    v0 = bz * sxy + by * sx + bx
+    v1 = v0 + 1
+    v2 = v0 + sx
+    v3 = v1 + sx
+    v4, v5, v6, v7 = v0 + sxy, v1 + sxy, v2 + sxy, v3 + sxy
+    six = np.stack([
+        np.stack([v0, v1, v3, v7], 1), np.stack([v0, v1, v7, v5], 1),
+        np.stack([v0, v5, v7, v4], 1), np.stack([v0, v3, v2, v7], 1),
+        np.stack([v0, v6, v4, v7], 1), np.stack([v0, v2, v6, v7], 1)], axis=1)
+    cells = np.sort(six.reshape(-1, 4), axis=1).astype(np.int64)
+    return coords, cells
+
+
+def box_mesh(dims, n):
+    """UnitCube(n) scaled per axis by ``dims`` (as ``examples/resonant_cavity/driver.py:34-40``)."""
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    coords, cells = unit_cube(nx, ny, nz)
+    coords = coords * np.asarray(dims, dtype=np.float64)[None, :]
+    return coords, cells
+
+
+def cell_midpoints(coords, cells):
+    return coords[cells].mean(axis=1)
+
+
+def centred_cube(domain_size, max_edge_len, centred_element_coordinate=None):
+    """``get_centred_cube`` -- ``sucemfem/Utilities/MeshGenerators.py:26-53``.
+
+    n = ceil(sqrt(2)*L/h) boxes per axis; coordinates scaled by L and shifted by
+    -L/2.  With ``centred_element_coordinate`` the mesh is translated so that the
+    point is at the centroid of the containing cell whose midpoint is nearest
+    (ties -> lowest cell index, i.e. the first Kuhn tet of a box).
+    """
+    domain_size = np.asarray(domain_size, dtype=np.float64) * np.ones(3)
+    n = np.array(np.ceil(np.sqrt(2) * domain_size / max_edge_len), dtype=np.int64)
+    coords, cells = unit_cube(*n)
+    coords = coords * domain_size
+    coords = coords - domain_size / 2
+    if centred_element_coordinate is not None:
+        p = np.asarray(centred_element_coordinate, dtype=np.float64)
+        cand = containing_cells(coords, cells, p)
+        mids = cell_midpoints(coords, cells[cand])
+        d = np.linalg.norm(mids - p, axis=1)
+        best = cand[np.flatnonzero(d <= d.min() * (1 + 1e-12) + 1e-300)[0]]
+        coords = coords - cell_midpoints(coords, cells[best:best + 1])[0]
+    return coords, cells
+
+
+def inscribed_tet():
+    """``InscribedTet`` -- ``sucemfem/Testing/Meshes.py:25-61`` (cells sorted as DOLFIN orders them)."""
+    coords = np.array([[-1. / 2, 1. / 2, -1. / 2], [1. / 2, 1. / 2, 1. / 2],
+                       [1. / 2, -1. / 2, -1. / 2], [-1. / 2, -1. / 2, 1. / 2],
+                       [1. / 6, 1. / 6, -1. / 6], [-1. / 6, 1. / 6, 1. / 6],
+                       [-1. / 6, -1. / 6, -1. / 6], [1. / 6, -1. / 6, 1. / 6]], np.float64)
+    cells = np.array([[1, 4, 6, 7], [1, 3, 5, 7], [3, 4, 7, 8], [1, 2, 5, 6], [2, 3, 5, 8],
+                      [2, 4, 6, 8], [4, 6, 7, 8], [1, 5, 6, 7], [3, 5, 7, 8], [2, 5, 6, 8],
+                      [5, 6, 7, 8]], np.int64) - 1
+    return coords, np.sort(cells, axis=1)
+
+
+# ---------------------------------------------------------------------------
+# Topology and DOF numbering (DOLFIN first encounter)
+# ---------------------------------------------------------------------------
+def _first_encounter(keys):
+    """Global ids of the rows of ``keys`` (n, k) numbered in order of first appearance."""
+    uniq, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty(len(uniq), dtype=np.int64)
+    rank[order] = np.arange(len(uniq))
+    return uniq[order], rank[inv.ravel()]
+
+
+def number_entities(cells):
+    """Edges and faces numbered by first encounter (scan cells, then local entities).
+
+    Restates the DOLFIN topology computation behind ``dolfin.FunctionSpace`` at
+    ``ProblemConfigurations/EMProblem.py:100-105``.  Returns a dict with
+    ``edges`` (nE,2), ``cell_edges`` (nc,6), ``faces`` (nF,3), ``cell_faces`` (nc,4)
+    and ``face_cells`` (nF,2; -1 where a face has a single cell).
+    """
+    cells = np.asarray(cells, dtype=np.int64)
+    nc = len(cells)
+    ek = cells[:, LOCAL_EDGES].reshape(-1, 2)
+    edges, eid = _first_encounter(ek)
+    fk = cells[:, LOCAL_FACES].reshape(-1, 3)
+    faces, fid = _first_encounter(fk)
+    cell_faces = fid.reshape(nc, 4)
+    face_cells = -np.ones((len(faces), 2), dtype=np.int64)
+    owner = np.repeat(np.arange(nc), 4)
+    lf = np.tile(np.arange(4), nc)
+    order = np.lexsort((owner, fid))
+    fid_s, own_s, lf_s = fid[order], owner[order], lf[order]
+    first = np.ones(len(fid_s), bool)
+    first[1:] = fid_s[1:] != fid_s[:-1]
+    face_cells[fid_s[first], 0] = own_s[first]
+    face_cells[fid_s[~first], 1] = own_s[~first]
+    face_local = -np.ones((len(faces), 2), dtype=np.int64)
+    face_local[fid_s[first], 0] = lf_s[first]
+    face_local[fid_s[~first], 1] = lf_s[~first]
+    return dict(edges=edges, cell_edges=eid.reshape(nc, 6), faces=faces,
+                cell_faces=cell_faces, face_cells=face_cells, face_local=face_local)
+
+
+def number_entities_sequential(cells):
+    """Literal sequential first-encounter scan (used to pin ``number_entities``)."""
+    emap, fmap = {}, {}
+    ce = np.zeros((len(cells), 6), np.int64)
+    cf = np.zeros((len(cells), 4), np.int64)
+    for c, cell in enumerate(cells):
+        for l, (a, b) in enumerate(LOCAL_EDGES):
+            k = (cell[a], cell[b])
+            if k not in emap:
+                emap[k] = len(emap)
+            ce[c, l] = emap[k]
+        for l, f in enumerate(LOCAL_FACES):
+            k = tuple(cell[f])
+            if k not in fmap:
+                fmap[k] = len(fmap)
+            cf[c, l] = fmap[k]
+    return ce, cf
+
+
+def dofmap(order, topo):
+    """Cell -> global DOF map for Nedelec order 1 or 2.
+
+    Order 1: DOF = global edge id.  Order 2: local DOFs are 2 per local edge
+    (points 1/3 then 2/3 from the lower vertex) then 2 per local face; global
+    edge DOF ``2*E+k``, face DOF ``2*nE + 2*F + k``.
+    Returns ``(cell_dofs int64[nc, 6|20], ndofs)``.
+    """
+    ce, cf = topo["cell_edges"], topo["cell_faces"]
+    nE, nF = len(topo["edges"]), len(topo["faces"])
+    if order == 1:
+        return ce.copy(), nE
+    if order == 2:
+        nc = len(ce)
+        d = np.empty((nc, 20), np.int64)
+        d[:, 0:12:2] = 2 * ce
+        d[:, 1:12:2] = 2 * ce + 1
+        d[:, 12::2] = 2 * nE + 2 * cf
+        d[:, 13::2] = 2 * nE + 2 * cf + 1
+        return d, 2 * nE + 2 * nF
+    raise NotImplementedError("only orders 1 and 2")
+
+
+# ---------------------------------------------------------------------------
+# Polynomial machinery on the reference tetrahedron
+# ---------------------------------------------------------------------------
+MONO = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (1, 1, 0), (1, 0, 1),
+        (0, 2, 0), (0, 1, 1), (0, 0, 2)]
+_MIDX = {m: i for i, m in enumerate(MONO)}
+
+
+def _mono_vals(pts):
+    pts = np.atleast_2d(pts)
+    return np.stack([pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c for a, b, c in MONO], axis=1)
+
+
+def _times_coord(coef, axis):
+    """Multiply a scalar polynomial (coefficients over MONO) by x_axis."""
+    out = np.zeros_like(coef)
+    for i, m in enumerate(MONO):
+        if coef[i] != 0:
+            mm = list(m)
+            mm[axis] += 1
+            out[_MIDX[tuple(mm)]] += coef[i]
+    return out
+
+
+def _deriv(coef, axis):
+    out = np.zeros_like(coef)
+    for i, m in enumerate(MONO):
+        if coef[i] != 0 and m[axis] > 0:
+            mm = list(m)
+            mm[axis] -= 1
+            out[_MIDX[tuple(mm)]] += coef[i] * m[axis]
+    return out
+
+
+def _cross_x(v):
+    """x cross v for a vector polynomial v (3, nmono)."""
+    return np.stack([_times_coord(v[2], 1) - _times_coord(v[1], 2),
+                     _times_coord(v[0], 2) - _times_coord(v[2], 0),
+                     _times_coord(v[1], 0) - _times_coord(v[0], 1)])
+
+
+def _curl(v):
+    return np.stack([_deriv(v[2], 1) - _deriv(v[1], 2),
+                     _deriv(v[0], 2) - _deriv(v[2], 0),
+                     _deriv(v[1], 0) - _deriv(v[0], 1)])
+
+
+def _eval_field(v, pts):
+    return _mono_vals(pts) @ v.T           # (npts, 3)
+
+
+def _dual_functionals(order):
+    """(point, tangent) pairs of the FIAT nodal dual, in local DOF order."""
+    funcs = []
+    for a, b in LOCAL_EDGES:
+        t = REF_VERTS[b] - REF_VERTS[a]
+        svals = [0.5] if order == 1 else [1. / 3, 2. / 3]
+        for s in svals:
+            funcs.append((REF_VERTS[a] + s * t, t))
+    if order == 2:
+        for p, q, r in LOCAL_FACES:
+            cen = (REF_VERTS[p] + REF_VERTS[q] + REF_VERTS[r]) / 3
+            funcs.append((cen, REF_VERTS[q] - REF_VERTS[p]))
+            funcs.append((cen, REF_VERTS[r] - REF_VERTS[p]))
+    return funcs
+
+
+_BASIS_CACHE = {}
+
+
+def reference_basis(order):
+    """Nodal Nedelec-1 basis on the reference tet as vector polynomials (ndof, 3, nmono).
+
+    Space: P_{k-1}^3 + x cross (homogeneous P_{k-1})^3; nodal w.r.t. tangential
+    point evaluations with unnormalised edge/face tangents (FIAT convention).
+    """
+    if order in _BASIS_CACHE:
+        return _BASIS_CACHE[order]
+    nm = len(MONO)
+    prim = []
+    unit = np.eye(nm)
+    if order == 1:
+        for a in range(3):
+            v = np.zeros((3, nm)); v[a, 0] = 1; prim.append(v)
+        for a in range(3):
+            v = np.zeros((3, nm)); v[a, 0] = 1; prim.append(_cross_x(v))
+    elif order == 2:
+        for a in range(3):
+            for m in range(4):
+                v = np.zeros((3, nm)); v[a, m] = 1; prim.append(v)
+        for a in range(3):
+            for bb in range(3):
+                if (a, bb) == (2, 2):
+                    continue
+                v = np.zeros((3, nm)); v[a] = unit[_MIDX[tuple(np.eye(3, dtype=int)[bb])]]
+                prim.append(_cross_x(v))
+    else:
+        raise NotImplementedError
+    prim = np.array(prim)
+    duals = _dual_functionals(order)
+    V = np.array([[_eval_field(phi, pt[None])[0] @ t for phi in prim] for pt, t in duals])
+    C = np.linalg.solve(V, np.eye(len(prim)))
+    basis = np.einsum("kj,kdm->jdm", C, prim)
+    curls = np.array([_curl(b) for b in basis])
+    _BASIS_CACHE[order] = (basis, curls)
+    return basis, curls
+
+
+def apply_duals(order, fields):
+    """Apply the nodal dual to vector polynomials (n, 3, nmono) -> (n, ndof)."""
+    duals = _dual_functionals(order)
+    return np.array([[_eval_field(f, pt[None])[0] @ t for pt, t in duals] for f in fields])
+
+
+# ---------------------------------------------------------------------------
+# Quadrature (collapsed Gauss-Legendre, exact for the degrees used here)
+# ---------------------------------------------------------------------------
+def tet_rule(n=4):
+    x, w = np.polynomial.legendre.leggauss(n)
+    x = (x + 1) / 2; w = w / 2
+    pts, wts = [], []
+    for i, j, k in itertools.product(range(n), repeat=3):
+        a, b, c = x[i], x[j], x[k]
+        # Duffy: (a,b,c) in unit cube -> tet
+        px = a * (1 - b) * (1 - c)
+        py = b * (1 - c)
+        pz = c
+        jac = (1 - b) * (1 - c) ** 2
+        pts.append((px, py, pz)); wts.append(w[i] * w[j] * w[k] * jac)
+    return np.array(pts), np.array(wts)
+
+
+def tri_rule(n=4):
+    x, w = np.polynomial.legendre.leggauss(n)
+    x = (x + 1) / 2; w = w / 2
+    pts, wts = [], []
+    for i, j in itertools.product(range(n), repeat=2):
+        a, b = x[i], x[j]
+        pts.append((a * (1 - b), b)); wts.append(w[i] * w[j] * (1 - b))
+    return np.array(pts), np.array(wts)
+
+
+# ---------------------------------------------------------------------------
+# Physical basis evaluation and element matrices
+# ---------------------------------------------------------------------------
+def cell_geometry(X):
+    """Jacobian J (columns X1-X0, X2-X0, X3-X0), its determinant and inverse."""
+    J = np.stack([X[1] - X[0], X[2] - X[0], X[3] - X[0]], axis=1)
+    return J, np.linalg.det(J), np.linalg.inv(J)
+
+
+def eval_basis(order, X, pts):
+    """Physical basis values and curls at physical points of the cell with vertices X.
+
+    Covariant Piola map N = J^-T N^(x^), curl N = J curl N^ / det J.
+    Returns ``(vals (npts, ndof, 3), curls (npts, ndof, 3))``.
+    """
+    basis, curls = reference_basis(order)
+    J, detJ, Jinv = cell_geometry(X)
+    xh = (np.atleast_2d(pts) - X[0]) @ Jinv.T
+    m = _mono_vals(xh)
+    vh = np.einsum("pm,idm->pid", m, basis)
+    ch = np.einsum("pm,idm->pid", m, curls)
+    vals = vh @ Jinv            # (J^-T v)_a = sum_b Jinv[b,a] v_b
+    cvals = ch @ J.T / detJ
+    return vals, cvals
+
+
+def element_matrices(order, X, eps_r=1.0, inv_mu_r=1.0):
+    """Element mass T_e = int eps_r N_i.N_j and stiffness S_e = int (1/mu_r) curl N_i . curl N_j.
+
+    ``sucemfem/Forms.py:71-85`` (UFL forms integrated exactly by FFC).
+    """
+    qp, qw = tet_rule(4)
+    J, detJ, _ = cell_geometry(X)
+    pts = X[0] + qp @ J.T
+    vals, cvals = eval_basis(order, X, pts)
+    w = qw * abs(detJ)
+    T = eps_r * np.einsum("p,pid,pjd->ij", w, vals, vals)
+    S = inv_mu_r * np.einsum("p,pid,pjd->ij", w, cvals, cvals)
+    return T, S
+
+
+def whitney_closed_form(X):
+    """Textbook closed-form order-1 (Whitney) mass and stiffness, used to pin ``element_matrices``."""
+    J, detJ, Jinv = cell_geometry(X)
+    V = abs(detJ) / 6
+    # barycentric gradients: lambda_0 = 1 - sum, lambda_k = xhat_k
+    g = np.vstack([-Jinv.sum(axis=0), Jinv])        # (4,3)
+    T = np.zeros((6, 6)); S = np.zeros((6, 6))
+    for a, (i, j) in enumerate(LOCAL_EDGES):
+        for b, (k, l) in enumerate(LOCAL_EDGES):
+            d = lambda p, q: 1.0 if p == q else 0.0
+            T[a, b] = V / 20 * (g[j] @ g[l] * (1 + d(i, k)) - g[j] @ g[k] * (1 + d(i, l))
+                                - g[i] @ g[l] * (1 + d(j, k)) + g[i] @ g[k] * (1 + d(j, l)))
+            S[a, b] = V * 4 * np.cross(g[i], g[j]) @ np.cross(g[k], g[l])
+    return T, S
+
+
+def face_matrix(order, X, local_face):
+    """ABC face term int_F (n x N_i).(n x N_j) dS on local face ``local_face`` of the cell.
+
+    ``sucemfem/BoundaryConditions/ABC.py:56-57``; full ndof x ndof (zero off the face DOFs).
+    """
+    p, q, r = (X[i] for i in LOCAL_FACES[local_face])
+    nrm = np.cross(q - p, r - p)
+    area2 = np.linalg.norm(nrm)
+    n = nrm / area2
+    tp, tw = tri_rule(4)
+    pts = p + np.outer(tp[:, 0], q - p) + np.outer(tp[:, 1], r - p)
+    vals, _ = eval_basis(order, X, pts)
+    vt = vals - np.einsum("pid,d->pi", vals, n)[:, :, None] * n[None, None, :]
+    return np.einsum("p,pid,pjd->ij", tw * area2, vt, vt)
+
+
+# ---------------------------------------------------------------------------
+# Space, assembly and boundary data
+# ---------------------------------------------------------------------------
+class Space(object):
+    """Oracle function space: mesh + order + numbering + pattern."""
+
+    def __init__(self, coords, cells, order):
+        self.coords = np.asarray(coords, dtype=np.float64)
+        self.cells = np.sort(np.asarray(cells, dtype=np.int64), axis=1)
+        self.order = order
+        self.topo = number_entities(self.cells)
+        self.cell_dofs, self.ndofs = dofmap(order, self.topo)
+        fc = self.topo["face_cells"]
+        self.boundary_faces = np.flatnonzero(fc[:, 1] < 0)
+        self.boundary_face_cell = fc[self.boundary_faces, 0]
+        self.boundary_face_local = self.topo["face_local"][self.boundary_faces, 0]
+
+    @property
+    def nloc(self):
+        return self.cell_dofs.shape[1]
+
+    def pattern(self):
+        """DOLFIN sparsity pattern: union of all cell-local DOF pairs, sorted columns."""
+        d = self.cell_dofs
+        k = d.shape[1]
+        rows = np.repeat(d, k, axis=1).ravel()
+        cols = np.tile(d, (1, k)).ravel()
+        A = sp.coo_matrix((np.zeros(len(rows)), (rows, cols)), shape=(self.ndofs,) * 2).tocsr()
+        A.sum_duplicates()
+        A.sort_indices()
+        return A.indptr.astype(np.int64), A.indices.astype(np.int64)
+
+    def face_dofs_local(self, local_face):
+        """Local DOF indices whose tangential trace lives on local face ``local_face``."""
+        fverts = set(LOCAL_FACES[local_face])
+        loc = []
+        for l, (a, b) in enumerate(LOCAL_EDGES):
+            if a in fverts and b in fverts:
+                loc += [l] if self.order == 1 else [2 * l, 2 * l + 1]
+        if self.order == 2:
+            loc += [12 + 2 * local_face, 13 + 2 * local_face]
+        return np.array(loc, np.int64)
+
+    def pec_mask(self):
+        """DOFs on boundary facets (edges and faces of boundary faces).
+
+        ``BoundaryConditions/essential.py:86-99`` marks all boundary facets; the
+        topological DirichletBC constrains the DOFs of their closure.
+        """
+        mask = np.zeros(self.ndofs, bool)
+        for c, lf in zip(self.boundary_face_cell, self.boundary_face_local):
+            mask[self.cell_dofs[c, self.face_dofs_local(lf)]] = True
+        return mask
+
+
+def _scatter(space, rows_local_vals):
+    d = space.cell_dofs
+    k = d.shape[1]
+    rows = np.repeat(d, k, axis=1).ravel()
+    cols = np.tile(d, (1, k)).ravel()
+    A = sp.coo_matrix((rows_local_vals.ravel(), (rows, cols)), shape=(space.ndofs,) * 2).tocsr()
+    A.sum_duplicates()
+    A.sort_indices()
+    return A
+
+
+def assemble(space, eps_r=None, inv_mu_r=None, abc=True):
+    """Assemble ``{'M': T, 'S': S, 'S_0': S0}`` as scipy CSR on the DOLFIN pattern.
+
+    ``SystemMatrices.py:46-58`` + ``EMDrivenProblem.py:33-38``.  Explicit zeros
+    of the pattern are kept (as the uBLAS export at ``Converters.py:27-44``).
+    """
+    nc = len(space.cells)
+    eps_r = np.ones(nc) if eps_r is None else np.broadcast_to(np.asarray(eps_r, float), (nc,))
+    inv_mu_r = np.ones(nc) if inv_mu_r is None else np.broadcast_to(np.asarray(inv_mu_r, float), (nc,))
+    k = space.nloc
+    Te = np.empty((nc, k, k)); Se = np.empty((nc, k, k))
+    for c in range(nc):
+        Te[c], Se[c] = element_matrices(space.order, space.coords[space.cells[c]], eps_r[c], inv_mu_r[c])
+    out = {"M": _scatter(space, Te), "S": _scatter(space, Se)}
+    if abc:
+        S0e = np.zeros((nc, k, k))
+        for c, lf in zip(space.boundary_face_cell, space.boundary_face_local):
+            S0e[c] += face_matrix(space.order, space.coords[space.cells[c]], lf)
+        out["S_0"] = _scatter(space, S0e)
+    return out
+
+
+def element_matrices_batched(order, coords, cells, eps_r, inv_mu_r):
+    """Vectorised element matrices (same mathematics as ``element_matrices``), for larger meshes."""
+    basis, curls = reference_basis(order)
+    qp, qw = tet_rule(4)
+    m = _mono_vals(qp)
+    vh = np.einsum("pm,idm->pid", m, basis)          # (q, k, 3)
+    ch = np.einsum("pm,idm->pid", m, curls)
+    X = coords[cells]
+    J = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]], axis=2)
+    detJ = np.linalg.det(J)
+    Jinv = np.linalg.inv(J)
+    G = np.einsum("cab,ceb->cae", Jinv, Jinv)         # Jinv Jinv^T
+    K = np.einsum("cba,cbe->cae", J, J)               # J^T J
+    R = np.einsum("q,qia,qjb->abij", qw, vh, vh)
+    Q = np.einsum("q,qia,qjb->abij", qw, ch, ch)
+    T = np.einsum("c,cab,abij->cij", eps_r * np.abs(detJ), G, R)
+    S = np.einsum("c,cab,abij->cij", inv_mu_r / np.abs(detJ), K, Q)
+    return T, S
+
+
+def assemble_fast(space, eps_r=None, inv_mu_r=None, abc=True):
+    """Same result as ``assemble`` with batched element matrices (face terms per face)."""
+    nc = len(space.cells)
+    eps_r = np.ones(nc) if eps_r is None else np.broadcast_to(np.asarray(eps_r, float), (nc,))
+    inv_mu_r = np.ones(nc) if inv_mu_r is None else np.broadcast_to(np.asarray(inv_mu_r, float), (nc,))
+    Te, Se = element_matrices_batched(space.order, space.coords, space.cells, eps_r, inv_mu_r)
+    out = {"M": _scatter(space, Te), "S": _scatter(space, Se)}
+    if abc:
+        k = space.nloc
+        S0e = np.zeros((nc, k, k))
+        for c, lf in zip(space.boundary_face_cell, space.boundary_face_local):
+            S0e[c] += face_matrix(space.order, space.coords[space.cells[c]], lf)
+        out["S_0"] = _scatter(space, S0e)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Sources and the driven system
+# ---------------------------------------------------------------------------
+def barycentric(coords, cells, p):
+    X = coords[cells]
+    J = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]], axis=2)
+    lam = np.linalg.solve(J, (p - X[:, 0])[:, :, None])[:, :, 0]
+    return np.concatenate([1 - lam.sum(axis=1, keepdims=True), lam], axis=1)
+
+
+def containing_cells(coords, cells, p, tol=1e-12):
+    lam = barycentric(coords, cells, np.asarray(p, float))
+    return np.flatnonzero(np.all(lam >= -tol, axis=1))
+
+
+def locate_cell(coords, cells, p, tol=1e-12):
+    """Lowest-index cell containing ``p`` (the reference asserts that one exists,
+    ``Sources/point_source.py:79``)."""
+    cand = containing_cells(coords, cells, p, tol)
+    if len(cand) == 0:
+        raise AssertionError("point %r is outside the mesh" % (p,))
+    return int(cand[0])
+
+
+def point_source_contrib(space, position, value):
+    """``calc_pointsource_contrib`` -- ``Sources/point_source.py:49-94``.
+
+    Returns ``(dofnos, sum_k N_k(x_s) . value)`` for the cell containing x_s.
+    """
+    position = np.asarray(position, np.float64)
+    value = np.asarray(value)
+    c = locate_cell(space.coords, space.cells, position)
+    vals, _ = eval_basis(space.order, space.coords[space.cells[c]], position[None])
+    return space.cell_dofs[c].copy(), vals[0] @ value
+
+
+def filament_contrib(space, endpoints, I, no_pts):
+    """``FillamentCurrentSource.get_contribution`` -- ``Sources/fillament_current_source.py:62-82``."""
+    start, end = np.asarray(endpoints[0], float), np.asarray(endpoints[1], float)
+    delta = end - start
+    length = np.linalg.norm(delta)
+    vec = delta / length * I
+    pts = [start + delta * 0.5] if no_pts == 1 else start + delta * np.linspace(0, 1, no_pts)[:, None]
+    mag = vec * length / no_pts
+    acc = {}
+    for pt in pts:
+        dn, v = point_source_contrib(space, pt, mag)
+        for a, b in zip(dn, v):
+            acc[a] = acc.get(a, 0.) + b
+    keys = np.array(sorted(acc), np.int64)
+    return keys, np.array([acc[k] for k in keys])
+
+
+def driven_lhs(mats, frequency):
+    """``DrivenProblemABC.get_LHS_matrix`` -- ``EMDrivenProblem.py:67-72``."""
+    k0 = 2 * np.pi * frequency / c0
+    return mats["S"] - k0 ** 2 * mats["M"] + 1j * k0 * mats["S_0"]
+
+
+def driven_rhs(ndofs, dofnos, contribs, frequency):
+    """``DrivenProblemABC.get_RHS`` -- ``EMDrivenProblem.py:74-80``."""
+    k0 = 2 * np.pi * frequency / c0
+    b = np.zeros(ndofs, np.complex128)
+    np.add.at(b, np.asarray(dofnos, np.int64), -1j * k0 * Z0 * np.asarray(contribs))
+    return b
+
+
+def solve_direct(A, b):
+    """Direct sparse solve (the reference's UMFPACK path, ``LinalgSolvers.py:262-272``)."""
+    return spla.spsolve(A.tocsc(), b)
+
+
+# ---------------------------------------------------------------------------
+# Eigen (resonant cavity, PEC walls)
+# ---------------------------------------------------------------------------
+def cavity_eigs(space, nev, sigma=1.1, eps_r=None, inv_mu_r=None, dense_limit=4000):
+    """Lowest ``nev`` nonzero k^2 of the PEC-eliminated pencil (S_II, T_II).
+
+    Follows ``EMVectorWaveEigenproblem.py:61-105`` with the spectrum-shift
+    correction of ``examples/resonant_cavity/driver.py:66`` applied, i.e. it
+    returns lambda = k^2 directly.  Small problems use dense ``eigh``; larger ones
+    use shift-invert Lanczos about ``sigma`` selecting nu = 1/(lambda - sigma) > 0
+    (largest algebraic), which excludes the gradient null space (nu = -1/sigma).
+    """
+    mats = assemble_fast(space, eps_r, inv_mu_r, abc=False)
+    free = np.flatnonzero(~space.pec_mask())
+    S = mats["S"][free][:, free]
+    T = mats["M"][free][:, free]
+    if len(free) <= dense_limit:
+        import scipy.linalg
+        w, v = scipy.linalg.eigh(S.toarray(), T.toarray())
+        keep = np.flatnonzero(w > sigma)
+        return w[keep[:nev]], v[:, keep[:nev]], free
+    w, v = spla.eigsh(S.tocsc(), k=nev, M=T.tocsc(), sigma=sigma, which="LA")
+    order = np.argsort(w)
+    return w[order], v[:, order], free
+
+
+def analytic_cavity_k2(dims, count, steps=6):
+    """Analytic k^2 = (m pi/a)^2 + (n pi/b)^2 + (p pi/c)^2 with TE/TM multiplicity.
+
+    ``functional_tests/test_EigenProblems.py:42-65,111-122``: modes with at most
+    one zero index exist; all-nonzero index triples count twice (TE and TM).
+    """
+    a, b, c = dims
+    vals = []
+    for m, n, p in itertools.product(range(steps), repeat=3):
+        z = (m, n, p).count(0)
+        if z < 2:
+            k2 = (m * np.pi / a) ** 2 + (n * np.pi / b) ** 2 + (p * np.pi / c) ** 2
+            vals.append(k2)
+            if z == 0:
+                vals.append(k2)
+    return np.sort(vals)[:count]
+
+
+# ---------------------------------------------------------------------------
+# Discrete gradient (pins S.G = 0)
+# ---------------------------------------------------------------------------
+def discrete_gradient(space):
+    """Matrix G mapping Lagrange P_k DOFs to Nedelec DOFs with N(G phi) = grad phi.
+
+    Order 1: vertex DOFs, G[e, v_hi] = 1, G[e, v_lo] = -1.
+    Order 2: P2 DOFs = vertices then edge midpoints; entries are the nodal dual
+    of grad phi (tangential evaluations), computed per cell on the reference.
+    """
+    edges = space.topo["edges"]
+    nv = len(space.coords)
+    if space.order == 1:
+        ne = len(edges)
+        r = np.repeat(np.arange(ne), 2)
+        c = edges.ravel()
+        v = np.tile([-1.0, 1.0], ne)
+        return sp.csr_matrix((v, (r, c)), shape=(ne, nv))
+    # order 2: P2 Lagrange basis on the reference tet in barycentrics
+    nm = len(MONO)
+    lam = [np.zeros(nm) for _ in range(4)]
+    lam[0][0] = 1; lam[0][1] = -1; lam[0][2] = -1; lam[0][3] = -1
+    lam[1][1] = 1; lam[2][2] = 1; lam[3][3] = 1
+
+    def mul(p, q):
+        out = np.zeros(nm)
+        for i, mi in enumerate(MONO):
+            for j, mj in enumerate(MONO):
+                if p[i] != 0 and q[j] != 0:
+                    mm = tuple(np.add(mi, mj))
+                    out[_MIDX[mm]] += p[i] * q[j]
+        return out
+    p2 = [mul(lam[i], 2 * lam[i] - np.eye(nm)[0]) for i in range(4)]
+    p2 += [4 * mul(lam[a], lam[b]) for a, b in LOCAL_EDGES]
+    grads = np.array([np.stack([_deriv(f, 0), _deriv(f, 1), _deriv(f, 2)]) for f in p2])
+    Gloc = apply_duals(2, grads).T                    # (20 nedelec, 10 lagrange)
+    ne = len(edges)
+    rows, cols, vals = [], [], []
+    for c in range(len(space.cells)):
+        ldofs = np.concatenate([space.cells[c], nv + space.topo["cell_edges"][c]])
+        rows.append(np.repeat(space.cell_dofs[c], 10))
+        cols.append(np.tile(ldofs, 20))
+        vals.append(Gloc.ravel())
+    G = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(space.ndofs, nv + ne)).tocsr()
+    # every cell writes the same value for a shared entry; undo the summation
+    cnt = sp.coo_matrix((np.ones(sum(len(v) for v in vals)),
+                         (np.concatenate(rows), np.concatenate(cols))),
+                        shape=(space.ndofs, nv + ne)).tocsr()
+    G.data = G.data / cnt.data
+    return G

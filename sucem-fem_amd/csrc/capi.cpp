@@ -1,0 +1,635 @@
+// extern "C" entry points of libsucem_hip (declared in include/sucem_hip.h).
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace sucem {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+double* Context::host_scratch(size_t count) {
+  if (count > pinned_n) {
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    size_t want = std::max<size_t>(count, 4096);
+    SUCEM_HIP(hipHostMalloc((void**)&pinned, want * sizeof(double), hipHostMallocDefault));
+    pinned_n = want;
+  }
+  return pinned;
+}
+
+// export (AssemblyExport) helpers defined in assemble.hip / mesh_space.hip
+void mesh_translate(Context* c, Mesh* m, const double shift[3]);
+void export_lhs(Context* c, const System* sys, double k0, double2* out_d);
+
+}  // namespace sucem
+
+using namespace sucem;
+
+#define API_BEGIN try {
+#define API_END                                        \
+  return SUCEM_OK;                                     \
+  }                                                    \
+  catch (const sucem::Error& e) {                      \
+    set_last_error(e.what());                          \
+    return e.code;                                     \
+  }                                                    \
+  catch (const std::bad_alloc&) {                      \
+    set_last_error("host allocation failed");          \
+    return SUCEM_ERR_OOM;                              \
+  }                                                    \
+  catch (const std::exception& e) {                    \
+    set_last_error(e.what());                          \
+    return SUCEM_ERR_INVALID;                          \
+  }
+
+#define NOT_NULL(p) SUCEM_CHECK((p) != nullptr, SUCEM_ERR_INVALID, #p " must not be NULL")
+
+namespace {
+template <typename T>
+void to_host_i64(Context* c, int64_t* dst, const T* src_d, size_t n) {
+  std::vector<T> tmp(n);
+  d2h(c, tmp.data(), src_d, n);
+  c->sync();
+  for (size_t i = 0; i < n; ++i) dst[i] = (int64_t)tmp[i];
+}
+}  // namespace
+
+extern "C" {
+
+const char* sucem_last_error(void) { return g_last_error.c_str(); }
+
+int sucem_version(void) { return 10000; }
+
+int sucem_ctx_create(int device, void* hip_stream, sucem_ctx** out) {
+  API_BEGIN
+  NOT_NULL(out);
+  int ndev = 0;
+  SUCEM_HIP(hipGetDeviceCount(&ndev));
+  SUCEM_CHECK(device >= 0 && device < ndev, SUCEM_ERR_INVALID, "no such HIP device");
+  auto* h = new sucem_ctx();
+  Context& c = h->c;
+  c.device = device;
+  try {
+    SUCEM_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    SUCEM_HIP(hipGetDeviceProperties(&prop, device));
+    c.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    // Work is issued on exactly the caller's stream (NULL = the device's null
+    // stream), so it is ordered with the caller's own copies and kernels.
+    c.stream = (hipStream_t)hip_stream;
+    c.own_stream = false;
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int sucem_ctx_destroy(sucem_ctx* ctx) {
+  API_BEGIN
+  if (!ctx) return SUCEM_OK;
+  Context& c = ctx->c;
+  (void)hipSetDevice(c.device);
+  (void)hipStreamSynchronize(c.stream);
+  if (c.pinned) (void)hipHostFree(c.pinned);
+  if (c.own_stream) (void)hipStreamDestroy(c.stream);
+  delete ctx;
+  API_END
+}
+
+int sucem_ctx_synchronize(sucem_ctx* ctx) {
+  API_BEGIN
+  NOT_NULL(ctx);
+  ctx->c.use();
+  ctx->c.sync();
+  API_END
+}
+
+// ---------------------------------------------------------------- mesh
+int sucem_mesh_create(sucem_ctx* ctx, const double* xyz_h, int64_t nv, const int64_t* cells_h,
+                      int64_t nc, sucem_mesh** out) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(out); NOT_NULL(xyz_h); NOT_NULL(cells_h);
+  SUCEM_CHECK(nv >= 4 && nc >= 1, SUCEM_ERR_INVALID, "mesh needs at least one tetrahedron");
+  SUCEM_CHECK(nv < (int64_t(1) << 31) && nc * 20 < (int64_t(1) << 31), SUCEM_ERR_UNSUPPORTED,
+              "mesh too large for 32-bit indices");
+  Context* c = &ctx->c;
+  c->use();
+  std::vector<int32_t> cells(nc * 4);
+  for (int64_t i = 0; i < nc * 4; ++i) {
+    SUCEM_CHECK(cells_h[i] >= 0 && cells_h[i] < nv, SUCEM_ERR_INVALID, "cell vertex out of range");
+    cells[i] = (int32_t)cells_h[i];
+  }
+  auto* h = new sucem_mesh();
+  try {
+    Mesh& m = h->m;
+    m.ctx = c;
+    m.nv = nv;
+    m.nc = nc;
+    m.xyz.alloc(nv * 3);
+    m.cells.alloc(nc * 4);
+    h2d(c, m.xyz.p, xyz_h, nv * 3);
+    h2d(c, m.cells.p, cells.data(), nc * 4);
+    mesh_sort_cells(c, &m);
+    c->sync();
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int sucem_mesh_create_box(sucem_ctx* ctx, int64_t nx, int64_t ny, int64_t nz,
+                          const double* scale3_h, const double* shift3_h, sucem_mesh** out) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(out);
+  Context* c = &ctx->c;
+  c->use();
+  double sc[3] = {1, 1, 1}, sh[3] = {0, 0, 0};
+  if (scale3_h) std::memcpy(sc, scale3_h, sizeof(sc));
+  if (shift3_h) std::memcpy(sh, shift3_h, sizeof(sh));
+  auto* h = new sucem_mesh();
+  try {
+    mesh_create_box(c, &h->m, nx, ny, nz, sc, sh);
+    c->sync();
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int sucem_mesh_info(const sucem_mesh* m, int64_t* nv, int64_t* nc) {
+  API_BEGIN
+  NOT_NULL(m);
+  if (nv) *nv = m->m.nv;
+  if (nc) *nc = m->m.nc;
+  API_END
+}
+
+int sucem_mesh_export(sucem_mesh* m, double* xyz_h, int64_t* cells_h) {
+  API_BEGIN
+  NOT_NULL(m);
+  Context* c = m->m.ctx;
+  c->use();
+  if (xyz_h) d2h(c, xyz_h, m->m.xyz.p, m->m.nv * 3);
+  if (cells_h) to_host_i64(c, cells_h, m->m.cells.p, m->m.nc * 4);
+  c->sync();
+  API_END
+}
+
+int sucem_mesh_translate(sucem_mesh* m, const double* shift3_h) {
+  API_BEGIN
+  NOT_NULL(m); NOT_NULL(shift3_h);
+  Context* c = m->m.ctx;
+  c->use();
+  mesh_translate(c, &m->m, shift3_h);
+  c->sync();
+  API_END
+}
+
+int sucem_mesh_set_materials(sucem_mesh* m, const int64_t* region_h, int64_t nregions,
+                             const int64_t* region_ids_h, const double* eps_r_h,
+                             const double* mu_r_h) {
+  API_BEGIN
+  NOT_NULL(m); NOT_NULL(region_ids_h); NOT_NULL(eps_r_h); NOT_NULL(mu_r_h);
+  Context* c = m->m.ctx;
+  c->use();
+  DBuf<int32_t> reg;
+  if (region_h) {
+    std::vector<int32_t> r(m->m.nc);
+    for (int64_t i = 0; i < m->m.nc; ++i) {
+      SUCEM_CHECK(region_h[i] >= INT32_MIN && region_h[i] <= INT32_MAX, SUCEM_ERR_MATERIAL,
+                  "Material number " + std::to_string(region_h[i]) + " not found");
+      r[i] = (int32_t)region_h[i];
+    }
+    reg.alloc(m->m.nc);
+    h2d(c, reg.p, r.data(), m->m.nc);
+  }
+  int64_t bad = mesh_set_materials(c, &m->m, reg.p, nregions, region_ids_h, eps_r_h, mu_r_h);
+  if (bad >= 0) {
+    int64_t rid = region_h ? region_h[bad] : 0;
+    throw Error(SUCEM_ERR_MATERIAL, "Material number " + std::to_string(rid) + " not found");
+  }
+  API_END
+}
+
+int sucem_mesh_destroy(sucem_mesh* m) {
+  API_BEGIN
+  if (!m) return SUCEM_OK;
+  (void)hipSetDevice(m->m.ctx->device);
+  (void)hipStreamSynchronize(m->m.ctx->stream);
+  delete m;
+  API_END
+}
+
+// ---------------------------------------------------------------- space
+int sucem_space_create(sucem_ctx* ctx, sucem_mesh* m, int order, sucem_space** out) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(m); NOT_NULL(out);
+  Context* c = &ctx->c;
+  c->use();
+  auto* h = new sucem_space();
+  try {
+    h->s.ctx = c;
+    h->s.mesh = &m->m;
+    h->s.order = order;
+    space_build(c, &h->s);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int sucem_space_info(const sucem_space* s, int64_t* info) {
+  API_BEGIN
+  NOT_NULL(s); NOT_NULL(info);
+  const Space& p = s->s;
+  info[0] = p.ndofs; info[1] = p.nnz; info[2] = p.nE; info[3] = p.nF;
+  info[4] = p.nbf; info[5] = p.nloc; info[6] = p.nnz0; info[7] = p.npec;
+  API_END
+}
+
+int sucem_space_export_pattern(sucem_space* s, int64_t* indptr_h, int32_t* indices_h) {
+  API_BEGIN
+  NOT_NULL(s);
+  Context* c = s->s.ctx;
+  c->use();
+  if (indptr_h) to_host_i64(c, indptr_h, s->s.row_ptr.p, s->s.ndofs + 1);
+  if (indices_h) d2h(c, indices_h, s->s.col.p, s->s.nnz);
+  c->sync();
+  API_END
+}
+
+int sucem_space_export_dofmap(sucem_space* s, int64_t* cell_dofs_h) {
+  API_BEGIN
+  NOT_NULL(s); NOT_NULL(cell_dofs_h);
+  Context* c = s->s.ctx;
+  c->use();
+  to_host_i64(c, cell_dofs_h, s->s.cell_dofs.p, s->s.mesh->nc * s->s.nloc);
+  API_END
+}
+
+int sucem_space_export_entities(sucem_space* s, int64_t* edges_h, int64_t* faces_h) {
+  API_BEGIN
+  NOT_NULL(s);
+  Context* c = s->s.ctx;
+  c->use();
+  if (edges_h) to_host_i64(c, edges_h, s->s.edges.p, s->s.nE * 2);
+  if (faces_h) to_host_i64(c, faces_h, s->s.faces.p, s->s.nF * 3);
+  API_END
+}
+
+int sucem_space_export_pec(sucem_space* s, uint8_t* mask_h) {
+  API_BEGIN
+  NOT_NULL(s); NOT_NULL(mask_h);
+  Context* c = s->s.ctx;
+  c->use();
+  d2h(c, mask_h, s->s.pec.p, s->s.ndofs);
+  c->sync();
+  API_END
+}
+
+int sucem_space_destroy(sucem_space* s) {
+  API_BEGIN
+  if (!s) return SUCEM_OK;
+  (void)hipSetDevice(s->s.ctx->device);
+  (void)hipStreamSynchronize(s->s.ctx->stream);
+  delete s;
+  API_END
+}
+
+// ---------------------------------------------------------------- assembly
+int sucem_assemble(sucem_ctx* ctx, sucem_space* s, unsigned which, sucem_system** out) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(s); NOT_NULL(out);
+  SUCEM_CHECK(which != 0 && (which & ~7u) == 0, SUCEM_ERR_INVALID, "bad matrix selection");
+  Context* c = &ctx->c;
+  c->use();
+  auto* h = new sucem_system();
+  try {
+    assemble_system(c, &s->s, &h->s, which);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int sucem_system_export(sucem_system* sys, unsigned which_one, double* values_h) {
+  API_BEGIN
+  NOT_NULL(sys); NOT_NULL(values_h);
+  Context* c = sys->s.ctx;
+  c->use();
+  DBuf<double> tmp(sys->s.space->nnz);
+  export_values(c, &sys->s, which_one, tmp.p);
+  d2h(c, values_h, tmp.p, tmp.n);
+  c->sync();
+  API_END
+}
+
+int sucem_system_export_lhs(sucem_system* sys, double k0, double* values_h) {
+  API_BEGIN
+  NOT_NULL(sys); NOT_NULL(values_h);
+  Context* c = sys->s.ctx;
+  c->use();
+  DBuf<double2> tmp(sys->s.space->nnz);
+  export_lhs(c, &sys->s, k0, tmp.p);
+  d2h(c, (double2*)values_h, tmp.p, tmp.n);
+  c->sync();
+  API_END
+}
+
+int sucem_system_destroy(sucem_system* sys) {
+  API_BEGIN
+  if (!sys) return SUCEM_OK;
+  (void)hipSetDevice(sys->s.ctx->device);
+  (void)hipStreamSynchronize(sys->s.ctx->stream);
+  delete sys;
+  API_END
+}
+
+int sucem_system_apply(sucem_ctx* ctx, sucem_system* sys, double k0, int precision,
+                       int eliminate_pec, int64_t nvec, const void* x_d, void* y_d) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys); NOT_NULL(x_d); NOT_NULL(y_d);
+  SUCEM_CHECK(nvec >= 0, SUCEM_ERR_INVALID, "nvec must be >= 0");
+  Context* c = &ctx->c;
+  c->use();
+  Coef k = driven_coef(k0);
+  if (precision == 64) system_apply<double>(c, &sys->s, k, eliminate_pec != 0, nvec, x_d, y_d);
+  else if (precision == 32) system_apply<float>(c, &sys->s, k, eliminate_pec != 0, nvec, x_d, y_d);
+  else throw Error(SUCEM_ERR_INVALID, "precision must be 32 or 64");
+  API_END
+}
+
+int sucem_system_apply_coef(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
+                            int precision, int eliminate_pec, int64_t nvec, const void* x_d,
+                            void* y_d) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys); NOT_NULL(x_d); NOT_NULL(y_d);
+  SUCEM_CHECK(nvec >= 0, SUCEM_ERR_INVALID, "nvec must be >= 0");
+  Context* c = &ctx->c;
+  c->use();
+  Coef k;
+  k.s = cs; k.t = ct; k.s0 = c0;
+  if (precision == 64) system_apply<double>(c, &sys->s, k, eliminate_pec != 0, nvec, x_d, y_d);
+  else if (precision == 32) system_apply<float>(c, &sys->s, k, eliminate_pec != 0, nvec, x_d, y_d);
+  else throw Error(SUCEM_ERR_INVALID, "precision must be 32 or 64");
+  API_END
+}
+
+// ---------------------------------------------------------------- sources
+int sucem_locate_points(sucem_ctx* ctx, sucem_space* s, int64_t n, const double* pts_h,
+                        int64_t* cells_h) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(s); NOT_NULL(pts_h); NOT_NULL(cells_h);
+  Context* c = &ctx->c;
+  c->use();
+  if (n == 0) return SUCEM_OK;
+  DBuf<double> pts(n * 3);
+  DBuf<int32_t> cell(n);
+  h2d(c, pts.p, pts_h, n * 3);
+  locate_points(c, &s->s, n, pts.p, cell.p);
+  std::vector<int32_t> h(n);
+  d2h(c, h.data(), cell.p, n);
+  c->sync();
+  for (int64_t i = 0; i < n; ++i) {
+    if (h[i] == 0x7fffffff)
+      throw Error(SUCEM_ERR_OUTSIDE, "point (" + std::to_string(pts_h[3 * i]) + ", " +
+                                         std::to_string(pts_h[3 * i + 1]) + ", " +
+                                         std::to_string(pts_h[3 * i + 2]) + ") is outside the mesh");
+    cells_h[i] = h[i];
+  }
+  API_END
+}
+
+int sucem_basis_at_points(sucem_ctx* ctx, sucem_space* s, int64_t n, const double* pts_h,
+                          double* vals_h, int64_t* dofs_h) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(s); NOT_NULL(pts_h); NOT_NULL(vals_h);
+  Context* c = &ctx->c;
+  c->use();
+  if (n == 0) return SUCEM_OK;
+  const int nloc = s->s.nloc;
+  DBuf<double> pts(n * 3), vals(n * nloc * 3);
+  DBuf<int32_t> cell(n);
+  h2d(c, pts.p, pts_h, n * 3);
+  locate_points(c, &s->s, n, pts.p, cell.p);
+  std::vector<int32_t> h(n);
+  d2h(c, h.data(), cell.p, n);
+  c->sync();
+  for (int64_t i = 0; i < n; ++i)
+    SUCEM_CHECK(h[i] != 0x7fffffff, SUCEM_ERR_OUTSIDE, "point is outside the mesh");
+  basis_at_points(c, &s->s, n, pts.p, cell.p, vals.p);
+  d2h(c, vals_h, vals.p, vals.n);
+  if (dofs_h) {
+    std::vector<int32_t> cd(s->s.mesh->nc * 0 + nloc);
+    for (int64_t i = 0; i < n; ++i) {
+      d2h(c, cd.data(), s->s.cell_dofs.p + (int64_t)h[i] * nloc, nloc);
+      c->sync();
+      for (int k = 0; k < nloc; ++k) dofs_h[i * nloc + k] = cd[k];
+    }
+  }
+  c->sync();
+  API_END
+}
+
+int sucem_rhs_point_sources(sucem_ctx* ctx, sucem_space* s, int64_t nsrc, const double* pos_h,
+                            const double* val_h, const int64_t* rhs_h, int64_t nrhs,
+                            double scale_re, double scale_im, double* b_d) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(s); NOT_NULL(b_d);
+  SUCEM_CHECK(nsrc == 0 || (pos_h && val_h), SUCEM_ERR_INVALID, "missing source arrays");
+  Context* c = &ctx->c;
+  c->use();
+  if (nsrc == 0 || nrhs == 0) return SUCEM_OK;
+  std::vector<int32_t> rhs(nsrc, 0);
+  for (int64_t i = 0; i < nsrc; ++i) {
+    int64_t r = rhs_h ? rhs_h[i] : 0;
+    SUCEM_CHECK(r >= 0 && r < nrhs, SUCEM_ERR_INVALID, "source rhs index out of range");
+    rhs[i] = (int32_t)r;
+  }
+  DBuf<double> pos(nsrc * 3);
+  DBuf<double2> val(nsrc * 3);
+  DBuf<int32_t> rhs_d(nsrc), cell(nsrc);
+  h2d(c, pos.p, pos_h, nsrc * 3);
+  h2d(c, val.p, (const double2*)val_h, nsrc * 3);
+  h2d(c, rhs_d.p, rhs.data(), nsrc);
+  locate_points(c, &s->s, nsrc, pos.p, cell.p);
+  std::vector<int32_t> h(nsrc);
+  d2h(c, h.data(), cell.p, nsrc);
+  c->sync();
+  for (int64_t i = 0; i < nsrc; ++i)
+    SUCEM_CHECK(h[i] != 0x7fffffff, SUCEM_ERR_OUTSIDE, "point source is outside the mesh");
+  rhs_point_sources(c, &s->s, nsrc, pos.p, val.p, rhs_d.p, cell.p, nrhs,
+                    make_double2(scale_re, scale_im), (double2*)b_d);
+  c->sync();
+  API_END
+}
+
+// ---------------------------------------------------------------- solves
+void sucem_solve_opts_default(sucem_solve_opts* o) {
+  if (!o) return;
+  o->method = SUCEM_METHOD_COCG;
+  o->precision = 64;
+  o->precond = SUCEM_PRECOND_JACOBI;
+  o->max_iter = 20000;
+  o->check_every = 8;
+  o->eliminate_pec = 0;
+  o->tol = 1e-8;
+  o->ru_delta = 0.1;
+}
+
+static SolveOpts convert(const sucem_solve_opts* o) {
+  SolveOpts s;
+  if (o) {
+    s.method = o->method;
+    s.precision = o->precision;
+    s.precond = o->precond;
+    s.max_iter = o->max_iter;
+    s.check_every = o->check_every;
+    s.eliminate_pec = o->eliminate_pec;
+    s.tol = o->tol;
+    s.ru_delta = o->ru_delta;
+  }
+  SUCEM_CHECK(s.tol > 0 && s.ru_delta > 0 && s.ru_delta < 1, SUCEM_ERR_INVALID,
+              "tol must be > 0 and ru_delta in (0, 1)");
+  SUCEM_CHECK(s.precond == SUCEM_PRECOND_NONE || s.precond == SUCEM_PRECOND_JACOBI,
+              SUCEM_ERR_UNSUPPORTED, "unknown preconditioner");
+  return s;
+}
+
+static void convert_stats(const SolveStats& s, sucem_solve_stats* out) {
+  if (!out) return;
+  out->status = s.status;
+  out->iterations = s.iterations;
+  out->reliable_updates = s.reliable_updates;
+  out->pad_ = 0;
+  out->rel_residual = s.rel_residual;
+  out->elapsed_ms = s.elapsed_ms;
+  out->spmv_lo = s.spmv_lo;
+  out->spmv_hi = s.spmv_hi;
+}
+
+int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const double* b_d,
+                       double* x_d, const sucem_solve_opts* opts, sucem_solve_stats* stats,
+                       double* res_hist_h, int64_t hist_cap) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys); NOT_NULL(b_d); NOT_NULL(x_d);
+  Context* c = &ctx->c;
+  c->use();
+  SolveStats st;
+  solve_system(c, &sys->s, driven_coef(k0), (const double2*)b_d, (double2*)x_d, convert(opts), st,
+               res_hist_h, hist_cap);
+  convert_stats(st, stats);
+  API_END
+}
+
+int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
+                       const double* b_d, double* x_d, const sucem_solve_opts* opts,
+                       sucem_solve_stats* stats, double* res_hist_h, int64_t hist_cap) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys); NOT_NULL(b_d); NOT_NULL(x_d);
+  Context* c = &ctx->c;
+  c->use();
+  Coef k;
+  k.s = cs; k.t = ct; k.s0 = c0;
+  SolveStats st;
+  solve_system(c, &sys->s, k, (const double2*)b_d, (double2*)x_d, convert(opts), st, res_hist_h,
+               hist_cap);
+  convert_stats(st, stats);
+  API_END
+}
+
+int sucem_csr_create(sucem_ctx* ctx, int64_t n, int64_t nnz, const int64_t* indptr_h,
+                     const int32_t* indices_h, const double* values_h, sucem_csr** out) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(indptr_h); NOT_NULL(out);
+  SUCEM_CHECK(n > 0 && nnz >= 0 && nnz < (int64_t(1) << 31), SUCEM_ERR_INVALID, "bad CSR sizes");
+  SUCEM_CHECK(indptr_h[0] == 0 && indptr_h[n] == nnz, SUCEM_ERR_INVALID, "bad CSR indptr");
+  Context* c = &ctx->c;
+  c->use();
+  std::vector<int32_t> rp(n + 1), diag(n, -1);
+  for (int64_t i = 0; i <= n; ++i) rp[i] = (int32_t)indptr_h[i];
+  for (int64_t i = 0; i < n; ++i) {
+    SUCEM_CHECK(indptr_h[i + 1] >= indptr_h[i], SUCEM_ERR_INVALID, "CSR indptr not monotone");
+    for (int64_t j = indptr_h[i]; j < indptr_h[i + 1]; ++j) {
+      SUCEM_CHECK(indices_h[j] >= 0 && indices_h[j] < n, SUCEM_ERR_INVALID, "CSR column out of range");
+      if (indices_h[j] == i && diag[i] < 0) diag[i] = (int32_t)j;
+    }
+  }
+  std::vector<float2> v32(nnz);
+  for (int64_t j = 0; j < nnz; ++j)
+    v32[j] = make_float2((float)values_h[2 * j], (float)values_h[2 * j + 1]);
+  auto* h = new sucem_csr();
+  try {
+    Csr& A = h->a;
+    A.ctx = c;
+    A.n = n;
+    A.nnz = nnz;
+    A.row_ptr.alloc(n + 1);
+    A.col.alloc(nnz);
+    A.val64.alloc(nnz);
+    A.val32.alloc(nnz);
+    A.diag.alloc(n);
+    h2d(c, A.row_ptr.p, rp.data(), n + 1);
+    h2d(c, A.col.p, indices_h, nnz);
+    h2d(c, A.val64.p, (const double2*)values_h, nnz);
+    h2d(c, A.val32.p, v32.data(), nnz);
+    h2d(c, A.diag.p, diag.data(), n);
+    c->sync();
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int sucem_csr_apply(sucem_ctx* ctx, sucem_csr* A, const double* x_d, double* y_d) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(A); NOT_NULL(x_d); NOT_NULL(y_d);
+  Context* c = &ctx->c;
+  c->use();
+  csr_apply64(c, &A->a, (const double2*)x_d, (double2*)y_d);
+  API_END
+}
+
+int sucem_csr_solve(sucem_ctx* ctx, sucem_csr* A, const double* b_d, double* x_d,
+                    const sucem_solve_opts* opts, sucem_solve_stats* stats, double* res_hist_h,
+                    int64_t hist_cap) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(A); NOT_NULL(b_d); NOT_NULL(x_d);
+  Context* c = &ctx->c;
+  c->use();
+  SolveStats st;
+  solve_csr(c, &A->a, (const double2*)b_d, (double2*)x_d, convert(opts), st, res_hist_h, hist_cap);
+  convert_stats(st, stats);
+  API_END
+}
+
+int sucem_csr_destroy(sucem_csr* A) {
+  API_BEGIN
+  if (!A) return SUCEM_OK;
+  (void)hipSetDevice(A->a.ctx->device);
+  (void)hipStreamSynchronize(A->a.ctx->stream);
+  delete A;
+  API_END
+}
+
+}  // extern "C"

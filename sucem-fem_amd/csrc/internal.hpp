@@ -1,0 +1,128 @@
+// Device-resident objects behind the opaque C-ABI handles.
+//
+// HBM layout (all arrays device-resident, structure-of-arrays):
+//   mesh   : xyz double[nv*3], cells int32[nc*4] (vertex ids sorted per cell),
+//            eps_r / inv_mu_r double[nc]
+//   space  : cell_dofs int32[nc*nloc]; edges int32[nE*2]; faces int32[nF*3];
+//            CSR pattern row_ptr int32[N+1], col int32[nnz], diag int32[N];
+//            assembly gather map seg int32[nnz+1] + perm uint32[nc*nloc*nloc]
+//            (contribution id = cell*nloc^2 + i*nloc + j, sorted by (row, col));
+//            boundary faces bf_cell / bf_local int32[nbf]; pec uint8[N];
+//            compact ABC pattern s0_ptr int32[N+1] (rows without boundary DOFs
+//            are empty), s0_col int32[nnz0], s0_seg/s0_perm like above
+//   system : st64 double2[nnz] = (S, T) interleaved, st32 float2[nnz] (same
+//            values rounded), s0_64 double[nnz0], s0_32 float[nnz0]
+#pragma once
+#include "common.hpp"
+
+namespace sucem {
+
+struct Mesh {
+  Context* ctx = nullptr;
+  int64_t nv = 0, nc = 0;
+  DBuf<double> xyz;       // nv*3
+  DBuf<int32_t> cells;    // nc*4, sorted
+  DBuf<double> eps_r;     // nc
+  DBuf<double> inv_mu_r;  // nc
+};
+
+struct Space {
+  Context* ctx = nullptr;
+  Mesh* mesh = nullptr;
+  int order = 1, nloc = 6, nfd = 3;
+  int64_t nE = 0, nF = 0, nbf = 0, ndofs = 0, nnz = 0, nnz0 = 0, npec = 0;
+  DBuf<int32_t> cell_dofs;        // nc*nloc
+  DBuf<int32_t> cell_edges;       // nc*6
+  DBuf<int32_t> cell_faces;       // nc*4
+  DBuf<int32_t> edges;            // nE*2
+  DBuf<int32_t> faces;            // nF*3
+  DBuf<int32_t> bf_cell, bf_local;
+  DBuf<uint8_t> pec;              // ndofs
+  DBuf<int32_t> row_ptr, col, diag;
+  DBuf<int32_t> seg;
+  DBuf<uint32_t> perm;
+  DBuf<int32_t> s0_ptr, s0_col, s0_diag, s0_seg;
+  DBuf<uint32_t> s0_perm;
+  // reference tensors on device
+  DBuf<double> ref_mass, ref_stiff, ref_face, ref_basis;
+  DBuf<int32_t> ref_face_dofs;
+};
+
+struct System {
+  Context* ctx = nullptr;
+  Space* space = nullptr;
+  unsigned which = 0;
+  DBuf<double2> st64;
+  DBuf<float2> st32;
+  DBuf<double> s0_64;
+  DBuf<float> s0_32;
+};
+
+// Generic complex CSR matrix (user-supplied, e.g. from scipy).
+struct Csr {
+  Context* ctx = nullptr;
+  int64_t n = 0, nnz = 0;
+  DBuf<int32_t> row_ptr, col;
+  DBuf<double2> val64;
+  DBuf<float2> val32;
+  DBuf<int32_t> diag;
+};
+
+// mesh_space.hip
+void mesh_create_box(Context* c, Mesh* m, int64_t nx, int64_t ny, int64_t nz,
+                     const double scale[3], const double shift[3]);
+void mesh_sort_cells(Context* c, Mesh* m);
+int64_t mesh_set_materials(Context* c, Mesh* m, const int32_t* region_d, int64_t nreg,
+                           const int64_t* ids_h, const double* eps_h, const double* mu_h);
+void space_build(Context* c, Space* s);
+
+// assemble.hip
+void assemble_system(Context* c, Space* s, System* sys, unsigned which);
+void export_values(Context* c, const System* sys, unsigned which_one, double* out_d);
+
+// Coefficients of the operator y = s*S x + t*T x + i*s0*S0 x.  The driven
+// operator S - k0^2 T + i k0 S0 is {1, -k0^2, k0}.
+struct Coef {
+  double s = 1.0, t = 0.0, s0 = 0.0;
+};
+inline Coef driven_coef(double k0) { Coef k; k.s = 1.0; k.t = -k0 * k0; k.s0 = k0; return k; }
+
+// ops.hip
+template <typename R>
+void system_apply(Context* c, const System* sys, Coef k, bool eliminate_pec, int64_t nvec,
+                  const void* x, void* y);
+void csr_apply64(Context* c, const Csr* A, const double2* x, double2* y);
+void locate_points(Context* c, const Space* s, int64_t n, const double* pts_d, int32_t* cell_d);
+void rhs_point_sources(Context* c, const Space* s, int64_t nsrc, const double* pos_d,
+                       const double2* val_d, const int32_t* rhs_d, const int32_t* cell_d,
+                       int64_t nrhs, double2 scale, double2* b_d);
+void basis_at_points(Context* c, const Space* s, int64_t n, const double* pts_d,
+                     const int32_t* cell_d, double* vals_d);
+
+// krylov.hip
+struct SolveOpts {
+  int method = 0;        // SUCEM_METHOD_*
+  int precision = 64;    // 64 or 32 (32 = FP32 inner with FP64 reliable updates)
+  int precond = 1;       // SUCEM_PRECOND_*
+  int max_iter = 10000;
+  int check_every = 8;
+  double tol = 1e-8;
+  double ru_delta = 0.1;
+  int eliminate_pec = 0;
+};
+struct SolveStats {
+  int status = 0, iterations = 0, reliable_updates = 0;
+  double rel_residual = 0, elapsed_ms = 0;
+  int64_t spmv_lo = 0, spmv_hi = 0;
+};
+void solve_system(Context* c, const System* sys, Coef k, const double2* b, double2* x,
+                  const SolveOpts& o, SolveStats& st, double* hist_h, int64_t hist_cap);
+void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const SolveOpts& o,
+               SolveStats& st, double* hist_h, int64_t hist_cap);
+
+}  // namespace sucem
+
+struct sucem_mesh { sucem::Mesh m; };
+struct sucem_space { sucem::Space s; };
+struct sucem_system { sucem::System s; };
+struct sucem_csr { sucem::Csr a; };

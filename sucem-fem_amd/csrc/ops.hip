@@ -1,0 +1,477 @@
+// Sparse operators, point location and point-source right-hand sides.
+//
+// The driven operator is never materialised: y = (S - k0^2 T) x + i k0 S0 x
+// streams the interleaved real (S, T) values of the shared pattern once and adds
+// the compact ABC rows, replacing the three CSR conversions and two sparse adds
+// of DrivenProblemABC.get_LHS_matrix (ProblemConfigurations/EMDrivenProblem.py:67-72).
+#include "internal.hpp"
+#include "kernels.hpp"
+#include "ops.hpp"
+
+namespace sucem {
+namespace {
+
+constexpr int TB = 256;
+
+template <typename R> struct ST2;
+template <> struct ST2<float> { using T = float2; };
+template <> struct ST2<double> { using T = double2; };
+
+// One group of L lanes per row; each XCD gets a contiguous block of rows.
+// Epilogue (optional): part[lb] = sum_r op(w_r) * y_r (op = conj if CONJ), and
+// part[G + lb] = sum_r |y_r|^2.
+template <typename R, int L, bool DOT, bool CONJ>
+__global__ __launch_bounds__(TB) void k_sys_spmv(
+    int n, int G, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+    const typename ST2<R>::T* __restrict__ st, const int32_t* __restrict__ s0p,
+    const int32_t* __restrict__ s0c, const R* __restrict__ s0v, R cs, R ct, R c0,
+    const uint8_t* __restrict__ mask, const typename C2<R>::T* __restrict__ x,
+    typename C2<R>::T* __restrict__ y, const typename C2<R>::T* __restrict__ w, double2* part) {
+  using C = typename C2<R>::T;
+  constexpr int RPB = TB / L;
+  const int lb = xcd_block(G);
+  int chunk = (n + G - 1) / G;
+  chunk = (chunk + RPB - 1) / RPB * RPB;
+  const int start = lb * chunk;
+  const int end = min(n, start + chunk);
+  const int lane = threadIdx.x % L;
+  double2 d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0);
+  for (int base = start; base < end; base += RPB) {
+    const int r = base + threadIdx.x / L;
+    C as = mk(R(0), R(0)), at = mk(R(0), R(0)), a0 = mk(R(0), R(0));
+    if (r < end) {
+      const int j1 = rp[r + 1];
+      for (int j = rp[r] + lane; j < j1; j += L) {
+        const auto v = st[j];
+        const C xc = x[col[j]];
+        as.x += v.x * xc.x; as.y += v.x * xc.y;
+        at.x += v.y * xc.x; at.y += v.y * xc.y;
+      }
+      if (s0p) {
+        const int k1 = s0p[r + 1];
+        for (int j = s0p[r] + lane; j < k1; j += L) {
+          const R v = s0v[j];
+          const C xc = x[s0c[j]];
+          a0.x += v * xc.x; a0.y += v * xc.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) {
+      as.x += __shfl_xor(as.x, o); as.y += __shfl_xor(as.y, o);
+      at.x += __shfl_xor(at.x, o); at.y += __shfl_xor(at.y, o);
+      a0.x += __shfl_xor(a0.x, o); a0.y += __shfl_xor(a0.y, o);
+    }
+    if (lane == 0 && r < end) {
+      C yr = mk(cs * as.x + ct * at.x - c0 * a0.y, cs * as.y + ct * at.y + c0 * a0.x);
+      if (mask && mask[r]) yr = mk(R(0), R(0));
+      y[r] = yr;
+      if (DOT) {
+        const double2 yd = to_d(yr);
+        double2 wd = to_d(w[r]);
+        if (CONJ) wd.y = -wd.y;
+        d1.x += wd.x * yd.x - wd.y * yd.y;
+        d1.y += wd.x * yd.y + wd.y * yd.x;
+        d2.x += yd.x * yd.x + yd.y * yd.y;
+      }
+    }
+  }
+  if (DOT) {
+    d1 = block_sum<TB>(d1);
+    d2 = block_sum<TB>(d2);
+    if (threadIdx.x == 0) {
+      part[lb] = d1;
+      part[G + lb] = d2;
+    }
+  }
+}
+
+// Generic complex CSR: y = A x with the same epilogue.
+template <typename R, int L, bool DOT, bool CONJ>
+__global__ __launch_bounds__(TB) void k_csr_spmv(int n, int G, const int32_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ col,
+                                                 const typename C2<R>::T* __restrict__ val,
+                                                 const uint8_t* __restrict__ mask,
+                                                 const typename C2<R>::T* __restrict__ x,
+                                                 typename C2<R>::T* __restrict__ y,
+                                                 const typename C2<R>::T* __restrict__ w,
+                                                 double2* part) {
+  using C = typename C2<R>::T;
+  constexpr int RPB = TB / L;
+  const int lb = xcd_block(G);
+  int chunk = (n + G - 1) / G;
+  chunk = (chunk + RPB - 1) / RPB * RPB;
+  const int start = lb * chunk;
+  const int end = min(n, start + chunk);
+  const int lane = threadIdx.x % L;
+  double2 d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0);
+  for (int base = start; base < end; base += RPB) {
+    const int r = base + threadIdx.x / L;
+    C acc = mk(R(0), R(0));
+    if (r < end) {
+      const int j1 = rp[r + 1];
+      for (int j = rp[r] + lane; j < j1; j += L) {
+        const C a = val[j], xc = x[col[j]];
+        acc.x += a.x * xc.x - a.y * xc.y;
+        acc.y += a.x * xc.y + a.y * xc.x;
+      }
+    }
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) {
+      acc.x += __shfl_xor(acc.x, o);
+      acc.y += __shfl_xor(acc.y, o);
+    }
+    if (lane == 0 && r < end) {
+      if (mask && mask[r]) acc = mk(R(0), R(0));
+      y[r] = acc;
+      if (DOT) {
+        const double2 yd = to_d(acc);
+        double2 wd = to_d(w[r]);
+        if (CONJ) wd.y = -wd.y;
+        d1.x += wd.x * yd.x - wd.y * yd.y;
+        d1.y += wd.x * yd.y + wd.y * yd.x;
+        d2.x += yd.x * yd.x + yd.y * yd.y;
+      }
+    }
+  }
+  if (DOT) {
+    d1 = block_sum<TB>(d1);
+    d2 = block_sum<TB>(d2);
+    if (threadIdx.x == 0) {
+      part[lb] = d1;
+      part[G + lb] = d2;
+    }
+  }
+}
+
+template <typename R>
+__global__ void k_sys_dinv(int64_t n, const int32_t* diag, const typename ST2<R>::T* st,
+                           const int32_t* s0diag, const R* s0v, double cs, double ct,
+                           double c0, const uint8_t* mask, typename C2<R>::T* dinv) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const auto v = st[diag[r]];
+    double re = cs * (double)v.x + ct * (double)v.y;
+    double im = (s0diag && s0diag[r] >= 0) ? c0 * (double)s0v[s0diag[r]] : 0.0;
+    double den = re * re + im * im;
+    bool off = (mask && mask[r]) || den == 0.0;
+    dinv[r] = off ? mk(R(0), R(0)) : mk(R(re / den), R(-im / den));
+  }
+}
+
+template <typename R>
+__global__ void k_csr_dinv(int64_t n, const int32_t* diag, const typename C2<R>::T* val,
+                           const uint8_t* mask, typename C2<R>::T* dinv) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    double re = 0, im = 0;
+    if (diag[r] >= 0) { re = val[diag[r]].x; im = val[diag[r]].y; }
+    double den = re * re + im * im;
+    bool off = (mask && mask[r]) || den == 0.0;
+    dinv[r] = off ? mk(R(0), R(0)) : mk(R(re / den), R(-im / den));
+  }
+}
+
+template <typename R>
+__global__ void k_csr_identity_dinv(int64_t n, const uint8_t* mask, typename C2<R>::T* dinv) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x)
+    dinv[r] = (mask && mask[r]) ? mk(R(0), R(0)) : mk(R(1), R(0));
+}
+
+__global__ void k_locate(int64_t nc, const double* xyz, const int32_t* cells, int64_t n,
+                         const double* pts, double tol, int32_t* out) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t* cell = cells + 4 * c;
+    const double* x0 = xyz + 3 * (int64_t)cell[0];
+    double J[3][3];
+    for (int k = 0; k < 3; ++k) {
+      const double* xk = xyz + 3 * (int64_t)cell[k + 1];
+      for (int r = 0; r < 3; ++r) J[r][k] = xk[r] - x0[r];
+    }
+    double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    double id = 1.0 / det;
+    double Ji[3][3] = {{c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id,
+                        (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+                       {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id,
+                        (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+                       {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id,
+                        (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+    for (int64_t p = 0; p < n; ++p) {
+      double d[3] = {pts[3 * p] - x0[0], pts[3 * p + 1] - x0[1], pts[3 * p + 2] - x0[2]};
+      double l1 = Ji[0][0] * d[0] + Ji[0][1] * d[1] + Ji[0][2] * d[2];
+      double l2 = Ji[1][0] * d[0] + Ji[1][1] * d[1] + Ji[1][2] * d[2];
+      double l3 = Ji[2][0] * d[0] + Ji[2][1] * d[1] + Ji[2][2] * d[2];
+      double l0 = 1.0 - l1 - l2 - l3;
+      if (l0 >= -tol && l1 >= -tol && l2 >= -tol && l3 >= -tol) atomicMin(&out[p], (int32_t)c);
+    }
+  }
+}
+
+// Physical basis values N_i(x) = J^-T N^_i(x^) of the cell containing x.
+__device__ void basis_values(const double* xyz, const int32_t* cell, const double* ref_basis,
+                             int nloc, const double* p, double out[20][3]) {
+  const double* x0 = xyz + 3 * (int64_t)cell[0];
+  double J[3][3];
+  for (int k = 0; k < 3; ++k) {
+    const double* xk = xyz + 3 * (int64_t)cell[k + 1];
+    for (int r = 0; r < 3; ++r) J[r][k] = xk[r] - x0[r];
+  }
+  double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  double id = 1.0 / det;
+  double Ji[3][3] = {{c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id,
+                      (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+                     {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id,
+                      (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+                     {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id,
+                      (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+  double d[3] = {p[0] - x0[0], p[1] - x0[1], p[2] - x0[2]};
+  double xh[3];
+  for (int a = 0; a < 3; ++a) xh[a] = Ji[a][0] * d[0] + Ji[a][1] * d[1] + Ji[a][2] * d[2];
+  const double mono[10] = {1.0,           xh[0],         xh[1],         xh[2],
+                           xh[0] * xh[0], xh[0] * xh[1], xh[0] * xh[2], xh[1] * xh[1],
+                           xh[1] * xh[2], xh[2] * xh[2]};
+  for (int i = 0; i < nloc; ++i) {
+    double nh[3];
+    for (int dd = 0; dd < 3; ++dd) {
+      double s = 0;
+      for (int m = 0; m < 10; ++m) s += ref_basis[(i * 3 + dd) * 10 + m] * mono[m];
+      nh[dd] = s;
+    }
+    for (int a = 0; a < 3; ++a) out[i][a] = Ji[0][a] * nh[0] + Ji[1][a] * nh[1] + Ji[2][a] * nh[2];
+  }
+}
+
+__global__ void k_basis_at(int64_t n, const double* pts, const int32_t* cellid, const double* xyz,
+                           const int32_t* cells, const double* ref_basis, int nloc, double* vals) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    double v[20][3];
+    basis_values(xyz, cells + 4 * (int64_t)cellid[p], ref_basis, nloc, pts + 3 * p, v);
+    for (int i = 0; i < nloc; ++i)
+      for (int a = 0; a < 3; ++a) vals[(p * nloc + i) * 3 + a] = v[i][a];
+  }
+}
+
+// One wavefront per right-hand side; its sources are added in input order
+// (b[dof] += scale * N_dof(x_s) . value_s), so accumulation is deterministic.
+__global__ __launch_bounds__(64) void k_rhs(int64_t nsrc, const double* pos, const double2* val,
+                                            const int32_t* rhs, const int32_t* cellid,
+                                            int64_t nrhs, double2 scale, const double* xyz,
+                                            const int32_t* cells, const int32_t* cell_dofs,
+                                            const double* ref_basis, int nloc, int64_t ndofs,
+                                            double2* b) {
+  for (int64_t r = blockIdx.x; r < nrhs; r += gridDim.x) {
+    for (int64_t s = 0; s < nsrc; ++s) {
+      if (rhs[s] != r) continue;
+      const int32_t c = cellid[s];
+      if (threadIdx.x < nloc) {
+        double v[20][3];
+        basis_values(xyz, cells + 4 * (int64_t)c, ref_basis, nloc, pos + 3 * s, v);
+        const int i = threadIdx.x;
+        double2 acc = make_double2(0, 0);
+        for (int a = 0; a < 3; ++a) {
+          acc.x += v[i][a] * val[3 * s + a].x;
+          acc.y += v[i][a] * val[3 * s + a].y;
+        }
+        const int64_t dof = cell_dofs[(int64_t)c * nloc + i];
+        double2 add = make_double2(scale.x * acc.x - scale.y * acc.y,
+                                   scale.x * acc.y + scale.y * acc.x);
+        double2 old = b[r * ndofs + dof];
+        b[r * ndofs + dof] = make_double2(old.x + add.x, old.y + add.y);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <typename R, int L>
+void launch_sys(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
+                void* y, const void* w, bool conj, double2* part, int G) {
+  const Space* s = sys->space;
+  using C = typename C2<R>::T;
+  const typename ST2<R>::T* st;
+  const R* s0v;
+  if constexpr (sizeof(R) == 4) { st = sys->st32.p; s0v = sys->s0_32.p; }
+  else { st = sys->st64.p; s0v = sys->s0_64.p; }
+  const bool has_s0 = (sys->which & SUCEM_MAT_S0) && s->nnz0 > 0;
+  const int32_t* s0p = has_s0 ? s->s0_ptr.p : nullptr;
+  R cs = (R)k.s, ct = (R)k.t, c0 = (R)k.s0;
+  int n = (int)s->ndofs;
+  if (!part) {
+    SUCEM_LAUNCH((k_sys_spmv<R, L, false, false>), G, TB, 0, c->stream, n, G, s->row_ptr.p,
+                 s->col.p, st, s0p, s->s0_col.p, s0v, cs, ct, c0, mask, (const C*)x, (C*)y,
+                 (const C*)w, part);
+  } else if (conj) {
+    SUCEM_LAUNCH((k_sys_spmv<R, L, true, true>), G, TB, 0, c->stream, n, G, s->row_ptr.p,
+                 s->col.p, st, s0p, s->s0_col.p, s0v, cs, ct, c0, mask, (const C*)x, (C*)y,
+                 (const C*)w, part);
+  } else {
+    SUCEM_LAUNCH((k_sys_spmv<R, L, true, false>), G, TB, 0, c->stream, n, G, s->row_ptr.p,
+                 s->col.p, st, s0p, s->s0_col.p, s0v, cs, ct, c0, mask, (const C*)x, (C*)y,
+                 (const C*)w, part);
+  }
+}
+
+template <typename R, int L>
+void launch_csr(Context* c, const Csr* A, const uint8_t* mask, const void* x, void* y,
+                const void* w, bool conj, double2* part, int G) {
+  using C = typename C2<R>::T;
+  const C* val;
+  if constexpr (sizeof(R) == 4) val = A->val32.p; else val = A->val64.p;
+  int n = (int)A->n;
+  if (!part) {
+    SUCEM_LAUNCH((k_csr_spmv<R, L, false, false>), G, TB, 0, c->stream, n, G, A->row_ptr.p,
+                 A->col.p, val, mask, (const C*)x, (C*)y, (const C*)w, part);
+  } else if (conj) {
+    SUCEM_LAUNCH((k_csr_spmv<R, L, true, true>), G, TB, 0, c->stream, n, G, A->row_ptr.p,
+                 A->col.p, val, mask, (const C*)x, (C*)y, (const C*)w, part);
+  } else {
+    SUCEM_LAUNCH((k_csr_spmv<R, L, true, false>), G, TB, 0, c->stream, n, G, A->row_ptr.p,
+                 A->col.p, val, mask, (const C*)x, (C*)y, (const C*)w, part);
+  }
+}
+
+}  // namespace
+
+int lanes_for(int64_t n, int64_t nnz) {
+  double avg = n ? (double)nnz / (double)n : 1.0;
+  if (avg <= 6) return 4;
+  if (avg <= 20) return 8;
+  if (avg <= 48) return 16;
+  return 32;
+}
+
+int op_grid(Context* c, int64_t n) {
+  // a multiple of 8 (one share per XCD); enough blocks to fill every CU
+  int64_t want = (int64_t)c->num_cus * 4;
+  int64_t rows_min = 256;  // at least a few rows per block
+  int64_t g = (n + rows_min - 1) / rows_min;
+  if (g > want) g = want;
+  g = (g + 7) / 8 * 8;
+  if (g < 8) g = 8;
+  return (int)g;
+}
+
+template <typename R>
+void sys_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
+               void* y, const void* w, bool conj, double2* part, int G, int L) {
+  switch (L) {
+    case 4: launch_sys<R, 4>(c, sys, k, mask, x, y, w, conj, part, G); break;
+    case 8: launch_sys<R, 8>(c, sys, k, mask, x, y, w, conj, part, G); break;
+    case 16: launch_sys<R, 16>(c, sys, k, mask, x, y, w, conj, part, G); break;
+    case 32: launch_sys<R, 32>(c, sys, k, mask, x, y, w, conj, part, G); break;
+    default: launch_sys<R, 64>(c, sys, k, mask, x, y, w, conj, part, G); break;
+  }
+}
+template void sys_apply<float>(Context*, const System*, Coef, const uint8_t*, const void*, void*,
+                               const void*, bool, double2*, int, int);
+template void sys_apply<double>(Context*, const System*, Coef, const uint8_t*, const void*,
+                                void*, const void*, bool, double2*, int, int);
+
+template <typename R>
+void csr_apply(Context* c, const Csr* A, const uint8_t* mask, const void* x, void* y,
+               const void* w, bool conj, double2* part, int G, int L) {
+  switch (L) {
+    case 4: launch_csr<R, 4>(c, A, mask, x, y, w, conj, part, G); break;
+    case 8: launch_csr<R, 8>(c, A, mask, x, y, w, conj, part, G); break;
+    case 16: launch_csr<R, 16>(c, A, mask, x, y, w, conj, part, G); break;
+    case 32: launch_csr<R, 32>(c, A, mask, x, y, w, conj, part, G); break;
+    default: launch_csr<R, 64>(c, A, mask, x, y, w, conj, part, G); break;
+  }
+}
+template void csr_apply<float>(Context*, const Csr*, const uint8_t*, const void*, void*,
+                               const void*, bool, double2*, int, int);
+template void csr_apply<double>(Context*, const Csr*, const uint8_t*, const void*, void*,
+                                const void*, bool, double2*, int, int);
+
+template <typename R>
+void sys_dinv(Context* c, const System* sys, Coef k, const uint8_t* mask, void* dinv) {
+  const Space* s = sys->space;
+  const bool has_s0 = (sys->which & SUCEM_MAT_S0) && s->nnz0 > 0;
+  if constexpr (sizeof(R) == 4)
+    SUCEM_LAUNCH((k_sys_dinv<float>), grid_for(s->ndofs, TB), TB, 0, c->stream, s->ndofs,
+                 s->diag.p, sys->st32.p, has_s0 ? s->s0_diag.p : nullptr, sys->s0_32.p, k.s, k.t,
+                 k.s0, mask, (float2*)dinv);
+  else
+    SUCEM_LAUNCH((k_sys_dinv<double>), grid_for(s->ndofs, TB), TB, 0, c->stream, s->ndofs,
+                 s->diag.p, sys->st64.p, has_s0 ? s->s0_diag.p : nullptr, sys->s0_64.p, k.s, k.t,
+                 k.s0, mask, (double2*)dinv);
+}
+template void sys_dinv<float>(Context*, const System*, Coef, const uint8_t*, void*);
+template void sys_dinv<double>(Context*, const System*, Coef, const uint8_t*, void*);
+
+template <typename R>
+void csr_dinv(Context* c, const Csr* A, const uint8_t* mask, bool identity, void* dinv, int64_t n) {
+  using C = typename C2<R>::T;
+  if (identity) {
+    SUCEM_LAUNCH((k_csr_identity_dinv<R>), grid_for(n, TB), TB, 0, c->stream, n, mask, (C*)dinv);
+    return;
+  }
+  const C* val;
+  if constexpr (sizeof(R) == 4) val = A->val32.p; else val = A->val64.p;
+  SUCEM_LAUNCH((k_csr_dinv<R>), grid_for(A->n, TB), TB, 0, c->stream, A->n, A->diag.p, val, mask,
+               (C*)dinv);
+}
+template void csr_dinv<float>(Context*, const Csr*, const uint8_t*, bool, void*, int64_t);
+template void csr_dinv<double>(Context*, const Csr*, const uint8_t*, bool, void*, int64_t);
+
+template <typename R>
+void system_apply(Context* c, const System* sys, Coef k, bool eliminate_pec, int64_t nvec,
+                  const void* x, void* y) {
+  const Space* s = sys->space;
+  SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S and T must be assembled");
+  const int G = op_grid(c, s->ndofs);
+  const int L = lanes_for(s->ndofs, s->nnz);
+  const size_t bytes = sizeof(typename C2<R>::T);
+  for (int64_t v = 0; v < nvec; ++v)
+    sys_apply<R>(c, sys, k, eliminate_pec ? s->pec.p : nullptr,
+                 (const char*)x + v * s->ndofs * bytes, (char*)y + v * s->ndofs * bytes, nullptr,
+                 false, nullptr, G, L);
+}
+template void system_apply<float>(Context*, const System*, Coef, bool, int64_t, const void*,
+                                  void*);
+template void system_apply<double>(Context*, const System*, Coef, bool, int64_t, const void*,
+                                   void*);
+
+void csr_apply64(Context* c, const Csr* A, const double2* x, double2* y) {
+  csr_apply<double>(c, A, nullptr, x, y, nullptr, false, nullptr, op_grid(c, A->n),
+                    lanes_for(A->n, A->nnz));
+}
+
+void locate_points(Context* c, const Space* s, int64_t n, const double* pts_d, int32_t* cell_d) {
+  const Mesh* m = s->mesh;
+  // initialise to "not found"
+  std::vector<int32_t> init(n, 0x7fffffff);
+  h2d(c, cell_d, init.data(), n);
+  SUCEM_LAUNCH(k_locate, grid_for(m->nc, TB), TB, 0, c->stream, m->nc, m->xyz.p, m->cells.p, n,
+               pts_d, 1e-12, cell_d);
+  c->sync();
+}
+
+void rhs_point_sources(Context* c, const Space* s, int64_t nsrc, const double* pos_d,
+                       const double2* val_d, const int32_t* rhs_d, const int32_t* cell_d,
+                       int64_t nrhs, double2 scale, double2* b_d) {
+  const Mesh* m = s->mesh;
+  unsigned g = (unsigned)(nrhs < 65535 ? nrhs : 65535);
+  if (g == 0 || nsrc == 0) return;
+  SUCEM_LAUNCH(k_rhs, g, 64, 0, c->stream, nsrc, pos_d, val_d, rhs_d, cell_d, nrhs, scale,
+               m->xyz.p, m->cells.p, s->cell_dofs.p, s->ref_basis.p, s->nloc, s->ndofs, b_d);
+}
+
+void basis_at_points(Context* c, const Space* s, int64_t n, const double* pts_d,
+                     const int32_t* cell_d, double* vals_d) {
+  const Mesh* m = s->mesh;
+  if (n == 0) return;
+  SUCEM_LAUNCH(k_basis_at, grid_for(n, 64), 64, 0, c->stream, n, pts_d, cell_d, m->xyz.p,
+               m->cells.p, s->ref_basis.p, s->nloc, vals_d);
+}
+
+}  // namespace sucem

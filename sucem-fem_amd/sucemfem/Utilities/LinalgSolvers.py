@@ -1,0 +1,210 @@
+"""Sparse linear solvers on the device (sucemfem/Utilities/LinalgSolvers.py:30-322).
+
+Same classes, logging data and call pattern as the reference.  The Krylov
+iterations run in libsucem_hip: a matrix produced by
+``DrivenProblemABC.get_LHS_matrix`` is solved with the fused S/T/S0 operator,
+any other matrix is uploaded as a complex CSR.  Preconditioners: ``None``,
+``'diagonal'`` (Jacobi) and ``'ilu'``; the device has no incomplete-LU
+factorisation, so ``'ilu'`` warns and uses Jacobi.  Tolerances default to
+SciPy's (relative residual 1e-5, at most 10*N iterations) as in the reference,
+which passes none.
+"""
+import ctypes
+import time
+import warnings
+
+import numpy as np
+import scipy.sparse
+
+from .. import _hip
+from ..runtime import Handle, get_context
+
+
+def _device_opts(method, precond, tol, maxiter, precision, check_every):
+    o = _hip.SolveOpts()
+    _hip.load().sucem_solve_opts_default(ctypes.byref(o))
+    o.method = method
+    o.precond = precond
+    o.tol = float(tol)
+    o.max_iter = int(maxiter)
+    o.precision = int(precision)
+    o.check_every = int(check_every)
+    return o
+
+
+class DeviceCSR(object):
+    """A complex CSR matrix uploaded to the device."""
+
+    def __init__(self, A, ctx=None):
+        self.ctx = ctx or get_context()
+        A = scipy.sparse.csr_matrix(A, dtype=np.complex128)
+        A.sort_indices()
+        self.shape = A.shape
+        indptr = np.ascontiguousarray(A.indptr, np.int64)
+        indices = np.ascontiguousarray(A.indices, np.int32)
+        data = np.ascontiguousarray(A.data, np.complex128)
+        self.handle = Handle(self.ctx, "sucem_csr_destroy")
+        _hip.call("sucem_csr_create", self.ctx.handle, A.shape[0], A.nnz, _hip.ptr(indptr),
+                  _hip.ptr(indices), _hip.ptr(data), ctypes.byref(self.handle.h))
+
+    def apply(self, x_dev, y_dev):
+        _hip.call("sucem_csr_apply", self.ctx.handle, self.handle.h, _hip.ptr(x_dev), _hip.ptr(y_dev))
+
+
+class SystemSolverBase(object):
+    """Logging base class; subclasses choose the Krylov method."""
+    method = _hip.METHOD_BICGSTAB
+    default_tol = 1e-5
+
+    def __init__(self, A, preconditioner_type=None, tol=None, maxiter=None, precision=64,
+                 check_every=8):
+        self._logging_data = {'time': [], 'id': [], 'res': []}
+        self._user_callbacks = []
+        self._timestamp('init')
+        self._A = A
+        self.tol = self.default_tol if tol is None else tol
+        self.maxiter = maxiter
+        self.precision = precision
+        self.check_every = check_every
+        self.stats = None
+        self.set_preconditioner(preconditioner_type)
+        self._callback_count = 0
+
+    def add_user_callback(self, cb):
+        self._user_callbacks.append(cb)
+
+    def _timestamp(self, id, res=np.nan):
+        self._logging_data['time'].append(time.time())
+        self._logging_data['id'].append(id)
+        self._logging_data['res'].append(res)
+        for cb in self._user_callbacks:
+            cb(self, res)
+
+    def set_b(self, b):
+        self._b = b
+
+    def set_preconditioner(self, M_type):
+        self._timestamp('preconditioner::start')
+        if M_type is None:
+            self._precond = _hip.PRECOND_NONE
+        elif M_type.lower() == 'diagonal':
+            self._precond = _hip.PRECOND_JACOBI
+        elif M_type.lower() == 'ilu':
+            warnings.warn("incomplete LU is not available on the device; using Jacobi ('diagonal')")
+            self._precond = _hip.PRECOND_JACOBI
+        else:
+            warnings.warn("Preconditioner type '%s' not recognised. Not using preconditioner." % M_type)
+            self._precond = _hip.PRECOND_NONE
+        self._M_type = M_type
+        self._timestamp('preconditioner::end')
+
+    def _call_solver(self):
+        A, b = self._A, np.asarray(self._b)
+        shape = b.shape
+        b = np.ascontiguousarray(b.reshape(-1), np.complex128)
+        n = A.shape[0]
+        maxiter = self.maxiter or 10 * n
+        opts = _device_opts(self.method, self._precond, self.tol, maxiter, self.precision,
+                            self.check_every)
+        stats = _hip.SolveStats()
+        hist = np.zeros(maxiter, np.float64)
+        op = getattr(A, "device_operator", None)
+        if op is not None:
+            system, k0, _ = op
+            ctx = system.ctx
+            b_d = ctx.to_device(b)
+            x_d = ctx.tensor(n)
+            _hip.call("sucem_solve_driven", ctx.handle, system.handle.h, float(k0), _hip.ptr(b_d),
+                      _hip.ptr(x_d), ctypes.byref(opts), ctypes.byref(stats), _hip.ptr(hist),
+                      maxiter)
+        else:
+            dev = DeviceCSR(A)
+            ctx = dev.ctx
+            b_d = ctx.to_device(b)
+            x_d = ctx.tensor(n)
+            _hip.call("sucem_csr_solve", ctx.handle, dev.handle.h, _hip.ptr(b_d), _hip.ptr(x_d),
+                      ctypes.byref(opts), ctypes.byref(stats), _hip.ptr(hist), maxiter)
+        self.stats = stats.as_dict()
+        nb = np.linalg.norm(b)
+        for k in range(stats.iterations):
+            self._callback_count += 1
+            self._timestamp(self._callback_count, res=hist[k] * nb)
+        x = x_d.cpu().numpy()
+        if np.isrealobj(self._b) and np.isrealobj(A.data if hasattr(A, "data") else 0):
+            x = x.real.copy()
+        info = 0 if stats.status == 0 else (stats.iterations if stats.status > 0 else -1)
+        return x.reshape(shape) if len(shape) == 1 else x, info
+
+    def solve(self, b):
+        """Solve A x = b; non-convergence is reported through ``self.stats``, as the
+        reference ignores SciPy's info flag (LinalgSolvers.py:142-145)."""
+        self._timestamp('solve::begin')
+        self.set_b(b)
+        x, info = self._call_solver()
+        self.info = info
+        self._timestamp('solve::end')
+        return x
+
+    def get_logging_data(self):
+        return self._logging_data
+
+    def _get_time_for_id(self, id):
+        return self._logging_data['time'][self._logging_data['id'].index(id)]
+
+    def _get_elapsed_time(self, id0, id1, force_total=False):
+        if force_total:
+            return self._logging_data['time'][-1] - self._logging_data['time'][0]
+        return self._get_time_for_id(id1) - self._get_time_for_id(id0)
+
+    def print_logging_data(self):
+        for k in self._logging_data:
+            print(k)
+            print(self._logging_data[k])
+
+    def print_timing_info(self):
+        print('solve time:', self._get_elapsed_time('solve::begin', 'solve::end'))
+        print('preconditioner time:',
+              self._get_elapsed_time('preconditioner::start', 'preconditioner::end'))
+        print('total time:', self._get_elapsed_time(None, None, True))
+
+
+class BiCGStabSolver(SystemSolverBase):
+    """Preconditioned BiCGStab (LinalgSolvers.py:239-248)."""
+    method = _hip.METHOD_BICGSTAB
+
+
+class COCGSolver(SystemSolverBase):
+    """Conjugate-orthogonal CG for complex-symmetric A (one SpMV per iteration)."""
+    method = _hip.METHOD_COCG
+
+
+class GMRESSolver(SystemSolverBase):
+    """GMRES (LinalgSolvers.py:251-260) has no device implementation yet."""
+
+    def _call_solver(self):
+        raise NotImplementedError("GMRES is not implemented on the device; use BiCGStabSolver "
+                                  "or COCGSolver")
+
+
+class UMFPACKSolver(SystemSolverBase):
+    """Stand-in for the direct UMFPACK solve (LinalgSolvers.py:262-277).
+
+    There is no sparse direct factorisation on the device; the system is solved
+    by BiCGStab to a relative residual of 1e-12 instead.
+    """
+    method = _hip.METHOD_BICGSTAB
+    default_tol = 1e-12
+
+    def __init__(self, A, preconditioner_type='diagonal', **kw):
+        SystemSolverBase.__init__(self, A, preconditioner_type, **kw)
+
+
+def calculate_residual(A, x, b):
+    """||A x - b||_2 with b reshaped to x (LinalgSolvers.py:297-305)."""
+    return np.linalg.norm(A * x - np.asarray(b).reshape(np.shape(x)))
+
+
+def solve_sparse_system(A, b, preconditioner_type='ilu', **kw):
+    """BiCGStab solve of A x = b (LinalgSolvers.py:308-322)."""
+    solver = BiCGStabSolver(A, preconditioner_type, **kw)
+    return solver.solve(b)

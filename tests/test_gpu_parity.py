@@ -1,0 +1,287 @@
+"""GPU parity: libsucem_hip (through the C-ABI) against the oracle and the reference fixtures."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    torch.cuda.set_device(0)
+    from sucemfem.runtime import get_context
+    return get_context(0)
+
+
+def device_space(coords, cells, order, eps=None, mu=None):
+    from sucemfem.Mesh import Mesh, FunctionSpace
+    from sucemfem import _hip
+    mesh = Mesh(coords, cells)
+    V = FunctionSpace(mesh, "Nedelec 1st kind H(curl)", order)
+    nc = len(cells)
+    if eps is None and mu is None:
+        ids, e, m, region = np.array([0]), np.array([1.0]), np.array([1.0]), None
+    else:
+        eps = np.ones(nc) if eps is None else np.asarray(eps, float)
+        mu = np.ones(nc) if mu is None else np.asarray(mu, float)
+        pairs = np.stack([eps, mu], 1)
+        uniq, region = np.unique(pairs, axis=0, return_inverse=True)
+        ids = np.arange(len(uniq))
+        e, m = uniq[:, 0].copy(), uniq[:, 1].copy()
+        region = np.ascontiguousarray(region.ravel(), np.int64)
+    mh = mesh.device_handle(V.ctx)
+    _hip.call("sucem_mesh_set_materials", mh.h, _hip.ptr(region), len(ids),
+              _hip.ptr(np.ascontiguousarray(ids, np.int64)), _hip.ptr(np.ascontiguousarray(e)),
+              _hip.ptr(np.ascontiguousarray(m)))
+    return mesh, V
+
+
+def assemble_device(V, which=7):
+    from sucemfem.SystemMatrices import AssembledSystem
+    return AssembledSystem(V, which)
+
+
+def device_csr(V, sysm, name):
+    indptr, indices = V.pattern()
+    n = V.dim()
+    return scipy.sparse.csr_matrix((sysm.values(name), indices, indptr), shape=(n, n))
+
+
+MESHES = {
+    "inscribed": lambda: O.inscribed_tet(),
+    "cube3": lambda: O.unit_cube(3, 3, 3),
+    "box423": lambda: O.box_mesh([1.0, 0.5, 0.75], (4, 2, 3)),
+    "centred5": lambda: O.centred_cube([0.15] * 3, 0.05, [0, 0, 0.]),
+}
+
+
+@pytest.mark.parametrize("mesh", list(MESHES))
+@pytest.mark.parametrize("order", [1, 2])
+def test_numbering_dofmap_pattern_bit_exact(dev, mesh, order):
+    _say("numbering %s order %d" % (mesh, order))
+    co, ce = MESHES[mesh]()
+    osp = O.Space(co, ce, order)
+    _, V = device_space(co, ce, order)
+    info = V.info()
+    assert info[0] == osp.ndofs
+    assert info[2] == len(osp.topo["edges"]) and info[3] == len(osp.topo["faces"])
+    assert info[4] == len(osp.boundary_faces)
+    edges, faces = V.entities()
+    assert np.array_equal(edges, osp.topo["edges"])
+    assert np.array_equal(faces, osp.topo["faces"])
+    assert np.array_equal(V.cell_dofs(), osp.cell_dofs)
+    indptr, indices = osp.pattern()
+    d_indptr, d_indices = V.pattern()
+    assert np.array_equal(d_indptr, indptr) and np.array_equal(d_indices, indices)
+    assert np.array_equal(V.pec_mask(), osp.pec_mask())
+
+
+@pytest.mark.parametrize("mesh", list(MESHES))
+@pytest.mark.parametrize("order", [1, 2])
+def test_assembly_matches_oracle(dev, mesh, order):
+    _say("assembly %s order %d" % (mesh, order))
+    co, ce = MESHES[mesh]()
+    rng = np.random.default_rng(7)
+    eps = rng.choice([1.0, 2.2, 4.4], size=len(ce))
+    mu = rng.choice([1.0, 2.0], size=len(ce))
+    osp = O.Space(co, ce, order)
+    om = O.assemble_fast(osp, eps_r=eps, inv_mu_r=1.0 / mu)
+    _, V = device_space(co, ce, order, eps, mu)
+    sysm = assemble_device(V)
+    for name, oname in (("S", "S"), ("T", "M"), ("S0", "S_0")):
+        A = device_csr(V, sysm, name)
+        B = om[oname]
+        err = abs(A - B).max()
+        scale = abs(B).max()
+        assert err <= 1e-13 * scale, (name, err, scale)
+
+
+def test_assembly_is_deterministic(dev):
+    co, ce = O.unit_cube(4, 4, 4)
+    _, V = device_space(co, ce, 2)
+    a = assemble_device(V).values("S")
+    b = assemble_device(V).values("S")
+    assert np.array_equal(a, b)
+
+
+def _driven_problem(mesh, order, freq, sources, materials=None, region=None):
+    from sucemfem.Sources import current_source
+    from sucemfem.BoundaryConditions import ABCBoundaryCondition, BoundaryConditions
+    from sucemfem.ProblemConfigurations.EMDrivenProblem import DrivenProblemABC
+    abc = ABCBoundaryCondition()
+    abc.set_region_number(1)
+    bcs = BoundaryConditions()
+    bcs.add_boundary_condition(abc)
+    cs = current_source.CurrentSources()
+    for s in sources:
+        cs.add_source(s)
+    dp = DrivenProblemABC()
+    dp.set_mesh(mesh)
+    dp.set_basis_order(order)
+    dp.set_material_regions(materials or {0: dict(eps_r=1, mu_r=1)})
+    dp.set_region_meshfunction(region)
+    dp.set_boundary_conditions(bcs)
+    dp.set_sources(cs)
+    dp.set_frequency(freq)
+    dp.init_problem()
+    return dp
+
+
+def test_driven_lhs_rhs_reference_fixture(dev, golden):
+    """test_EMDrivenProblem.py:64-82 through the reference-shaped API."""
+    from sucemfem.Testing import Meshes
+    from sucemfem.Sources import point_source
+    z = golden("driven_order1_inscribed_tet.npz")
+    dip = point_source.PointCurrentSource()
+    dip.set_position(z["source_coord"])
+    dip.set_value(z["source_value"])
+    dp = _driven_problem(Meshes.InscribedTet().get_dolfin_mesh(), 1, float(z["frequency"]), [dip])
+    A = dp.get_LHS_matrix().todense()
+    assert np.allclose(A, z["lhs"], rtol=1e-10, atol=3e-15)
+    b = dp.get_RHS()
+    assert np.allclose(b, z["rhs"], rtol=1e-12, atol=1e-16)
+
+
+def test_filament_contributions_fixture(dev, golden):
+    """test_fillament_source.py:62-69 (26 significant DOFs; see the oracle test)."""
+    from sucemfem.Utilities.MeshGenerators import get_centred_cube
+    from sucemfem.Mesh import FunctionSpace
+    from sucemfem.Sources.fillament_current_source import FillamentCurrentSource
+    f = golden("filament_order2.npz")
+    mesh = get_centred_cube(f["domain_size"], float(f["max_edge_len"]))
+    V = FunctionSpace(mesh, "Nedelec 1st kind H(curl)", 2)
+    src = FillamentCurrentSource()
+    src.set_function_space(V)
+    src.set_no_integration_points(int(f["no_integration_points"]))
+    src.set_source_endpoints(f["source_endpoints"])
+    src.set_value(float(f["I"]))
+    dn, v = src.get_contribution()
+    mine = dict(zip(dn.astype(np.int64), v))
+    ref = dict(zip(f["dofnos"], f["rhs_contribs"]))
+    sig = [k for k, val in ref.items() if abs(val) > 1e-10]
+    assert len(sig) == 26
+    for k in sig:
+        assert abs(mine[k] - ref[k]) <= 1e-12 + 1e-10 * abs(ref[k])
+    # reversed filament negates every contribution (test_fillament_source.py:71-82)
+    src.set_source_endpoints(-f["source_endpoints"])
+    dn2, v2 = src.get_contribution()
+    back = dict(zip(dn2.astype(np.int64), v2))
+    for k in sig:
+        assert abs(back[k] + mine[k]) < 1e-13
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_order2_driven_solve_fixture(dev, golden, precision):
+    """Order-2 driven solve (reference_dofs-2-... fixture, UMFPACK in the reference)."""
+    from sucemfem.Utilities.MeshGenerators import get_centred_cube
+    from sucemfem.Sources import point_source
+    from sucemfem.Utilities.LinalgSolvers import COCGSolver
+    _say("order2 driven solve precision %d" % precision)
+    z = golden("driven_order2_dipole_x.npz")
+    lam = O.c0 / float(z["freq"])
+    mesh = get_centred_cube(z["domain_size"], float(z["max_edge_len"]), [0, 0, 0.])
+    dip = point_source.PointCurrentSource()
+    dip.set_position([0, 0, 0.])
+    dip.set_value(np.array([0, 0, 1.]) * lam / 1000)
+    dp = _driven_problem(mesh, 2, float(z["freq"]), [dip])
+    assert dp.get_global_dimension() == 5530
+    A = dp.get_LHS_matrix()
+    b = dp.get_RHS()
+    solver = COCGSolver(A, 'diagonal', tol=1e-11, precision=precision, maxiter=20000)
+    x = solver.solve(b)
+    assert solver.stats["status"] == 0, solver.stats
+    assert solver.stats["rel_residual"] <= 1e-11
+    rel = np.linalg.norm(x - z["x"]) / np.linalg.norm(z["x"])
+    assert rel < 1e-8, rel
+
+
+def test_bicgstab_generic_csr_matches_direct(dev):
+    """LinalgSolvers.BiCGStabSolver on an uploaded scipy matrix (not the fused operator)."""
+    from sucemfem.Utilities.LinalgSolvers import BiCGStabSolver
+    co, ce = O.unit_cube(3, 3, 3)
+    osp = O.Space(co * 0.1, ce, 1)
+    A = O.driven_lhs(O.assemble_fast(osp), 3e8).tocsr()
+    rng = np.random.default_rng(3)
+    b = rng.normal(size=osp.ndofs) + 1j * rng.normal(size=osp.ndofs)
+    solver = BiCGStabSolver(A, 'diagonal', tol=1e-10, maxiter=20000)
+    x = solver.solve(b)
+    xo = O.solve_direct(A, b)
+    assert solver.stats["status"] == 0, solver.stats
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-7
+    res = solver.get_logging_data()["res"]
+    assert np.isfinite(res[-2])
+
+
+def test_identity_solve(dev):
+    """test_LinalgSolvers.py:36-47: the identity system returns b."""
+    from sucemfem.Utilities.LinalgSolvers import solve_sparse_system
+    n = 1000
+    A = scipy.sparse.eye(n, n, format="csr")
+    b = np.random.default_rng(0).random(n)
+    x = solve_sparse_system(A, b, preconditioner_type='diagonal')
+    assert np.allclose(b, x, rtol=0, atol=1e-15)
+
+
+def test_material_errors(dev):
+    from sucemfem.Testing import Meshes
+    from sucemfem.Sources import point_source
+    from sucemfem.Mesh import MeshFunction
+    mesh = Meshes.InscribedTet().get_dolfin_mesh()
+    region = MeshFunction('uint', mesh, 3)
+    region.set_all(0)
+    region[3] = 7
+    dip = point_source.PointCurrentSource()
+    dip.set_position([0, 0, 0.])
+    dip.set_value([1, 0, 0.])
+    with pytest.raises(ValueError, match="Material number 7"):
+        _driven_problem(mesh, 1, 1e8, [dip], {0: dict(eps_r=1, mu_r=1)}, region)
+
+
+def test_point_outside_mesh_raises(dev):
+    from sucemfem.Testing import Meshes
+    from sucemfem.Sources import point_source
+    dip = point_source.PointCurrentSource()
+    dip.set_position([3, 3, 3.])
+    dip.set_value([1, 0, 0.])
+    dp = _driven_problem(Meshes.InscribedTet().get_dolfin_mesh(), 1, 1e8, [dip])
+    with pytest.raises(AssertionError):
+        dp.get_RHS()
+
+
+def test_fused_operator_matches_materialised(dev):
+    """y = (S - k0^2 T + i k0 S0) x on the device vs the exported LHS, FP64 and FP32."""
+    import torch
+    from sucemfem import _hip
+    co, ce = O.unit_cube(4, 4, 4)
+    _, V = device_space(co * 0.3, ce, 2, eps=np.random.default_rng(1).choice([1, 4.4], len(ce)))
+    sysm = assemble_device(V)
+    k0 = 2 * np.pi * 1e9 / O.c0
+    indptr, indices = V.pattern()
+    n = V.dim()
+    A = scipy.sparse.csr_matrix((sysm.lhs_values(k0), indices, indptr), shape=(n, n))
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))
+    y_ref = (A @ x.T).T
+    xd = torch.as_tensor(x, dtype=torch.complex128, device="cuda")
+    yd = torch.zeros_like(xd)
+    _hip.call("sucem_system_apply", V.ctx.handle, sysm.handle.h, k0, 64, 0, 3, _hip.ptr(xd),
+              _hip.ptr(yd))
+    y = yd.cpu().numpy()
+    assert np.abs(y - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+    xs = xd.to(torch.complex64)
+    ys = torch.zeros_like(xs)
+    _hip.call("sucem_system_apply", V.ctx.handle, sysm.handle.h, k0, 32, 0, 3, _hip.ptr(xs),
+              _hip.ptr(ys))
+    assert np.abs(ys.cpu().numpy() - y_ref).max() <= 2e-5 * np.abs(y_ref).max()

@@ -1,0 +1,217 @@
+"""The FP64 oracle pinned against the reference's golden data and against the mathematics."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle as O
+
+
+# ---- reference fixtures ---------------------------------------------------------------
+def test_driven_lhs_order1_fixture(golden):
+    """LHS_matrix.npy (test_EMDrivenProblem.py:64-72): real part S - k0^2 T, imaginary k0 S0."""
+    z = golden("driven_order1_inscribed_tet.npz")
+    sp_ = O.Space(*O.inscribed_tet(), 1)
+    A = O.driven_lhs(O.assemble(sp_), float(z["frequency"])).toarray()
+    assert np.allclose(A, z["lhs"], rtol=1e-10, atol=3e-15)
+
+
+def test_driven_rhs_order1_fixture(golden):
+    """RHS_vector.pickle (test_EMDrivenProblem.py:74-82)."""
+    z = golden("driven_order1_inscribed_tet.npz")
+    sp_ = O.Space(*O.inscribed_tet(), 1)
+    dn, cv = O.point_source_contrib(sp_, z["source_coord"], z["source_value"])
+    b = O.driven_rhs(sp_.ndofs, dn, cv, float(z["frequency"]))
+    assert np.allclose(b, z["rhs"], rtol=1e-12, atol=1e-16)
+
+
+def test_filament_order2_fixture(golden):
+    """fillament_source_test_data.pickle (test_fillament_source.py:62-69).
+
+    The filament lies on faces shared by two cells; the 26 significant DOFs are
+    independent of which cell is used (tangential continuity), the remaining
+    DOF ids of the fixture carry |value| < 1e-15.
+    """
+    f = golden("filament_order2.npz")
+    co, ce = O.centred_cube(f["domain_size"], float(f["max_edge_len"]))
+    sp_ = O.Space(co, ce, 2)
+    dn, v = O.filament_contrib(sp_, f["source_endpoints"], float(f["I"]),
+                               int(f["no_integration_points"]))
+    mine = dict(zip(dn, v))
+    ref = dict(zip(f["dofnos"], f["rhs_contribs"]))
+    sig = {k for k, val in ref.items() if abs(val) > 1e-10}
+    assert len(sig) == 26
+    for k in sig:
+        assert abs(mine[k] - ref[k]) <= 1e-12 + 1e-10 * abs(ref[k])
+    for k in set(mine) | set(ref):
+        if k not in sig:
+            assert abs(mine.get(k, 0.0)) < 1e-14 and abs(ref.get(k, 0.0)) < 1e-14
+
+
+def test_filament_reversal_symmetry(golden):
+    """test_fillament_source.py:71-82: reversing the filament negates the contributions."""
+    f = golden("filament_order2.npz")
+    co, ce = O.centred_cube(f["domain_size"], float(f["max_edge_len"]))
+    sp_ = O.Space(co, ce, 2)
+    ep = f["source_endpoints"]
+    dn, v = O.filament_contrib(sp_, ep, 1.0, 20)
+    dn2, v2 = O.filament_contrib(sp_, -ep, 1.0, 20)
+    a, b = dict(zip(dn, v)), dict(zip(dn2, v2))
+    for k in set(a) | set(b):
+        assert abs(a.get(k, 0.) + b.get(k, 0.)) < 1e-13
+
+
+def _order2_pointsource_system(z, centred=True):
+    freq = float(z["freq"])
+    lam = O.c0 / freq
+    co, ce = O.centred_cube(z["domain_size"], float(z["max_edge_len"]), [0, 0, 0.] if centred else None)
+    sp_ = O.Space(co, ce, 2)
+    A = O.driven_lhs(O.assemble_fast(sp_), freq)
+    dn, cv = O.point_source_contrib(sp_, [0, 0, 0.], np.array([0, 0, 1.]) * lam / 1000)
+    return sp_, A, O.driven_rhs(sp_.ndofs, dn, cv, freq)
+
+
+def test_order2_driven_solution_fixture(golden):
+    """reference_dofs-2-...pickle: full order-2 UMFPACK solve (ntff_generate_dofs.py:40-95)."""
+    z = golden("driven_order2_dipole_x.npz")
+    sp_, A, b = _order2_pointsource_system(z)
+    assert sp_.ndofs == 5530
+    x = O.solve_direct(A, b)
+    assert np.linalg.norm(x - z["x"]) / np.linalg.norm(z["x"]) < 1e-12
+
+
+def test_order2_filament_solution_fixture(golden):
+    """power_flux_reference_dofs_...pickle (1314 DOFs) and its DOLFIN mesh XML."""
+    z = golden("driven_order2_filament_x.npz")
+    freq = float(z["freq"])
+    lam = O.c0 / freq
+    co, ce = O.centred_cube(np.array([lam] * 3) * 0.25, lam / 6)
+    assert np.array_equal(ce, z["mesh_cells"]) and len(co) == int(z["mesh_nv"])
+    sp_ = O.Space(co, ce, 2)
+    A = O.driven_lhs(O.assemble_fast(sp_), freq)
+    dn, cv = O.filament_contrib(sp_, z["source_endpoints"], float(z["I"]), 100)
+    x = O.solve_direct(A, O.driven_rhs(sp_.ndofs, dn, cv, freq))
+    assert np.linalg.norm(x - z["x"]) / np.linalg.norm(z["x"]) < 1e-12
+
+
+@pytest.mark.slow
+def test_order2_filament9_fixture_residual(golden):
+    """examples/filament_source/data/dofs-2-...-100.pickle came from SciPy's iterative
+    solver at its default tolerance: in the oracle system its relative residual is < 1e-5."""
+    z = golden("driven_order2_filament9_x.npz")
+    co, ce = O.centred_cube(z["domain_size"], float(z["max_edge_len"]))
+    sp_ = O.Space(co, ce, 2)
+    assert sp_.ndofs == 30186
+    A = O.driven_lhs(O.assemble_fast(sp_), float(z["freq"]))
+    l = float(z["l"])
+    dn, cv = O.filament_contrib(sp_, np.array([[0, 0, -l / 2], [0, 0, l / 2]]), float(z["I"]),
+                                int(z["source_integration_points"]))
+    b = O.driven_rhs(sp_.ndofs, dn, cv, float(z["freq"]))
+    r = np.linalg.norm(A @ z["x"] - b) / np.linalg.norm(b)
+    assert r < 1e-5
+
+
+def test_unitcube1_boundary_edges(golden):
+    """test_Geometry.py:31-35: UnitCube(1,1,1) has 18 boundary edges."""
+    co, ce = O.unit_cube(1, 1, 1)
+    sp_ = O.Space(co, ce, 1)
+    assert sp_.pec_mask().sum() == int(golden("unitcube1_topology.npz")["n_boundary_edges"])
+
+
+# ---- mathematics ----------------------------------------------------------------------
+def test_whitney_closed_form():
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        X = rng.normal(size=(4, 3))
+        T, S = O.element_matrices(1, X)
+        Tw, Sw = O.whitney_closed_form(X)
+        assert np.allclose(T, Tw, rtol=1e-12, atol=1e-14 * abs(Tw).max())
+        assert np.allclose(S, Sw, rtol=1e-12, atol=1e-14 * abs(Sw).max())
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_basis_is_nodal(order):
+    basis, _ = O.reference_basis(order)
+    D = O.apply_duals(order, basis)
+    assert np.allclose(D, np.eye(len(basis)), atol=1e-13)
+
+
+def test_numbering_matches_sequential_scan():
+    rng = np.random.default_rng(2)
+    co, ce = O.unit_cube(3, 2, 2)
+    perm = rng.permutation(len(ce))
+    ce = ce[perm]
+    topo = O.number_entities(ce)
+    e, f = O.number_entities_sequential(ce)
+    assert np.array_equal(topo["cell_edges"], e)
+    assert np.array_equal(topo["cell_faces"], f)
+
+
+def test_unit_cube_counts():
+    """Closed-form entity counts of the Kuhn UnitCube (n=4: 604 edges, 864 faces)."""
+    co, ce = O.unit_cube(4, 4, 4)
+    t = O.number_entities(ce)
+    assert len(t["edges"]) == 604 and len(t["faces"]) == 864 and len(ce) == 384
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_curl_of_gradients_vanishes(order):
+    """S G = 0 for the discrete gradient; T symmetric positive definite."""
+    co, ce = O.box_mesh([1.0, 0.5, 0.75], 2)
+    sp_ = O.Space(co, ce, order)
+    m = O.assemble_fast(sp_, abc=False)
+    G = O.discrete_gradient(sp_)
+    assert abs(m["S"] @ G).max() < 1e-12 * abs(m["S"]).max()
+    T = m["M"].toarray()
+    assert np.allclose(T, T.T, atol=1e-15) and np.linalg.eigvalsh(T).min() > 0
+
+
+def test_cavity_order1_null_space_and_modes():
+    """cfg 4x4x4 box 1.0 x 0.5 x 0.75: null space = interior P1 vertices (27); lowest mode
+    within 2% of the analytic TE101 value (order-1 discretisation error)."""
+    co, ce = O.box_mesh([1.0, 0.5, 0.75], 4)
+    sp_ = O.Space(co, ce, 1)
+    m = O.assemble_fast(sp_, abc=False)
+    free = np.flatnonzero(~sp_.pec_mask())
+    assert sp_.ndofs == 604 and len(free) == 316
+    w = scipy.linalg.eigh(m["S"][free][:, free].toarray(), m["M"][free][:, free].toarray(),
+                          eigvals_only=True)
+    assert (np.abs(w) < 1e-8).sum() == 27
+    k2, _, _ = O.cavity_eigs(sp_, 10, 1.1)
+    ana = O.analytic_cavity_k2([1.0, 0.5, 0.75], 10)
+    assert abs(k2[0] - ana[0]) / ana[0] < 0.02
+    assert np.all(np.abs(k2 - ana) / ana < 0.08)
+
+
+def test_cavity_order2_close_to_analytic():
+    co, ce = O.box_mesh([1.0, 0.5, 0.75], 4)
+    sp_ = O.Space(co, ce, 2)
+    k2, _, _ = O.cavity_eigs(sp_, 10, 1.1)
+    ana = O.analytic_cavity_k2([1.0, 0.5, 0.75], 10)
+    assert np.all(np.abs(k2 - ana) / ana < 0.007)
+
+
+def test_oracle_rejects_point_outside():
+    sp_ = O.Space(*O.inscribed_tet(), 1)
+    with pytest.raises(AssertionError):
+        O.point_source_contrib(sp_, [5, 5, 5.], [1, 0, 0.])
+
+
+def test_face_matrix_planted_error_is_caught(golden):
+    """A plausible mistake (normal component kept in the ABC term) fails the LHS fixture."""
+    z = golden("driven_order1_inscribed_tet.npz")
+    sp_ = O.Space(*O.inscribed_tet(), 1)
+    m = O.assemble(sp_)
+    # full N.N instead of the tangential trace
+    k = sp_.nloc
+    wrong = np.zeros((len(sp_.cells), k, k))
+    for c, lf in zip(sp_.boundary_face_cell, sp_.boundary_face_local):
+        X = sp_.coords[sp_.cells[c]]
+        p, q, r = (X[i] for i in O.LOCAL_FACES[lf])
+        nrm = np.cross(q - p, r - p)
+        tp, tw = O.tri_rule(4)
+        pts = p + np.outer(tp[:, 0], q - p) + np.outer(tp[:, 1], r - p)
+        vals, _ = O.eval_basis(1, X, pts)
+        wrong[c] += np.einsum("p,pid,pjd->ij", tw * np.linalg.norm(nrm), vals, vals)
+    m["S_0"] = O.fem._scatter(sp_, wrong)
+    A = O.driven_lhs(m, float(z["frequency"])).toarray()
+    assert not np.allclose(A, z["lhs"], rtol=1e-10, atol=3e-15)
