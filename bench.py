@@ -1,0 +1,179 @@
+"""Driven-solve throughput on one MI355X.
+
+Workload: 1 m cube, 40^3 boxes x 6 tets (384,000 tets), second-order Nedelec
+(2,480,240 DOFs), eps_r per box drawn (seed 0) from {1.0, 2.2, 4.4}, mu_r = 1,
+first-order ABC on all 19,200 boundary triangles, z-directed dipole (moment
+lambda/1000) at the cube centre, f = 1 GHz.  One step = one driven solve of
+A x = b to relative residual 1e-8 (FP64 ||b - A x|| / ||b||) with the FP32
+operator and FP64 reliable updates, matrices and b resident in HBM.
+
+Prints one JSON line.  ``--gpus`` above 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "sucem-fem_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(prob, fp64_iterations, budget_s=20.0):
+    """Oracle FP64 COCG (scipy CSR, one core) on the same system: time a bounded
+    number of iterations and scale to a full solve of ``fp64_iterations``."""
+    import numpy as np
+    import scipy.sparse
+    import oracle as O
+    indptr, indices = prob.space.pattern()
+    n = prob.ndofs
+    A = scipy.sparse.csr_matrix((prob.system.lhs_values(prob.k0), indices, indptr), shape=(n, n))
+    b = prob.rhs_dipoles(prob.source_pos, prob.source_moment, prob.k0).cpu().numpy()
+    t0 = time.perf_counter()
+    O.cocg_jacobi(A, b, tol=0.0, maxiter=2)
+    per_it = (time.perf_counter() - t0) / 2
+    k = max(2, int(budget_s / max(per_it, 1e-9)))
+    t0 = time.perf_counter()
+    _, its, _ = O.cocg_jacobi(A, b, tol=0.0, maxiter=k)
+    per_it = (time.perf_counter() - t0) / its
+    solve_s = per_it * fp64_iterations
+    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": "%d FP64 Jacobi-COCG iterations of the same 2,480,240-DOF system on the host "
+                      "(scipy CSR SpMV, %.3f s/iteration), scaled by the %d iterations the FP64 "
+                      "device solve needs" % (its, per_it, fp64_iterations)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=40, help="boxes per axis")
+    ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--precision", type=int, default=32)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--max-iter", type=int, default=60000)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=200)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d"
+              % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    torch.cuda.set_device(0)
+    from sucemfem.Utilities.Workloads import driven_cube
+
+    t0 = time.perf_counter()
+    prob = driven_cube(n=a.n, order=a.order)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log("setup %.2fs: N=%d nnz=%d nnz_abc=%d" % (setup_s, prob.ndofs, prob.nnz, prob.nnz0))
+    b = prob.rhs_dipoles(prob.source_pos, prob.source_moment, prob.k0)
+    x = torch.zeros_like(b)
+    stream = torch.cuda.current_stream()
+
+    def step(precision):
+        return prob.solve(prob.k0, b, x, precision=precision, tol=a.tol, max_iter=a.max_iter)
+
+    for i in range(a.warmup):
+        st = step(a.precision)
+        log("warmup %d: %s" % (i, st))
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        stats.append(step(a.precision))
+        log("step %d: iters %d ru %d rel %.3e %.1f ms" % (
+            i, stats[-1]["iterations"], stats[-1]["reliable_updates"], stats[-1]["rel_residual"],
+            stats[-1]["elapsed_ms"]))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ok = all(s["status"] == 0 and s["rel_residual"] <= a.tol for s in stats)
+
+    # dominant kernel: the fused FP32 SpMV, timed with HIP events on the solve's stream
+    xs = torch.randn(prob.ndofs, dtype=torch.complex64, device="cuda")
+    ys = torch.zeros_like(xs)
+    prob.apply(prob.k0, xs, ys, precision=a.precision)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(a.spmv_reps):
+        prob.apply(prob.k0, xs, ys, precision=a.precision)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    spmv_ms = e0.elapsed_time(e1) / a.spmv_reps
+    spmv_bytes = prob.spmv_bytes(a.precision)
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    log("spmv %.4f ms, %.1f GB/s" % (spmv_ms, achieved))
+
+    extra = {}
+    fp64_iters = None
+    if not a.no_fp64:
+        t1 = time.perf_counter()
+        s64 = step(64)
+        torch.cuda.synchronize()
+        fp64_s = time.perf_counter() - t1
+        fp64_iters = s64["iterations"]
+        extra["fp64_solves_per_s"] = 1.0 / fp64_s
+        extra["fp64_iterations"] = fp64_iters
+        extra["fp64_rel_residual"] = s64["rel_residual"]
+        extra["speedup_vs_fp64"] = (a.steps / elapsed) / (1.0 / fp64_s)
+        log("fp64 solve %.2fs iters %d rel %.3e" % (fp64_s, fp64_iters, s64["rel_residual"]))
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "spmv_fp32_driven40_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+
+    cpu = None
+    if not a.no_cpu_baseline and fp64_iters:
+        try:
+            cpu = cpu_baseline(prob, fp64_iters)
+        except Exception as e:  # noqa: BLE001 - the baseline is reported, never fatal
+            log("cpu baseline failed: %r" % (e,))
+
+    out = {
+        "metric": "solves/s + p50/p99 latency at rel. residual 1e-8, FP32+IR vs plain FP64 GPU",
+        "value": a.steps / elapsed if ok else 0.0,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64-ir" if a.precision == 32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": "driven cube %d^3 boxes x6 tets, order %d Nedelec, eps_r per box "
+                               "seed 0 from {1.0,2.2,4.4}, ABC, z dipole at centre, 1 GHz"
+                               % (a.n, a.order),
+                   "ndofs": prob.ndofs, "nnz": prob.nnz, "tol": a.tol, "solver": "Jacobi-COCG",
+                   "parallelism": "single GPU"},
+        "converged": ok,
+        "iterations": [s["iterations"] for s in stats],
+        "rel_residual": [s["rel_residual"] for s in stats],
+        "latency_ms_p50": float(np.percentile([s["elapsed_ms"] for s in stats], 50)),
+        "latency_ms_p99": float(np.percentile([s["elapsed_ms"] for s in stats], 99)),
+        "setup_s": setup_s,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": traffic,
+                     "kernel": "k_sys_spmv (fused S - k0^2 T + i k0 S0, FP32 values, complex64 x)",
+                     "bytes_per_launch": spmv_bytes, "avg_ms": spmv_ms},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

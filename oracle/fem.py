@@ -717,3 +717,37 @@ def discrete_gradient(space):
                         shape=(space.ndofs, nv + ne)).tocsr()
     G.data = G.data / cnt.data
     return G
+
+
+# ---------------------------------------------------------------------------
+# Krylov (CPU baseline of the driven solve)
+# ---------------------------------------------------------------------------
+def cocg_jacobi(A, b, tol=1e-8, maxiter=10000):
+    """Jacobi-preconditioned COCG for complex-symmetric A in FP64 (scipy CSR SpMV).
+
+    The CPU counterpart of the device driven solve; the reference's own
+    iterative path is ``solve_sparse_system`` (Utilities/LinalgSolvers.py:308-322).
+    Returns ``(x, iterations, relative residual history)``.
+    """
+    A = A.tocsr()
+    D = 1.0 / A.diagonal()
+    nb = np.linalg.norm(b)
+    x = np.zeros_like(b, dtype=np.complex128)
+    r = np.asarray(b, np.complex128).copy()
+    z = D * r
+    p = z.copy()
+    rho = r @ z
+    hist = []
+    for it in range(maxiter):
+        q = A @ p
+        alpha = rho / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        z = D * r
+        rho_new = r @ z
+        p = z + (rho_new / rho) * p
+        rho = rho_new
+        hist.append(np.linalg.norm(r) / nb)
+        if hist[-1] <= tol:
+            break
+    return x, it + 1, np.array(hist)

@@ -492,7 +492,7 @@ void sucem_solve_opts_default(sucem_solve_opts* o) {
   o->check_every = 8;
   o->eliminate_pec = 0;
   o->tol = 1e-8;
-  o->ru_delta = 0.1;
+  o->ru_delta = 0.01;
 }
 
 static SolveOpts convert(const sucem_solve_opts* o) {

@@ -107,7 +107,7 @@ struct SolveOpts {
   int max_iter = 10000;
   int check_every = 8;
   double tol = 1e-8;
-  double ru_delta = 0.1;
+  double ru_delta = 0.01;
   int eliminate_pec = 0;
 };
 struct SolveStats {

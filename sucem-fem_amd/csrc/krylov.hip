@@ -7,9 +7,9 @@
 //
 // Precision: FP64 throughout, or FP32 vectors/matrix with FP64 reliable updates
 // (x accumulated in FP64, true residual b - A x recomputed with the FP64 matrix
-// whenever the FP32 recursion has reduced the residual by ru_delta, and the FP32
-// residual replaced by it; the search direction is kept).  All dot products
-// accumulate in FP64.
+// whenever the FP32 recursion has reduced the residual by ru_delta, the FP32
+// residual replaced by it and the recursion restarted from it).  All dot
+// products accumulate in FP64.
 //
 // Every scalar (alpha, beta, ...) is derived on the device by each block from the
 // per-block partial sums written by the previous kernel, so an iteration needs
@@ -436,7 +436,11 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
       ++it;
       break;
     }
-    if (!cocg) {
+    if (cocg) {
+      // restart the direction from the replaced residual: keeping the old direction
+      // after a residual replacement stalls COCG on these indefinite systems
+      SUCEM_LAUNCH((k_cocg_start<C>), G, TB, 0, c->stream, n, G, dinv.p, r.p, p.p);
+    } else {
       // restart BiCGStab from the replaced residual (rt = r, rho = |r|^2)
       SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p, ptrue,
                    prho[(it + 1) & 1], 0);

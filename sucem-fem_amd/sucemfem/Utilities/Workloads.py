@@ -1,0 +1,96 @@
+"""Device-resident driven and cavity problems on structured boxes.
+
+The driven cube: an L-metre cube of n^3 boxes (6 Kuhn tetrahedra each), eps_r
+constant per box and drawn with ``numpy.random.default_rng(seed)`` from a
+list of values (mu_r = 1), first-order ABC on the whole boundary, and a dipole
+source.  Everything (mesh, material lookup, numbering, pattern, assembly, RHS)
+is built by libsucem_hip without a host copy of the mesh.
+"""
+from __future__ import division
+
+import ctypes
+
+import numpy as np
+
+from .. import _hip
+from ..Consts import c0, Z0
+from ..Mesh import FunctionSpace, Mesh
+from ..SystemMatrices import AssembledSystem
+from ..runtime import get_context
+
+
+def box_regions(n, values, seed):
+    """Region id per cell: one draw per box from ``values`` (cells of a box are consecutive)."""
+    rng = np.random.default_rng(seed)
+    nb = int(np.prod(n))
+    box = rng.integers(0, len(values), size=nb)
+    return np.repeat(box, 6).astype(np.int64)
+
+
+class DeviceProblem(object):
+    """Mesh + space + assembled system living on the device."""
+
+    def __init__(self, n, dims, order, which, eps_values=(1.0,), seed=0, shift=(0.0, 0.0, 0.0),
+                 ctx=None):
+        self.ctx = ctx or get_context()
+        self.n = (int(n),) * 3 if np.isscalar(n) else tuple(int(v) for v in n)
+        self.dims = np.asarray(dims, np.float64) * np.ones(3)
+        self.order = order
+        self.mesh = Mesh.box_on_device(self.n, scale=self.dims, shift=shift, ctx=self.ctx,
+                                       export=False)
+        mh = self.mesh._device[1]
+        eps_values = np.asarray(eps_values, np.float64)
+        self.region = box_regions(self.n, eps_values, seed) if len(eps_values) > 1 else None
+        ids = np.arange(len(eps_values), dtype=np.int64)
+        mu = np.ones(len(eps_values))
+        _hip.call("sucem_mesh_set_materials", mh.h, _hip.ptr(self.region), len(ids), _hip.ptr(ids),
+                  _hip.ptr(np.ascontiguousarray(eps_values)), _hip.ptr(mu))
+        self.space = FunctionSpace(self.mesh, "Nedelec 1st kind H(curl)", order, ctx=self.ctx)
+        self.system = AssembledSystem(self.space, which)
+        info = self.space.info()
+        self.ndofs, self.nnz, self.nnz0 = int(info[0]), int(info[1]), int(info[6])
+
+    def rhs_dipoles(self, positions, moments, k0, rhs_index=None, nrhs=1):
+        """b[r] = -j k0 Z0 sum N(x_s).p_s on the device (complex128, nrhs x N)."""
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
+        val = np.ascontiguousarray(moments, np.complex128).reshape(-1, 3)
+        idx = None if rhs_index is None else np.ascontiguousarray(rhs_index, np.int64)
+        b = self.ctx.tensor(nrhs * self.ndofs)
+        _hip.call("sucem_rhs_point_sources", self.ctx.handle, self.space.handle.h, len(pos),
+                  _hip.ptr(pos), _hip.ptr(val), _hip.ptr(idx), nrhs, 0.0, -k0 * Z0, _hip.ptr(b))
+        return b
+
+    def solve(self, k0, b, x, precision=64, tol=1e-8, max_iter=50000, method=_hip.METHOD_COCG,
+              check_every=16, ru_delta=0.01, hist=None):
+        o = _hip.SolveOpts()
+        _hip.load().sucem_solve_opts_default(ctypes.byref(o))
+        o.method, o.precision, o.tol, o.max_iter = method, precision, tol, max_iter
+        o.check_every, o.ru_delta = check_every, ru_delta
+        st = _hip.SolveStats()
+        _hip.call("sucem_solve_driven", self.ctx.handle, self.system.handle.h, float(k0),
+                  _hip.ptr(b), _hip.ptr(x), ctypes.byref(o), ctypes.byref(st), _hip.ptr(hist),
+                  0 if hist is None else len(hist))
+        return st.as_dict()
+
+    def apply(self, k0, x, y, precision=32, nvec=1):
+        _hip.call("sucem_system_apply", self.ctx.handle, self.system.handle.h, float(k0), precision,
+                  0, nvec, _hip.ptr(x), _hip.ptr(y))
+
+    def spmv_bytes(self, precision):
+        """Algorithmic HBM bytes of one fused SpMV (one vector)."""
+        sv = 8 if precision == 32 else 16          # (S, T) pair
+        s0v = 4 if precision == 32 else 8
+        sx = 8 if precision == 32 else 16          # complex vector element
+        n, nnz, nnz0 = self.ndofs, self.nnz, self.nnz0
+        return (nnz * (4 + sv) + (n + 1) * 4 + nnz0 * (4 + s0v) + (n + 1) * 4 + 2 * n * sx)
+
+
+def driven_cube(n=40, L=1.0, order=2, freq=1e9, eps_values=(1.0, 2.2, 4.4), seed=0, ctx=None):
+    """The driven-cube workload: z dipole of moment lambda/1000 at the cube centre."""
+    prob = DeviceProblem(n, L, order, _hip.MAT_S | _hip.MAT_T | _hip.MAT_S0, eps_values, seed,
+                         ctx=ctx)
+    lam = c0 / freq
+    prob.k0 = 2 * np.pi * freq / c0
+    prob.source_pos = np.array([[L / 2, L / 2, L / 2]])
+    prob.source_moment = np.array([[0, 0, lam / 1000]], np.complex128)
+    return prob
