@@ -60,6 +60,28 @@ typedef struct sucem_solve_opts {
   double ru_delta;    /* FP32: reliable update when the residual fell by this factor */
 } sucem_solve_opts;
 
+typedef struct sucem_eig_opts {
+  int ncv;              /* Lanczos basis size (0: max(2 nev + 1, 20)) */
+  int max_restarts;
+  int inner_precision;  /* 64 or 32 (FP32 inner solves with FP64 reliable updates) */
+  int inner_max_iter;
+  double tol;           /* relative Ritz residual of nu = 1/(k^2 - sigma) */
+  double inner_tol;     /* relative residual of the inner (S - sigma T) solves */
+  uint64_t seed;        /* start vector */
+} sucem_eig_opts;
+
+typedef struct sucem_eig_stats {
+  int status;           /* 0 converged, 1 restart limit reached */
+  int nconv;
+  int restarts;
+  int lanczos_steps;
+  int inner_failures;
+  int pad_;
+  int64_t inner_iterations;
+  double max_ritz_residual;
+  double elapsed_ms;
+} sucem_eig_stats;
+
 typedef struct sucem_solve_stats {
   int status;           /* 0 converged, >0 iteration limit, <0 breakdown (scipy 'info') */
   int iterations;
@@ -157,6 +179,16 @@ int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const doubl
 int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
                        const double* b_d, double* x_d, const sucem_solve_opts* opts,
                        sucem_solve_stats* stats, double* res_hist_h, int64_t hist_cap);
+
+/* ---- cavity eigenproblem (replaces DefaultEigenSolver.solve_problem,
+ *      EMVectorWaveEigenproblem.py:61-105): the nev lowest k^2 > sigma of the PEC-eliminated
+ *      pencil (S, T), ascending, returned as k^2 directly (the reference returns
+ *      nu = 1/(k^2 - sigma)).  resid_h (nullable): ||S x - k^2 T x|| / (k^2 ||T x||);
+ *      vecs_d (nullable): nev x N complex128 T-normalised modes (zero on PEC DOFs). ---- */
+void sucem_eig_opts_default(sucem_eig_opts* o);
+int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,
+               const sucem_eig_opts* opts, double* k2_h, double* resid_h, double* vecs_d,
+               sucem_eig_stats* stats);
 
 /* ---- generic complex CSR (LinalgSolvers on a user scipy matrix, LinalgSolvers.py:30-322) ---- */
 int sucem_csr_create(sucem_ctx* ctx, int64_t n, int64_t nnz, const int64_t* indptr_h,

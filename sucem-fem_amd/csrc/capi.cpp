@@ -556,6 +556,57 @@ int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, 
   API_END
 }
 
+void sucem_eig_opts_default(sucem_eig_opts* o) {
+  if (!o) return;
+  EigOpts d;
+  o->ncv = d.ncv;
+  o->max_restarts = d.max_restarts;
+  o->inner_precision = d.inner_precision;
+  o->inner_max_iter = d.inner_max_iter;
+  o->tol = d.tol;
+  o->inner_tol = d.inner_tol;
+  o->seed = d.seed;
+}
+
+int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,
+               const sucem_eig_opts* opts, double* k2_h, double* resid_h, double* vecs_d,
+               sucem_eig_stats* stats) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys); NOT_NULL(k2_h);
+  SUCEM_CHECK((sys->s.which & (SUCEM_MAT_S | SUCEM_MAT_T)) == (SUCEM_MAT_S | SUCEM_MAT_T),
+              SUCEM_ERR_INVALID, "the eigen solver needs S and T");
+  Context* c = &ctx->c;
+  c->use();
+  EigOpts o;
+  if (opts) {
+    o.ncv = opts->ncv;
+    o.max_restarts = opts->max_restarts;
+    o.inner_precision = opts->inner_precision;
+    o.inner_max_iter = opts->inner_max_iter;
+    o.tol = opts->tol;
+    o.inner_tol = opts->inner_tol;
+    o.seed = opts->seed;
+  }
+  SUCEM_CHECK(o.inner_precision == 32 || o.inner_precision == 64, SUCEM_ERR_INVALID,
+              "inner precision must be 32 or 64");
+  SUCEM_CHECK(o.tol > 0 && o.inner_tol > 0 && o.max_restarts >= 0 && o.inner_max_iter > 0,
+              SUCEM_ERR_INVALID, "bad eigen options");
+  EigStats st;
+  eigs_system(c, &sys->s, sigma, nev, o, k2_h, resid_h, (double2*)vecs_d, st);
+  if (stats) {
+    stats->status = st.status;
+    stats->nconv = st.nconv;
+    stats->restarts = st.restarts;
+    stats->lanczos_steps = st.lanczos_steps;
+    stats->inner_failures = st.inner_failures;
+    stats->pad_ = 0;
+    stats->inner_iterations = st.inner_iterations;
+    stats->max_ritz_residual = st.max_ritz_residual;
+    stats->elapsed_ms = st.elapsed_ms;
+  }
+  API_END
+}
+
 int sucem_csr_create(sucem_ctx* ctx, int64_t n, int64_t nnz, const int64_t* indptr_h,
                      const int32_t* indices_h, const double* values_h, sucem_csr** out) {
   API_BEGIN

@@ -1,0 +1,380 @@
+// Resonant-cavity eigen solver (replaces DefaultEigenSolver.solve_problem,
+// ProblemConfigurations/EMVectorWaveEigenproblem.py:61-105: LU of S - sigma M and
+// ARPACK shift-invert).
+//
+// Thick-restart Lanczos on OP = (S - sigma T)^-1 T, self-adjoint in the T inner
+// product, on the PEC-eliminated pencil (rows/columns of the DOFs on boundary
+// facets removed).  The wanted modes are the largest positive Ritz values
+// nu = 1/(lambda - sigma) > 0, i.e. the lowest k^2 above sigma; the gradient null
+// space of the curl-curl operator maps to nu = -1/sigma < 0 and is never wanted.
+// Inner solves use the device COCG (FP64, or FP32 with FP64 reliable updates);
+// orthogonalisation, the projected matrix and its eigen-decomposition are FP64,
+// and each returned k^2 is the FP64 Rayleigh quotient x^T S x / x^T T x.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "internal.hpp"
+#include "kernels.hpp"
+
+namespace sucem {
+namespace {
+
+constexpr int TB = 256;
+
+struct Range { int start, end; };
+__device__ __forceinline__ Range rows_of(int n, int G) {
+  int lb = xcd_block(G);
+  int chunk = (n + G - 1) / G;
+  Range r;
+  r.start = lb * chunk;
+  r.end = min(n, r.start + chunk);
+  return r;
+}
+
+// deterministic pseudo-random start vector (splitmix64), zero on masked DOFs
+__global__ void k_start_vector(int64_t n, unsigned long long seed, const uint8_t* mask,
+                               double2* v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    double u = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    v[i] = make_double2((mask && mask[i]) ? 0.0 : u, 0.0);
+  }
+}
+
+// part[i*G + lb] = sum_r Re(conj(a_i[r]) y[r]) for i < m  (vectors a_i = base + i*n)
+__global__ __launch_bounds__(TB) void k_multidot(int n, int G, int m, const double2* base,
+                                                 const double2* y, double2* part) {
+  Range rr = rows_of(n, G);
+  for (int i = 0; i < m; ++i) {
+    const double2* a = base + (int64_t)i * n;
+    double2 s = make_double2(0, 0);
+    for (int r = rr.start + threadIdx.x; r < rr.end; r += TB)
+      s.x += a[r].x * y[r].x + a[r].y * y[r].y;
+    s = block_sum<TB>(s);
+    if (threadIdx.x == 0) part[(int64_t)i * G + xcd_block(G)] = s;
+  }
+}
+
+// y -= sum_i h[i] v_i
+__global__ void k_multiaxpy(int n, int G, int m, const double* h, const double2* base,
+                            double2* y) {
+  Range rr = rows_of(n, G);
+  for (int r = rr.start + threadIdx.x; r < rr.end; r += TB) {
+    double2 acc = y[r];
+    for (int i = 0; i < m; ++i) {
+      const double2 v = base[(int64_t)i * n + r];
+      acc.x -= h[i] * v.x;
+      acc.y -= h[i] * v.y;
+    }
+    y[r] = acc;
+  }
+}
+
+// out_c = sum_i in_i * S[i*ld + c] for c < k (restart combination)
+__global__ void k_combine(int n, int G, int m, int k, const double* S, int ld, const double2* in,
+                          double2* out) {
+  Range rr = rows_of(n, G);
+  for (int r = rr.start + threadIdx.x; r < rr.end; r += TB) {
+    for (int c = 0; c < k; ++c) {
+      double2 acc = make_double2(0, 0);
+      for (int i = 0; i < m; ++i) {
+        const double2 v = in[(int64_t)i * n + r];
+        const double s = S[i * ld + c];
+        acc.x += s * v.x;
+        acc.y += s * v.y;
+      }
+      out[(int64_t)c * n + r] = acc;
+    }
+  }
+}
+
+__global__ void k_scale_copy(int n, int G, double s, const double2* in, double2* out) {
+  Range rr = rows_of(n, G);
+  for (int r = rr.start + threadIdx.x; r < rr.end; r += TB)
+    out[r] = make_double2(in[r].x * s, in[r].y * s);
+}
+
+// Symmetric eigen-decomposition of a small dense matrix (cyclic Jacobi), ascending.
+void jacobi_eig(std::vector<double> A, int n, std::vector<double>& w, std::vector<double>& V) {
+  V.assign(n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0, tot = 0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        tot += A[i * n + j] * A[i * n + j];
+        if (i != j) off += A[i * n + j] * A[i * n + j];
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = A[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        double app = A[p * n + p], aqq = A[q * n + q];
+        double theta = (aqq - app) / (2 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+        double c = 1 / std::sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          double vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = c * vkp - s * vkq;
+          V[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return A[a * n + a] < A[b * n + b]; });
+  w.resize(n);
+  std::vector<double> Vs(n * n);
+  for (int c = 0; c < n; ++c) {
+    w[c] = A[idx[c] * n + idx[c]];
+    for (int r = 0; r < n; ++r) Vs[r * n + c] = V[r * n + idx[c]];
+  }
+  V.swap(Vs);
+}
+
+}  // namespace
+
+void eigs_system(Context* c, const System* sys, double sigma, int nev, const EigOpts& o,
+                 double* lam_h, double* resid_h, double2* vecs_d, EigStats& st) {
+  const Space* s = sys->space;
+  SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S and T must be assembled");
+  const int n = (int)s->ndofs;
+  const int64_t nfree = s->ndofs - s->npec;
+  SUCEM_CHECK(nev > 0 && nev < nfree, SUCEM_ERR_INVALID, "nev must be in [1, free DOFs)");
+  int m = o.ncv > 0 ? o.ncv : std::max(2 * nev + 1, 20);
+  m = (int)std::min<int64_t>(m, nfree);
+  SUCEM_CHECK(m > nev, SUCEM_ERR_INVALID, "ncv must exceed nev");
+  auto t0 = std::chrono::steady_clock::now();
+  st = EigStats();
+  const int G = [&] {
+    int64_t g = std::min<int64_t>((int64_t)c->num_cus * 4, (n + 255) / 256);
+    g = (g + 7) / 8 * 8;
+    return (int)std::max<int64_t>(g, 8);
+  }();
+  const uint8_t* mask = s->pec.p;
+
+  DBuf<double2> V((int64_t)(m + 1) * n), TV((int64_t)(m + 1) * n);
+  DBuf<double2> W((int64_t)(m + 1) * n);   // restart scratch
+  DBuf<double2> y(n), ty(n);
+  DBuf<double2> part((int64_t)(m + 1) * G);
+  DBuf<double> hdev(m + 1);
+  std::vector<double> H((m + 1) * (m + 1), 0.0);
+  auto h = [&](int i, int j) -> double& { return H[i * (m + 1) + j]; };
+
+  Coef Tk;  // T
+  Tk.s = 0; Tk.t = 1; Tk.s0 = 0;
+  Coef Sk;  // S
+  Sk.s = 1; Sk.t = 0; Sk.s0 = 0;
+  Coef Ak;  // S - sigma T
+  Ak.s = 1; Ak.t = -sigma; Ak.s0 = 0;
+
+  auto vec = [&](DBuf<double2>& B, int i) { return B.p + (int64_t)i * n; };
+  auto multidot = [&](const double2* base, int cnt, const double2* v, std::vector<double>& out) {
+    SUCEM_LAUNCH(k_multidot, G, TB, 0, c->stream, n, G, cnt, base, v, part.p);
+    std::vector<double2> hp((int64_t)cnt * G);
+    SUCEM_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double2) * cnt * G, hipMemcpyDeviceToHost,
+                             c->stream));
+    c->sync();
+    out.assign(cnt, 0.0);
+    for (int i = 0; i < cnt; ++i)
+      for (int b = 0; b < G; ++b) out[i] += hp[(int64_t)i * G + b].x;
+  };
+  auto apply = [&](Coef k, const double2* x, double2* yv) {
+    system_apply<double>(c, sys, k, true, 1, x, yv);
+  };
+  SolveOpts so;
+  so.method = SUCEM_METHOD_COCG;
+  so.precision = o.inner_precision;
+  so.precond = SUCEM_PRECOND_JACOBI;
+  so.max_iter = o.inner_max_iter;
+  so.check_every = 8;
+  so.tol = o.inner_tol;
+  so.eliminate_pec = 1;
+
+  // v0: random, T-normalised
+  SUCEM_LAUNCH(k_start_vector, grid_for(n, TB), TB, 0, c->stream, (int64_t)n,
+               (unsigned long long)o.seed, mask, vec(V, 0));
+  apply(Tk, vec(V, 0), vec(TV, 0));
+  {
+    std::vector<double> nn;
+    multidot(vec(V, 0), 1, vec(TV, 0), nn);
+    double inv = 1.0 / std::sqrt(nn[0]);
+    SUCEM_LAUNCH(k_scale_copy, G, TB, 0, c->stream, n, G, inv, vec(V, 0), vec(V, 0));
+    SUCEM_LAUNCH(k_scale_copy, G, TB, 0, c->stream, n, G, inv, vec(TV, 0), vec(TV, 0));
+  }
+  int k = 0;            // kept vectors at the start of this cycle
+  std::vector<double> theta, Sv;
+  std::vector<int> wanted;
+  double beta = 0;
+  bool done = false;
+  for (int cycle = 0; cycle <= o.max_restarts && !done; ++cycle) {
+    int mm = m;
+    for (int j = k; j < m; ++j) {
+      SolveStats ist;
+      solve_system(c, sys, Ak, vec(TV, j), y.p, so, ist, nullptr, 0);
+      st.inner_iterations += ist.iterations;
+      st.lanczos_steps += 1;
+      if (ist.status != 0) st.inner_failures += 1;
+      // full re-orthogonalisation (classical Gram-Schmidt, twice) in the T inner product
+      for (int pass = 0; pass < 2; ++pass) {
+        std::vector<double> hh;
+        multidot(TV.p, j + 1, y.p, hh);
+        for (int i = 0; i <= j; ++i) h(i, j) += hh[i];
+        SUCEM_HIP(hipMemcpyAsync(hdev.p, hh.data(), sizeof(double) * (j + 1),
+                                 hipMemcpyHostToDevice, c->stream));
+        SUCEM_LAUNCH(k_multiaxpy, G, TB, 0, c->stream, n, G, j + 1, hdev.p, V.p, y.p);
+        c->sync();
+      }
+      apply(Tk, y.p, ty.p);
+      std::vector<double> nn;
+      multidot(y.p, 1, ty.p, nn);
+      beta = std::sqrt(std::max(nn[0], 0.0));
+      h(j + 1, j) = beta;
+      if (beta <= 1e-14 * std::fabs(h(j, j)) || !std::isfinite(beta)) {
+        mm = j + 1;   // invariant subspace
+        break;
+      }
+      SUCEM_LAUNCH(k_scale_copy, G, TB, 0, c->stream, n, G, 1.0 / beta, y.p, vec(V, j + 1));
+      SUCEM_LAUNCH(k_scale_copy, G, TB, 0, c->stream, n, G, 1.0 / beta, ty.p, vec(TV, j + 1));
+    }
+    // Rayleigh-Ritz on the projected (symmetrised) matrix
+    std::vector<double> P(mm * mm);
+    for (int i = 0; i < mm; ++i)
+      for (int j = 0; j < mm; ++j) {
+        double a = i <= j ? h(i, j) : h(j, i);
+        P[i * mm + j] = a;
+      }
+    jacobi_eig(P, mm, theta, Sv);
+    // wanted: the nev largest positive theta
+    wanted.clear();
+    for (int i = mm - 1; i >= 0 && (int)wanted.size() < nev; --i)
+      if (theta[i] > 0) wanted.push_back(i);
+    SUCEM_CHECK((int)wanted.size() == nev, SUCEM_ERR_INVALID,
+                "fewer positive Ritz values than requested modes (sigma above the spectrum?)");
+    double worst = 0;
+    bool conv = true;
+    for (int w : wanted) {
+      double r = std::fabs(beta * Sv[(mm - 1) * mm + w]);
+      double rel = r / std::fabs(theta[w]);
+      worst = std::max(worst, rel);
+      if (rel > o.tol) conv = false;
+    }
+    st.max_ritz_residual = worst;
+    st.restarts = cycle;
+    if (conv || mm < m || cycle == o.max_restarts) {
+      done = true;
+      st.status = conv ? 0 : 1;
+      // Ritz vectors of the wanted values into W[0..nev)
+      std::vector<double> Sw(mm * nev);
+      for (int i = 0; i < mm; ++i)
+        for (int q = 0; q < nev; ++q) Sw[i * nev + q] = Sv[i * mm + wanted[q]];
+      DBuf<double> Sd(Sw.size());
+      h2d(c, Sd.p, Sw.data(), Sw.size());
+      SUCEM_LAUNCH(k_combine, G, TB, 0, c->stream, n, G, mm, nev, Sd.p, nev, V.p, W.p);
+      c->sync();
+      break;
+    }
+    // thick restart: keep kk Ritz vectors with the largest theta plus the residual vector
+    int kk = std::min(m - 1, nev + (m - nev) / 2);
+    std::vector<int> keep;
+    for (int i = mm - 1; i >= 0 && (int)keep.size() < kk; --i) keep.push_back(i);
+    std::vector<double> Sk_(mm * kk);
+    for (int i = 0; i < mm; ++i)
+      for (int q = 0; q < kk; ++q) Sk_[i * kk + q] = Sv[i * mm + keep[q]];
+    DBuf<double> Sd(Sk_.size());
+    h2d(c, Sd.p, Sk_.data(), Sk_.size());
+    SUCEM_LAUNCH(k_combine, G, TB, 0, c->stream, n, G, mm, kk, Sd.p, kk, V.p, W.p);
+    SUCEM_HIP(hipMemcpyAsync(vec(V, 0), W.p, sizeof(double2) * (int64_t)kk * n,
+                             hipMemcpyDeviceToDevice, c->stream));
+    SUCEM_LAUNCH(k_combine, G, TB, 0, c->stream, n, G, mm, kk, Sd.p, kk, TV.p, W.p);
+    SUCEM_HIP(hipMemcpyAsync(vec(TV, 0), W.p, sizeof(double2) * (int64_t)kk * n,
+                             hipMemcpyDeviceToDevice, c->stream));
+    SUCEM_HIP(hipMemcpyAsync(vec(V, kk), vec(V, mm), sizeof(double2) * n, hipMemcpyDeviceToDevice,
+                             c->stream));
+    SUCEM_HIP(hipMemcpyAsync(vec(TV, kk), vec(TV, mm), sizeof(double2) * n,
+                             hipMemcpyDeviceToDevice, c->stream));
+    c->sync();
+    std::fill(H.begin(), H.end(), 0.0);
+    for (int q = 0; q < kk; ++q) {
+      h(q, q) = theta[keep[q]];
+      double coupling = beta * Sv[(mm - 1) * mm + keep[q]];
+      h(q, kk) = coupling;
+      h(kk, q) = coupling;
+    }
+    k = kk;
+    // the coupling column is rebuilt by the re-orthogonalisation of the next step
+    for (int q = 0; q < kk; ++q) h(q, kk) = 0.0;
+  }
+  // FP64 Rayleigh quotients and true residuals of the Ritz vectors
+  for (int q = 0; q < nev; ++q) {
+    double2* x = vec(W, q);
+    apply(Sk, x, y.p);
+    apply(Tk, x, ty.p);
+    std::vector<double> a, b;
+    multidot(x, 1, y.p, a);
+    multidot(x, 1, ty.p, b);
+    double lam = a[0] / b[0];
+    // r = S x - lam T x, relative to |lam| |T x|
+    std::vector<double> hl(1, lam);
+    SUCEM_HIP(hipMemcpyAsync(hdev.p, hl.data(), sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SUCEM_LAUNCH(k_multiaxpy, G, TB, 0, c->stream, n, G, 1, hdev.p, ty.p, y.p);
+    std::vector<double> rr, tt;
+    multidot(y.p, 1, y.p, rr);
+    multidot(ty.p, 1, ty.p, tt);
+    lam_h[q] = lam;
+    if (resid_h) resid_h[q] = std::sqrt(rr[0]) / (std::fabs(lam) * std::sqrt(tt[0]));
+    // T-normalise into the output
+    if (vecs_d)
+      SUCEM_LAUNCH(k_scale_copy, G, TB, 0, c->stream, n, G, 1.0 / std::sqrt(b[0]), x,
+                   vecs_d + (int64_t)q * n);
+  }
+  // ascending k^2
+  std::vector<int> ord(nev);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return lam_h[a] < lam_h[b]; });
+  std::vector<double> l2(nev), r2(nev);
+  for (int q = 0; q < nev; ++q) {
+    l2[q] = lam_h[ord[q]];
+    r2[q] = resid_h ? resid_h[ord[q]] : 0;
+  }
+  for (int q = 0; q < nev; ++q) {
+    lam_h[q] = l2[q];
+    if (resid_h) resid_h[q] = r2[q];
+  }
+  if (vecs_d) {
+    bool sorted = true;
+    for (int q = 0; q < nev; ++q) sorted &= ord[q] == q;
+    if (!sorted) {
+      DBuf<double2> tmp((int64_t)nev * n);
+      for (int q = 0; q < nev; ++q)
+        SUCEM_HIP(hipMemcpyAsync(tmp.p + (int64_t)q * n, vecs_d + (int64_t)ord[q] * n,
+                                 sizeof(double2) * n, hipMemcpyDeviceToDevice, c->stream));
+      SUCEM_HIP(hipMemcpyAsync(vecs_d, tmp.p, sizeof(double2) * (int64_t)nev * n,
+                               hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
+  c->sync();
+  st.nconv = st.status == 0 ? nev : 0;
+  st.elapsed_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace sucem

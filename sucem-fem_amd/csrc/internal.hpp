@@ -120,6 +120,24 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
 void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const SolveOpts& o,
                SolveStats& st, double* hist_h, int64_t hist_cap);
 
+// eigen.hip
+struct EigOpts {
+  int ncv = 0;               // Lanczos basis size (0: max(2 nev + 1, 20))
+  int max_restarts = 100;
+  int inner_precision = 64;
+  int inner_max_iter = 20000;
+  double tol = 1e-10;        // relative Ritz residual of nu
+  double inner_tol = 1e-11;  // inner (S - sigma T) solves
+  unsigned long long seed = 12345;
+};
+struct EigStats {
+  int status = 0, nconv = 0, restarts = 0, lanczos_steps = 0, inner_failures = 0;
+  int64_t inner_iterations = 0;
+  double max_ritz_residual = 0, elapsed_ms = 0;
+};
+void eigs_system(Context* c, const System* sys, double sigma, int nev, const EigOpts& o,
+                 double* lam_h, double* resid_h, double2* vecs_d, EigStats& st);
+
 }  // namespace sucem
 
 struct sucem_mesh { sucem::Mesh m; };

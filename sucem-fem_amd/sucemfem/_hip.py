@@ -35,6 +35,23 @@ class SolveStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
 
 
+class EigOpts(ctypes.Structure):
+    _fields_ = [("ncv", ctypes.c_int), ("max_restarts", ctypes.c_int),
+                ("inner_precision", ctypes.c_int), ("inner_max_iter", ctypes.c_int),
+                ("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
+class EigStats(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("nconv", ctypes.c_int), ("restarts", ctypes.c_int),
+                ("lanczos_steps", ctypes.c_int), ("inner_failures", ctypes.c_int),
+                ("pad_", ctypes.c_int), ("inner_iterations", ctypes.c_int64),
+                ("max_ritz_residual", ctypes.c_double), ("elapsed_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+
+
 class SucemError(RuntimeError):
     pass
 
@@ -78,6 +95,8 @@ SIGNATURES = {
     "sucem_solve_opts_default": (None, [_P]),
     "sucem_solve_driven": (ctypes.c_int, [_P, _P, _D, _P, _P, _P, _P, _P, _I64]),
     "sucem_solve_system": (ctypes.c_int, [_P, _P, _D, _D, _D, _P, _P, _P, _P, _P, _I64]),
+    "sucem_eig_opts_default": (None, [_P]),
+    "sucem_eigs": (ctypes.c_int, [_P, _P, _D, ctypes.c_int, _P, _P, _P, _P, _P]),
     "sucem_csr_create": (ctypes.c_int, [_P, _I64, _I64, _P, _P, _P, _PP]),
     "sucem_csr_apply": (ctypes.c_int, [_P, _P, _P, _P]),
     "sucem_csr_solve": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _I64]),
