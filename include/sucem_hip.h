@@ -151,6 +151,12 @@ int sucem_system_apply(sucem_ctx* ctx, sucem_system* sys, double k0, int precisi
 int sucem_system_apply_coef(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
                             int precision, int eliminate_pec, int64_t nvec, const void* x_d,
                             void* y_d);
+/* Autotune the fused SpMV launch (lanes per row, loads per lane, blocks) for precision 32 or
+ * 64 by timing every candidate with HIP events; the fastest is kept on the system and used by
+ * apply and solve.  cfg3_h (nullable) receives {lanes, loads, blocks}, ms_h the time per SpMV.
+ * (No reference counterpart: launch tuning of the device operator.) */
+int sucem_system_tune(sucem_ctx* ctx, sucem_system* sys, int precision, int reps, int* cfg3_h,
+                      double* ms_h);
 /* Materialised A values (complex128 interleaved) on the full pattern. */
 int sucem_system_export_lhs(sucem_system* sys, double k0, double* values_h);
 

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "ops.hpp"
 
 namespace sucem {
 
@@ -374,6 +375,22 @@ int sucem_system_apply(sucem_ctx* ctx, sucem_system* sys, double k0, int precisi
   if (precision == 64) system_apply<double>(c, &sys->s, k, eliminate_pec != 0, nvec, x_d, y_d);
   else if (precision == 32) system_apply<float>(c, &sys->s, k, eliminate_pec != 0, nvec, x_d, y_d);
   else throw Error(SUCEM_ERR_INVALID, "precision must be 32 or 64");
+  API_END
+}
+
+int sucem_system_tune(sucem_ctx* ctx, sucem_system* sys, int precision, int reps, int* cfg3_h,
+                      double* ms_h) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys);
+  SUCEM_CHECK(reps > 0, SUCEM_ERR_INVALID, "reps must be positive");
+  Context* c = &ctx->c;
+  c->use();
+  SpmvCfg f;
+  if (precision == 32) f = tune_sys<float>(c, &sys->s, reps);
+  else if (precision == 64) f = tune_sys<double>(c, &sys->s, reps);
+  else throw Error(SUCEM_ERR_INVALID, "precision must be 32 or 64");
+  if (cfg3_h) { cfg3_h[0] = f.L; cfg3_h[1] = f.U; cfg3_h[2] = f.G; }
+  if (ms_h) *ms_h = f.ms;
   API_END
 }
 

@@ -48,8 +48,15 @@ struct Space {
   DBuf<int32_t> ref_face_dofs;
 };
 
+// SpMV launch configuration: lanes per row, loads per lane, blocks (0 = heuristic).
+struct SpmvCfg {
+  int L = 0, U = 0, G = 0;
+  double ms = 0;
+};
+
 struct System {
   Context* ctx = nullptr;
+  SpmvCfg cfg[2];  // tuned FP32 / FP64 operator
   Space* space = nullptr;
   unsigned which = 0;
   DBuf<double2> st64;
@@ -66,6 +73,7 @@ struct Csr {
   DBuf<double2> val64;
   DBuf<float2> val32;
   DBuf<int32_t> diag;
+  SpmvCfg cfg;
 };
 
 // mesh_space.hip

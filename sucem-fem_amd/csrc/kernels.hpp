@@ -1,4 +1,4 @@
-// Device helpers shared by the operator and Krylov translation units.
+// Device helpers shared by the operator, Krylov and eigen translation units.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -20,6 +20,23 @@ __device__ __forceinline__ T cmul(T a, T b) {
 __device__ __forceinline__ double2 to_d(float2 a) { return make_double2(a.x, a.y); }
 __device__ __forceinline__ double2 to_d(double2 a) { return a; }
 
+template <typename C>
+__device__ __forceinline__ C from_d(double2 v);
+template <> __device__ __forceinline__ float2 from_d<float2>(double2 v) {
+  return make_float2((float)v.x, (float)v.y);
+}
+template <> __device__ __forceinline__ double2 from_d<double2>(double2 v) { return v; }
+
+// a / b, or 0 when b == 0 (an exactly converged recursion then stops moving)
+__device__ __forceinline__ double2 cdiv0(double2 a, double2 b) {
+  double den = b.x * b.x + b.y * b.y;
+  if (den == 0.0) return make_double2(0.0, 0.0);
+  return make_double2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
+}
+__device__ __forceinline__ double2 cmuld(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
 // Wavefront-then-block sum of a double2; every thread of the block gets the result.
 // Deterministic: fixed reduction tree.
 template <int TB>
@@ -38,19 +55,126 @@ __device__ __forceinline__ double2 block_sum(double2 v) {
   return s;
 }
 
-// Sum of G per-block partials, computed redundantly (and identically) by every block.
-template <int TB>
-__device__ __forceinline__ double2 sum_partials(const double2* p, int G) {
-  double2 v = make_double2(0.0, 0.0);
-  for (int i = threadIdx.x; i < G; i += TB) { v.x += p[i].x; v.y += p[i].y; }
-  return block_sum<TB>(v);
-}
-
 // Logical block index that gives each XCD a contiguous share of the rows
-// (blocks are dispatched round-robin over the 8 XCDs).
+// (blocks are dispatched round-robin over the 8 XCDs; G is a multiple of 8).
 __device__ __forceinline__ int xcd_block(int G) {
   const int b = blockIdx.x;
   return (b & 7) * (G >> 3) + (b >> 3);
+}
+
+// ---- device scalars of the Krylov recursions -------------------------------------------
+enum ScalarSlot {
+  S_RHO = 0,      // complex: current rho
+  S_ALPHA = 2,    // complex
+  S_BETA = 4,     // complex
+  S_OMEGA = 6,    // complex (BiCGStab)
+  S_RR = 8,       // |r|^2 of the recursion
+  S_TRUE = 9,     // |b - A x|^2 (FP64)
+  S_BNORM = 10,   // |b|^2
+  S_COUNT = 12
+};
+
+enum FinalizeMode {
+  FIN_NONE = 0,
+  FIN_COCG_ALPHA,   // s0 = p.q            -> alpha = rho / s0
+  FIN_COCG_BETA,    // s0 = r.z, s1 = |r|^2 -> beta = s0/rho, rho = s0, hist
+  FIN_RESID,        // s0 = |r|^2 (true), s1 = rho of the replaced residual
+  FIN_NORM,         // s0 = |b|^2
+  FIN_BICG_ALPHA,   // s0 = conj(rt).v      -> alpha = rho / s0
+  FIN_BICG_OMEGA,   // s0 = conj(s).t, s1 = |t|^2 -> omega = conj(s0)/s1
+  FIN_BICG_BETA,    // s0 = conj(rt).r, s1 = |r|^2 -> beta = (s0/rho)(alpha/omega), rho = s0
+};
+
+// One reduction site: per-block partial sums, a ticket counter (zero between launches)
+// and the scalar array the last block finalises into.
+struct RedSite {
+  double2* part;      // 2*G entries
+  unsigned* counter;  // one per site
+  double* scal;       // S_COUNT doubles
+  double* hist;       // residual history (nullable)
+  int mode;
+  int it;
+};
+
+__device__ __forceinline__ void finalize(const RedSite& rs, double2 s0, double2 s1) {
+  double* sc = rs.scal;
+  const double2 rho = make_double2(sc[S_RHO], sc[S_RHO + 1]);
+  double2 out;
+  switch (rs.mode) {
+    case FIN_COCG_ALPHA:
+    case FIN_BICG_ALPHA:
+      out = cdiv0(rho, s0);
+      sc[S_ALPHA] = out.x; sc[S_ALPHA + 1] = out.y;
+      break;
+    case FIN_COCG_BETA:
+      out = cdiv0(s0, rho);
+      sc[S_BETA] = out.x; sc[S_BETA + 1] = out.y;
+      sc[S_RHO] = s0.x; sc[S_RHO + 1] = s0.y;
+      sc[S_RR] = s1.x;
+      if (rs.hist) rs.hist[rs.it] = sqrt(s1.x);
+      break;
+    case FIN_RESID:
+      sc[S_TRUE] = s0.x;
+      sc[S_RHO] = s1.x; sc[S_RHO + 1] = s1.y;
+      break;
+    case FIN_NORM:
+      sc[S_BNORM] = s0.x;
+      break;
+    case FIN_BICG_OMEGA:
+      out = s1.x == 0.0 ? make_double2(0.0, 0.0) : make_double2(s0.x / s1.x, -s0.y / s1.x);
+      sc[S_OMEGA] = out.x; sc[S_OMEGA + 1] = out.y;
+      break;
+    case FIN_BICG_BETA: {
+      const double2 al = make_double2(sc[S_ALPHA], sc[S_ALPHA + 1]);
+      const double2 om = make_double2(sc[S_OMEGA], sc[S_OMEGA + 1]);
+      out = cmuld(cdiv0(s0, rho), cdiv0(al, om));
+      sc[S_BETA] = out.x; sc[S_BETA + 1] = out.y;
+      sc[S_RHO] = s0.x; sc[S_RHO + 1] = s0.y;
+      sc[S_RR] = s1.x;
+      if (rs.hist) rs.hist[rs.it] = sqrt(s1.x);
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// Block sums of (v0, v1) -> partials; the last block to arrive sums all partials in
+// block order (deterministic) and finalises the scalars.  Must be called by every
+// thread of every block.
+template <int TB>
+__device__ __forceinline__ void reduce_finalize(const RedSite& rs, int G, double2 v0, double2 v1) {
+  v0 = block_sum<TB>(v0);
+  v1 = block_sum<TB>(v1);
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    const int lb = xcd_block(G);
+    rs.part[lb] = v0;
+    rs.part[G + lb] = v1;
+    __threadfence();
+    const unsigned ticket = atomicAdd(rs.counter, 1u);
+    is_last = ticket == (unsigned)(G - 1);
+  }
+  __syncthreads();
+  if (is_last) {
+    __threadfence();
+    double2 a = make_double2(0, 0), b = make_double2(0, 0);
+    for (int i = threadIdx.x; i < G; i += TB) {
+      const double2 p = rs.part[i], q = rs.part[G + i];
+      a.x += p.x; a.y += p.y;
+      b.x += q.x; b.y += q.y;
+    }
+    a = block_sum<TB>(a);
+    b = block_sum<TB>(b);
+    if (threadIdx.x == 0) {
+      finalize(rs, a, b);
+      *rs.counter = 0u;
+    }
+  }
+}
+
+__device__ __forceinline__ double2 scal2(const double* sc, int slot) {
+  return make_double2(sc[slot], sc[slot + 1]);
 }
 
 }  // namespace sucem

@@ -11,9 +11,10 @@
 // residual replaced by it and the recursion restarted from it).  All dot
 // products accumulate in FP64.
 //
-// Every scalar (alpha, beta, ...) is derived on the device by each block from the
-// per-block partial sums written by the previous kernel, so an iteration needs
-// no host round trip; the host only polls the residual history every
+// Scalars (alpha, beta, rho, omega) live on the device.  Every reducing kernel
+// writes per-block partials; the last block to finish sums them in block order
+// (deterministic) and updates the scalars, so an iteration is a fixed sequence of
+// launches with no host round trip.  The host polls the residual history every
 // `check_every` iterations.
 #include <chrono>
 #include <cmath>
@@ -27,15 +28,6 @@ namespace {
 
 constexpr int TB = 256;
 
-__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
-  double den = b.x * b.x + b.y * b.y;
-  return make_double2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
-}
-__device__ __forceinline__ double2 cmuld(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// rows [start, end) of logical block lb under the XCD-aware partition
 struct Range { int start, end; };
 __device__ __forceinline__ Range block_rows(int n, int G) {
   int lb = xcd_block(G);
@@ -46,11 +38,6 @@ __device__ __forceinline__ Range block_rows(int n, int G) {
   return r;
 }
 
-template <typename C>
-__device__ __forceinline__ C from_d(double2 v);
-template <> __device__ __forceinline__ float2 from_d<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
-template <> __device__ __forceinline__ double2 from_d<double2>(double2 v) { return v; }
-
 // ---- common vector kernels ------------------------------------------------------
 template <typename C>
 __global__ void k_zero_c(int64_t n, C* v) {
@@ -59,46 +46,48 @@ __global__ void k_zero_c(int64_t n, C* v) {
     v[i] = from_d<C>(make_double2(0, 0));
 }
 
-// part[lb] = (sum |b|^2 over unmasked rows, 0)
+__global__ void k_zero_u32(int n, unsigned* p) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+__global__ void k_zero_f64(int n, double* p) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+// |b|^2 over unmasked rows
 __global__ __launch_bounds__(TB) void k_norm2(int n, int G, const double2* b, const uint8_t* mask,
-                                              double2* part) {
+                                              RedSite rs) {
   Range rr = block_rows(n, G);
   double2 s = make_double2(0, 0);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB)
     if (!(mask && mask[i])) s.x += b[i].x * b[i].x + b[i].y * b[i].y;
-  s = block_sum<TB>(s);
-  if (threadIdx.x == 0) part[xcd_block(G)] = s;
+  reduce_finalize<TB>(rs, G, s, make_double2(0, 0));
 }
 
-// r = (b - Ax) (masked rows zero), stored in precision of C; rlo_dot partial = sum r (dinv r)
-// (bilinear when !CONJ; conj(rt) r when CONJ with rt = r), true |r|^2 partial in FP64.
+// r = b - Ax (masked rows zero), stored in the working precision.  Reduces the
+// FP64 |r|^2 and rho of the stored residual: r.(dinv r) for COCG, |r|^2 otherwise.
 template <typename C>
 __global__ __launch_bounds__(TB) void k_residual(int n, int G, const double2* b, const double2* ax,
                                                  const uint8_t* mask, const C* dinv, C* r,
-                                                 double2* part_true, double2* part_rho, int cocg) {
+                                                 int cocg, RedSite rs) {
   Range rr = block_rows(n, G);
   double2 tr = make_double2(0, 0), rho = make_double2(0, 0);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
     double2 v = make_double2(b[i].x - ax[i].x, b[i].y - ax[i].y);
     if (mask && mask[i]) v = make_double2(0, 0);
     tr.x += v.x * v.x + v.y * v.y;
-    C rl = from_d<C>(v);
+    const C rl = from_d<C>(v);
     r[i] = rl;
-    double2 rd = to_d(rl);
+    const double2 rd = to_d(rl);
     if (cocg) {
-      double2 z = to_d(cmul(dinv[i], rl));
+      const double2 z = to_d(cmul(dinv[i], rl));
       rho.x += rd.x * z.x - rd.y * z.y;
       rho.y += rd.x * z.y + rd.y * z.x;
     } else {
       rho.x += rd.x * rd.x + rd.y * rd.y;
     }
   }
-  tr = block_sum<TB>(tr);
-  rho = block_sum<TB>(rho);
-  if (threadIdx.x == 0) {
-    part_true[xcd_block(G)] = tr;
-    if (part_rho) part_rho[xcd_block(G)] = rho;
-  }
+  reduce_finalize<TB>(rs, G, tr, rho);
 }
 
 // x64 += xlo; xlo = 0  (FP32 reliable update)
@@ -106,7 +95,7 @@ template <typename C>
 __global__ void k_accum(int n, int G, C* xlo, double2* x64) {
   Range rr = block_rows(n, G);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
-    double2 a = to_d(xlo[i]);
+    const double2 a = to_d(xlo[i]);
     x64[i].x += a.x;
     x64[i].y += a.y;
     xlo[i] = from_d<C>(make_double2(0, 0));
@@ -114,179 +103,123 @@ __global__ void k_accum(int n, int G, C* xlo, double2* x64) {
 }
 
 template <typename C>
-__global__ void k_copy_cast(int n, int G, const C* src, C* dst) {
+__global__ void k_copy(int n, int G, const C* src, C* dst) {
   Range rr = block_rows(n, G);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) dst[i] = src[i];
 }
 
-template <typename C>
-__global__ void k_store_x(int n, int G, const C* xlo, double2* x64) {
-  Range rr = block_rows(n, G);
-  for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) x64[i] = to_d(xlo[i]);
-}
-
 // ---- COCG -------------------------------------------------------------------
-// p = dinv r (start)
+// p = dinv r
 template <typename C>
 __global__ void k_cocg_start(int n, int G, const C* dinv, const C* r, C* p) {
   Range rr = block_rows(n, G);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) p[i] = cmul(dinv[i], r[i]);
 }
 
-// alpha = rho / mu ; x += alpha p ; r -= alpha q ; partials of rho' = r.(dinv r), |r|^2
+// x += alpha p ; r -= alpha q ; reduce rho' = r.(dinv r) and |r|^2 (-> beta)
 template <typename C>
-__global__ __launch_bounds__(TB) void k_cocg_xr(int n, int G, const double2* prho_cur,
-                                                const double2* pmu, C* x, C* r, const C* p,
-                                                const C* q, const C* dinv, double2* prho_next,
-                                                double2* prr_next) {
-  const double2 rho = sum_partials<TB>(prho_cur, G);
-  const double2 mu = sum_partials<TB>(pmu, G);
-  const double2 al = cdiv(rho, mu);
-  const C a = from_d<C>(al);
+__global__ __launch_bounds__(TB) void k_cocg_xr(int n, int G, const double* scal, C* x, C* r,
+                                                const C* p, const C* q, const C* dinv,
+                                                RedSite rs) {
+  const C a = from_d<C>(scal2(scal, S_ALPHA));
   Range rr = block_rows(n, G);
   double2 nr = make_double2(0, 0), n2 = make_double2(0, 0);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
-    C pi = p[i], qi = q[i], xi = x[i], ri = r[i];
-    C ap = cmul(a, pi), aq = cmul(a, qi);
+    const C pi = p[i], qi = q[i];
+    C xi = x[i], ri = r[i];
+    const C ap = cmul(a, pi), aq = cmul(a, qi);
     xi.x += ap.x; xi.y += ap.y;
     ri.x -= aq.x; ri.y -= aq.y;
     x[i] = xi;
     r[i] = ri;
-    double2 rd = to_d(ri), z = to_d(cmul(dinv[i], ri));
+    const double2 rd = to_d(ri), z = to_d(cmul(dinv[i], ri));
     nr.x += rd.x * z.x - rd.y * z.y;
     nr.y += rd.x * z.y + rd.y * z.x;
     n2.x += rd.x * rd.x + rd.y * rd.y;
   }
-  nr = block_sum<TB>(nr);
-  n2 = block_sum<TB>(n2);
-  if (threadIdx.x == 0) {
-    prho_next[xcd_block(G)] = nr;
-    prr_next[xcd_block(G)] = n2;
-  }
+  reduce_finalize<TB>(rs, G, nr, n2);
 }
 
-// beta = rho'/rho ; p = dinv r + beta p ; hist[it] = |r|
+// p = dinv r + beta p
 template <typename C>
-__global__ __launch_bounds__(TB) void k_cocg_p(int n, int G, const double2* prho_cur,
-                                               const double2* prho_next, const double2* prr_next,
-                                               const C* dinv, const C* r, C* p, double* hist,
-                                               int it) {
-  const double2 rho = sum_partials<TB>(prho_cur, G);
-  const double2 rn = sum_partials<TB>(prho_next, G);
-  const double2 be = cdiv(rn, rho);
-  if (blockIdx.x == 0) {
-    const double2 r2 = sum_partials<TB>(prr_next, G);
-    if (threadIdx.x == 0) hist[it] = sqrt(r2.x);
-  }
-  const C b = from_d<C>(be);
+__global__ void k_cocg_p(int n, int G, const double* scal, const C* dinv, const C* r, C* p) {
+  const C b = from_d<C>(scal2(scal, S_BETA));
   Range rr = block_rows(n, G);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
-    C z = cmul(dinv[i], r[i]), pi = p[i];
-    C bp = cmul(b, pi);
+    const C z = cmul(dinv[i], r[i]);
+    const C bp = cmul(b, p[i]);
     p[i] = mk(z.x + bp.x, z.y + bp.y);
   }
 }
 
 // ---- BiCGStab (preconditioned; SciPy's formulation) ----------------------------
-// state[0..1] rho (parity), state[2] alpha, state[3] omega
+// p = r + beta (p - omega v) (p = r on a restart) ; phat = dinv p
 template <typename C>
-__global__ __launch_bounds__(TB) void k_bicg_p(int n, int G, int it, int restart,
-                                               const double2* prho, double2* state, const C* r,
-                                               C* p, const C* v, const C* dinv, C* phat) {
-  const double2 rho = sum_partials<TB>(prho, G);
-  double2 beta = make_double2(0, 0);
-  if (!restart) {
-    const double2 rho_prev = state[(it + 1) & 1];
-    beta = cmuld(cdiv(rho, rho_prev), cdiv(state[2], state[3]));
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) state[it & 1] = rho;
-  const C b = from_d<C>(beta), om = from_d<C>(state[3]);
+__global__ void k_bicg_p(int n, int G, int restart, const double* scal, const C* r, C* p,
+                         const C* v, const C* dinv, C* phat) {
+  const C b = from_d<C>(scal2(scal, S_BETA)), om = from_d<C>(scal2(scal, S_OMEGA));
   Range rr = block_rows(n, G);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
-    C ri = r[i];
-    C pv;
+    const C ri = r[i];
+    C pv = ri;
     if (!restart) {
-      C pi = p[i], ov = cmul(om, v[i]);
-      C t = mk(pi.x - ov.x, pi.y - ov.y);
-      C bt = cmul(b, t);
+      const C pi = p[i], ov = cmul(om, v[i]);
+      const C t = mk(pi.x - ov.x, pi.y - ov.y);
+      const C bt = cmul(b, t);
       pv = mk(ri.x + bt.x, ri.y + bt.y);
-    } else {
-      pv = ri;
     }
     p[i] = pv;
     phat[i] = cmul(dinv[i], pv);
   }
 }
 
-// alpha = rho / (rt^H v) ; s = r - alpha v ; shat = dinv s
+// s = r - alpha v ; shat = dinv s
 template <typename C>
-__global__ __launch_bounds__(TB) void k_bicg_s(int n, int G, int it, const double2* prho,
-                                               const double2* pv, double2* state, const C* r,
-                                               const C* v, const C* dinv, C* s, C* shat) {
-  const double2 rho = sum_partials<TB>(prho, G);
-  const double2 rv = sum_partials<TB>(pv, G);
-  const double2 al = cdiv(rho, rv);
-  if (blockIdx.x == 0 && threadIdx.x == 0) state[2] = al;
-  const C a = from_d<C>(al);
+__global__ void k_bicg_s(int n, int G, const double* scal, const C* r, const C* v, const C* dinv,
+                         C* s, C* shat) {
+  const C a = from_d<C>(scal2(scal, S_ALPHA));
   Range rr = block_rows(n, G);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
-    C av = cmul(a, v[i]), ri = r[i];
-    C si = mk(ri.x - av.x, ri.y - av.y);
+    const C av = cmul(a, v[i]), ri = r[i];
+    const C si = mk(ri.x - av.x, ri.y - av.y);
     s[i] = si;
     shat[i] = cmul(dinv[i], si);
   }
 }
 
-// omega = (t^H s)/(t^H t) ; x += alpha phat + omega shat ; r = s - omega t ;
-// partials rho' = rt^H r, |r|^2 ; hist
+// x += alpha phat + omega shat ; r = s - omega t ; reduce conj(rt).r and |r|^2 (-> beta)
 template <typename C>
-__global__ __launch_bounds__(TB) void k_bicg_x(int n, int G, int it, const double2* pt,
-                                               double2* state, C* x, C* r, const C* s,
-                                               const C* t, const C* phat, const C* shat,
-                                               const C* rt, double2* prho_next,
-                                               double2* prr_next) {
-  // pt[lb] = sum conj(s) t = conj(t^H s) ; pt[G + lb] = |t|^2
-  const double2 d1 = sum_partials<TB>(pt, G);
-  const double2 d2 = sum_partials<TB>(pt + G, G);
-  const double2 om = make_double2(d1.x / d2.x, -d1.y / d2.x);
-  if (blockIdx.x == 0 && threadIdx.x == 0) state[3] = om;
-  const C o = from_d<C>(om), a = from_d<C>(state[2]);
+__global__ __launch_bounds__(TB) void k_bicg_x(int n, int G, const double* scal, C* x, C* r,
+                                               const C* s, const C* t, const C* phat,
+                                               const C* shat, const C* rt, RedSite rs) {
+  const C o = from_d<C>(scal2(scal, S_OMEGA)), a = from_d<C>(scal2(scal, S_ALPHA));
   Range rr = block_rows(n, G);
   double2 nr = make_double2(0, 0), n2 = make_double2(0, 0);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
-    C ap = cmul(a, phat[i]), os = cmul(o, shat[i]), xi = x[i];
+    const C ap = cmul(a, phat[i]), os = cmul(o, shat[i]);
+    C xi = x[i];
     xi.x += ap.x + os.x;
     xi.y += ap.y + os.y;
     x[i] = xi;
-    C ot = cmul(o, t[i]), si = s[i];
-    C ri = mk(si.x - ot.x, si.y - ot.y);
+    const C ot = cmul(o, t[i]), si = s[i];
+    const C ri = mk(si.x - ot.x, si.y - ot.y);
     r[i] = ri;
-    double2 rd = to_d(ri), rtd = to_d(rt[i]);
+    const double2 rd = to_d(ri), rtd = to_d(rt[i]);
     nr.x += rtd.x * rd.x + rtd.y * rd.y;   // conj(rt) r
     nr.y += rtd.x * rd.y - rtd.y * rd.x;
     n2.x += rd.x * rd.x + rd.y * rd.y;
   }
-  nr = block_sum<TB>(nr);
-  n2 = block_sum<TB>(n2);
-  if (threadIdx.x == 0) {
-    prho_next[xcd_block(G)] = nr;
-    prr_next[xcd_block(G)] = n2;
-  }
-}
-
-__global__ __launch_bounds__(TB) void k_hist(int G, const double2* prr, double* hist, int it) {
-  const double2 r2 = sum_partials<TB>(prr, G);
-  if (threadIdx.x == 0) hist[it] = sqrt(r2.x);
+  reduce_finalize<TB>(rs, G, nr, n2);
 }
 
 // ---- host driver ----------------------------------------------------------------
 // Operator abstraction over the driven system and the generic CSR matrix.
 struct Op {
-  Context* c;
-  int64_t n;
-  int G, L;
-  const uint8_t* mask;
-  virtual void apply_lo(const void* x, void* y, const void* w, bool conj, double2* part) = 0;
+  Context* c = nullptr;
+  int64_t n = 0;
+  SpmvCfg flo, fhi;
+  const uint8_t* mask = nullptr;
+  virtual void apply_lo(const void* x, void* y, const void* w, int epi, const RedSite& rs) = 0;
   virtual void apply_hi(const double2* x, double2* y) = 0;
   virtual void dinv_lo(void* d) = 0;
   virtual ~Op() {}
@@ -294,73 +227,78 @@ struct Op {
 
 template <typename R>
 struct SysOp : Op {
-  const System* sys;
-  Coef k0;
+  const System* sys = nullptr;
+  Coef k;
   bool identity_prec = false;
-  void apply_lo(const void* x, void* y, const void* w, bool conj, double2* part) override {
-    sys_apply<R>(c, sys, k0, mask, x, y, w, conj, part, G, L);
+  void apply_lo(const void* x, void* y, const void* w, int epi, const RedSite& rs) override {
+    sys_apply<R>(c, sys, k, mask, x, y, w, epi, rs, flo);
   }
   void apply_hi(const double2* x, double2* y) override {
-    sys_apply<double>(c, sys, k0, mask, x, y, nullptr, false, nullptr, G, L);
+    RedSite none{};
+    sys_apply<double>(c, sys, k, mask, x, y, nullptr, 0, none, fhi);
   }
   void dinv_lo(void* d) override {
     if (identity_prec) csr_dinv<R>(c, nullptr, mask, true, d, n);
-    else sys_dinv<R>(c, sys, k0, mask, d);
+    else sys_dinv<R>(c, sys, k, mask, d);
   }
 };
 
 template <typename R>
 struct CsrOp : Op {
-  const Csr* A;
-  bool identity_prec;
-  void apply_lo(const void* x, void* y, const void* w, bool conj, double2* part) override {
-    csr_apply<R>(c, A, mask, x, y, w, conj, part, G, L);
+  const Csr* A = nullptr;
+  bool identity_prec = false;
+  void apply_lo(const void* x, void* y, const void* w, int epi, const RedSite& rs) override {
+    csr_apply<R>(c, A, mask, x, y, w, epi, rs, flo);
   }
   void apply_hi(const double2* x, double2* y) override {
-    csr_apply<double>(c, A, mask, x, y, nullptr, false, nullptr, G, L);
+    RedSite none{};
+    csr_apply<double>(c, A, mask, x, y, nullptr, 0, none, fhi);
   }
   void dinv_lo(void* d) override { csr_dinv<R>(c, A, mask, identity_prec, d, n); }
 };
 
-double sum_host(Context* c, const double2* part_d, int G, int comp = 0) {
-  double* h = c->host_scratch(2 * G);
-  SUCEM_HIP(hipMemcpyAsync(h, part_d, sizeof(double2) * G, hipMemcpyDeviceToHost, c->stream));
-  c->sync();
-  double s = 0;
-  for (int i = 0; i < G; ++i) s += h[2 * i + comp];
-  return s;
-}
-
-double read_one(Context* c, const double* d) {
+double read_scalar(Context* c, const double* d) {
   double* h = c->host_scratch(1);
   SUCEM_HIP(hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   c->sync();
   return h[0];
 }
 
+enum Site { SITE_NORM = 0, SITE_RESID, SITE_SPMV1, SITE_SPMV2, SITE_UPDATE, SITE_COUNT };
+
 template <typename R>
 void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats& st,
             double* hist_h, int64_t hist_cap) {
   using C = typename C2<R>::T;
   Context* c = op.c;
-  const int n = (int)op.n, G = op.G;
+  const int n = (int)op.n;
+  const int G = op_grid(c, n, 8);            // vector kernels
+  const int Gm = std::max(G, op.flo.G);      // partial buffer size
   const bool lo = sizeof(R) == 4;
   const bool cocg = o.method == SUCEM_METHOD_COCG;
   auto t0 = std::chrono::steady_clock::now();
   st = SolveStats();
 
-  DBuf<double2> part(8 * G);               // scratch partials
-  double2* pnorm = part.p;                 // G
-  double2* ptrue = part.p + G;             // G
-  double2* pmu = part.p + 2 * G;           // 2G (SpMV epilogue)
-  double2* prho[2] = {part.p + 4 * G, part.p + 5 * G};
-  double2* prr[2] = {part.p + 6 * G, part.p + 7 * G};
+  DBuf<double2> part((int64_t)SITE_COUNT * 2 * Gm);
+  DBuf<unsigned> counters(SITE_COUNT);
+  DBuf<double> scal(S_COUNT);
   DBuf<double> hist(o.max_iter + 1);
-  DBuf<double2> state(4);
+  SUCEM_LAUNCH(k_zero_u32, 1, 64, 0, c->stream, (int)SITE_COUNT, counters.p);
+  SUCEM_LAUNCH(k_zero_f64, 1, 64, 0, c->stream, (int)S_COUNT, scal.p);
+  auto site = [&](int s_, int mode, int it = 0) {
+    RedSite r;
+    r.part = part.p + (int64_t)s_ * 2 * Gm;
+    r.counter = counters.p + s_;
+    r.scal = scal.p;
+    r.hist = hist.p;
+    r.mode = mode;
+    r.it = it;
+    return r;
+  };
+
   DBuf<C> dinv(n), r(n), p(n), q(n);
   DBuf<C> xlo_buf(lo ? n : 0);
   DBuf<double2> ax(n);
-  // BiCGStab extras
   DBuf<C> rt(cocg ? 0 : n), v(cocg ? 0 : n), phat(cocg ? 0 : n), s(cocg ? 0 : n),
       shat(cocg ? 0 : n), t(cocg ? 0 : n);
   C* xlo = lo ? xlo_buf.p : (C*)x;
@@ -368,53 +306,46 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   op.dinv_lo(dinv.p);
   SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, x);
   if (lo) SUCEM_LAUNCH((k_zero_c<C>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, xlo);
-  SUCEM_LAUNCH(k_norm2, G, TB, 0, c->stream, n, G, b, op.mask, pnorm);
-  const double bnorm = std::sqrt(sum_host(c, pnorm, G));
+  SUCEM_LAUNCH(k_norm2, G, TB, 0, c->stream, n, G, b, op.mask, site(SITE_NORM, FIN_NORM));
+  const double bnorm = std::sqrt(read_scalar(c, scal.p + S_BNORM));
   if (bnorm == 0.0) {
     st.status = 0;
     st.rel_residual = 0;
+    c->sync();
     return;
   }
-  // r = b - A x0 (x0 = 0)
+  // r = b - A x0 with x0 = 0
   SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, ax.p);
-  SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p, ptrue,
-               prho[0], cocg ? 1 : 0);
+  SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p,
+               cocg ? 1 : 0, site(SITE_RESID, FIN_RESID));
+  if (cocg) SUCEM_LAUNCH((k_cocg_start<C>), G, TB, 0, c->stream, n, G, dinv.p, r.p, p.p);
+  else SUCEM_LAUNCH((k_copy<C>), G, TB, 0, c->stream, n, G, r.p, rt.p);
   double last_true = 1.0;
-  if (cocg) {
-    SUCEM_LAUNCH((k_cocg_start<C>), G, TB, 0, c->stream, n, G, dinv.p, r.p, p.p);
-  } else {
-    SUCEM_LAUNCH((k_copy_cast<C>), G, TB, 0, c->stream, n, G, r.p, rt.p);
-    // rho_0 = rt^H r = |r|^2 already in prho[0]
-  }
   int it = 0;
   int restart = 1;
-  st.status = o.max_iter;   // not converged unless we say otherwise
+  st.status = o.max_iter;
   double rel = 1.0;
   for (it = 0; it < o.max_iter; ++it) {
     if (cocg) {
-      op.apply_lo(p.p, q.p, p.p, false, pmu);
-      SUCEM_LAUNCH((k_cocg_xr<C>), G, TB, 0, c->stream, n, G, prho[it & 1], pmu, xlo, r.p, p.p, q.p,
-                   dinv.p, prho[(it + 1) & 1], prr[(it + 1) & 1]);
-      SUCEM_LAUNCH((k_cocg_p<C>), G, TB, 0, c->stream, n, G, prho[it & 1], prho[(it + 1) & 1],
-                   prr[(it + 1) & 1], dinv.p, r.p, p.p, hist.p, it);
+      op.apply_lo(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
+      SUCEM_LAUNCH((k_cocg_xr<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, p.p, q.p, dinv.p,
+                   site(SITE_UPDATE, FIN_COCG_BETA, it));
+      SUCEM_LAUNCH((k_cocg_p<C>), G, TB, 0, c->stream, n, G, scal.p, dinv.p, r.p, p.p);
       st.spmv_lo += 1;
     } else {
-      SUCEM_LAUNCH((k_bicg_p<C>), G, TB, 0, c->stream, n, G, it, restart, prho[it & 1], state.p,
-                   r.p, p.p, v.p, dinv.p, phat.p);
+      SUCEM_LAUNCH((k_bicg_p<C>), G, TB, 0, c->stream, n, G, restart, scal.p, r.p, p.p, v.p, dinv.p,
+                   phat.p);
       restart = 0;
-      op.apply_lo(phat.p, v.p, rt.p, true, pmu);
-      // pmu holds sum conj(rt) v
-      SUCEM_LAUNCH((k_bicg_s<C>), G, TB, 0, c->stream, n, G, it, prho[it & 1], pmu, state.p, r.p,
-                   v.p, dinv.p, s.p, shat.p);
-      op.apply_lo(shat.p, t.p, s.p, true, pmu);
-      SUCEM_LAUNCH((k_bicg_x<C>), G, TB, 0, c->stream, n, G, it, pmu, state.p, xlo, r.p, s.p, t.p,
-                   phat.p, shat.p, rt.p, prho[(it + 1) & 1], prr[(it + 1) & 1]);
-      SUCEM_LAUNCH(k_hist, 1, TB, 0, c->stream, G, prr[(it + 1) & 1], hist.p, it);
+      op.apply_lo(phat.p, v.p, rt.p, 2, site(SITE_SPMV1, FIN_BICG_ALPHA));
+      SUCEM_LAUNCH((k_bicg_s<C>), G, TB, 0, c->stream, n, G, scal.p, r.p, v.p, dinv.p, s.p, shat.p);
+      op.apply_lo(shat.p, t.p, s.p, 2, site(SITE_SPMV2, FIN_BICG_OMEGA));
+      SUCEM_LAUNCH((k_bicg_x<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, s.p, t.p, phat.p,
+                   shat.p, rt.p, site(SITE_UPDATE, FIN_BICG_BETA, it));
       st.spmv_lo += 2;
     }
     const bool check = ((it + 1) % o.check_every) == 0 || it + 1 == o.max_iter;
     if (!check) continue;
-    rel = read_one(c, hist.p + it) / bnorm;
+    rel = read_scalar(c, hist.p + it) / bnorm;
     if (!std::isfinite(rel)) {
       st.status = -1;   // breakdown
       break;
@@ -426,9 +357,9 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     op.apply_hi(x, ax.p);
     st.spmv_hi += 1;
     st.reliable_updates += 1;
-    SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p, ptrue,
-                 cocg ? prho[(it + 1) & 1] : nullptr, cocg ? 1 : 0);
-    const double true_rel = std::sqrt(sum_host(c, ptrue, G)) / bnorm;
+    SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p,
+                 cocg ? 1 : 0, site(SITE_RESID, FIN_RESID));
+    const double true_rel = std::sqrt(read_scalar(c, scal.p + S_TRUE)) / bnorm;
     rel = true_rel;
     last_true = true_rel;
     if (true_rel <= o.tol) {
@@ -436,15 +367,12 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
       ++it;
       break;
     }
+    // restart from the replaced residual (rho was recomputed by k_residual): keeping
+    // the old direction after a residual replacement stalls these indefinite systems
     if (cocg) {
-      // restart the direction from the replaced residual: keeping the old direction
-      // after a residual replacement stalls COCG on these indefinite systems
       SUCEM_LAUNCH((k_cocg_start<C>), G, TB, 0, c->stream, n, G, dinv.p, r.p, p.p);
     } else {
-      // restart BiCGStab from the replaced residual (rt = r, rho = |r|^2)
-      SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p, ptrue,
-                   prho[(it + 1) & 1], 0);
-      SUCEM_LAUNCH((k_copy_cast<C>), G, TB, 0, c->stream, n, G, r.p, rt.p);
+      SUCEM_LAUNCH((k_copy<C>), G, TB, 0, c->stream, n, G, r.p, rt.p);
       restart = 1;
     }
   }
@@ -453,55 +381,60 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     if (lo) SUCEM_LAUNCH((k_accum<C>), G, TB, 0, c->stream, n, G, xlo, x);
     op.apply_hi(x, ax.p);
     st.spmv_hi += 1;
-    SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p, ptrue,
-                 nullptr, 1);
-    rel = std::sqrt(sum_host(c, ptrue, G)) / bnorm;
+    SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p, 1,
+                 site(SITE_RESID, FIN_RESID));
+    rel = std::sqrt(read_scalar(c, scal.p + S_TRUE)) / bnorm;
     if (st.status > 0 && rel <= o.tol) st.status = 0;
+    if (it > o.max_iter) it = o.max_iter;
   }
   st.iterations = it;
   st.rel_residual = rel;
   if (hist_h && hist_cap > 0) {
-    int64_t m = std::min<int64_t>(hist_cap, it);
+    const int64_t m = std::min<int64_t>(hist_cap, it);
     SUCEM_HIP(hipMemcpyAsync(hist_h, hist.p, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
-  }
-  c->sync();
-  if (hist_h) {
-    int64_t m = std::min<int64_t>(hist_cap, it);
+    c->sync();
     for (int64_t i = 0; i < m; ++i) hist_h[i] /= bnorm;
   }
+  c->sync();
   st.elapsed_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-}  // namespace
-
-void solve_system(Context* c, const System* sys, Coef k0, const double2* b, double2* x,
-                  const SolveOpts& o, SolveStats& st, double* hist_h, int64_t hist_cap) {
-  SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S and T must be assembled");
+void check_opts(const SolveOpts& o) {
   SUCEM_CHECK(o.method == SUCEM_METHOD_COCG || o.method == SUCEM_METHOD_BICGSTAB,
               SUCEM_ERR_UNSUPPORTED, "method must be COCG or BiCGStab");
-  SUCEM_CHECK(o.precision == 32 || o.precision == 64, SUCEM_ERR_INVALID, "precision must be 32 or 64");
+  SUCEM_CHECK(o.precision == 32 || o.precision == 64, SUCEM_ERR_INVALID,
+              "precision must be 32 or 64");
   SUCEM_CHECK(o.max_iter > 0 && o.check_every > 0, SUCEM_ERR_INVALID, "bad iteration limits");
+}
+
+}  // namespace
+
+void solve_system(Context* c, const System* sys, Coef k, const double2* b, double2* x,
+                  const SolveOpts& o, SolveStats& st, double* hist_h, int64_t hist_cap) {
+  SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S and T must be assembled");
+  check_opts(o);
   const Space* s = sys->space;
   auto fill = [&](Op& op) {
     op.c = c;
     op.n = s->ndofs;
-    op.G = op_grid(c, s->ndofs);
-    op.L = lanes_for(s->ndofs, s->nnz);
     op.mask = o.eliminate_pec ? s->pec.p : nullptr;
+    op.fhi = sys_cfg<double>(c, sys);
   };
   if (o.precision == 32) {
     SysOp<float> op;
     fill(op);
+    op.flo = sys_cfg<float>(c, sys);
     op.sys = sys;
-    op.k0 = k0;
+    op.k = k;
     op.identity_prec = o.precond == SUCEM_PRECOND_NONE;
     krylov<float>(op, b, x, o, st, hist_h, hist_cap);
   } else {
     SysOp<double> op;
     fill(op);
+    op.flo = op.fhi;
     op.sys = sys;
-    op.k0 = k0;
+    op.k = k;
     op.identity_prec = o.precond == SUCEM_PRECOND_NONE;
     krylov<double>(op, b, x, o, st, hist_h, hist_cap);
   }
@@ -509,16 +442,12 @@ void solve_system(Context* c, const System* sys, Coef k0, const double2* b, doub
 
 void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const SolveOpts& o,
                SolveStats& st, double* hist_h, int64_t hist_cap) {
-  SUCEM_CHECK(o.method == SUCEM_METHOD_COCG || o.method == SUCEM_METHOD_BICGSTAB,
-              SUCEM_ERR_UNSUPPORTED, "method must be COCG or BiCGStab");
-  SUCEM_CHECK(o.precision == 32 || o.precision == 64, SUCEM_ERR_INVALID, "precision must be 32 or 64");
-  SUCEM_CHECK(o.max_iter > 0 && o.check_every > 0, SUCEM_ERR_INVALID, "bad iteration limits");
+  check_opts(o);
   auto fill = [&](Op& op) {
     op.c = c;
     op.n = A->n;
-    op.G = op_grid(c, A->n);
-    op.L = lanes_for(A->n, A->nnz);
     op.mask = nullptr;
+    op.flo = op.fhi = csr_cfg(c, A);
   };
   if (o.precision == 32) {
     SUCEM_CHECK(A->val32.p, SUCEM_ERR_INVALID, "matrix has no FP32 copy");

@@ -4,6 +4,8 @@
 // streams the interleaved real (S, T) values of the shared pattern once and adds
 // the compact ABC rows, replacing the three CSR conversions and two sparse adds
 // of DrivenProblemABC.get_LHS_matrix (ProblemConfigurations/EMDrivenProblem.py:67-72).
+#include <cmath>
+
 #include "internal.hpp"
 #include "kernels.hpp"
 #include "ops.hpp"
@@ -17,17 +19,19 @@ template <typename R> struct ST2;
 template <> struct ST2<float> { using T = float2; };
 template <> struct ST2<double> { using T = double2; };
 
-// One group of L lanes per row; each XCD gets a contiguous block of rows.
-// Epilogue (optional): part[lb] = sum_r op(w_r) * y_r (op = conj if CONJ), and
-// part[G + lb] = sum_r |y_r|^2.
-template <typename R, int L, bool DOT, bool CONJ>
+// One group of L lanes per row, U loads per lane issued before any is used;
+// each XCD gets a contiguous block of rows.  Epilogue (EPI > 0): per-row
+// op(w_r) * y_r (op = conj when EPI == 2) and |y_r|^2, reduced over the grid
+// with the last block finalising the Krylov scalars.
+template <typename R, int L, int U, int EPI>
 __global__ __launch_bounds__(TB) void k_sys_spmv(
     int n, int G, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
     const typename ST2<R>::T* __restrict__ st, const int32_t* __restrict__ s0p,
     const int32_t* __restrict__ s0c, const R* __restrict__ s0v, R cs, R ct, R c0,
     const uint8_t* __restrict__ mask, const typename C2<R>::T* __restrict__ x,
-    typename C2<R>::T* __restrict__ y, const typename C2<R>::T* __restrict__ w, double2* part) {
+    typename C2<R>::T* __restrict__ y, const typename C2<R>::T* __restrict__ w, RedSite rs) {
   using C = typename C2<R>::T;
+  using V = typename ST2<R>::T;
   constexpr int RPB = TB / L;
   const int lb = xcd_block(G);
   int chunk = (n + G - 1) / G;
@@ -41,11 +45,24 @@ __global__ __launch_bounds__(TB) void k_sys_spmv(
     C as = mk(R(0), R(0)), at = mk(R(0), R(0)), a0 = mk(R(0), R(0));
     if (r < end) {
       const int j1 = rp[r + 1];
-      for (int j = rp[r] + lane; j < j1; j += L) {
-        const auto v = st[j];
-        const C xc = x[col[j]];
-        as.x += v.x * xc.x; as.y += v.x * xc.y;
-        at.x += v.y * xc.x; at.y += v.y * xc.y;
+      for (int jb = rp[r] + lane; jb < j1; jb += L * U) {
+        int cc[U];
+        V vv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int j = jb + u * L;
+          const bool ok = j < j1;
+          cc[u] = ok ? col[j] : -1;
+          vv[u] = ok ? st[j] : V{R(0), R(0)};
+        }
+        C xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = cc[u] >= 0 ? x[cc[u]] : mk(R(0), R(0));
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          as.x += vv[u].x * xv[u].x; as.y += vv[u].x * xv[u].y;
+          at.x += vv[u].y * xv[u].x; at.y += vv[u].y * xv[u].y;
+        }
       }
       if (s0p) {
         const int k1 = s0p[r + 1];
@@ -66,28 +83,21 @@ __global__ __launch_bounds__(TB) void k_sys_spmv(
       C yr = mk(cs * as.x + ct * at.x - c0 * a0.y, cs * as.y + ct * at.y + c0 * a0.x);
       if (mask && mask[r]) yr = mk(R(0), R(0));
       y[r] = yr;
-      if (DOT) {
+      if (EPI) {
         const double2 yd = to_d(yr);
         double2 wd = to_d(w[r]);
-        if (CONJ) wd.y = -wd.y;
+        if (EPI == 2) wd.y = -wd.y;
         d1.x += wd.x * yd.x - wd.y * yd.y;
         d1.y += wd.x * yd.y + wd.y * yd.x;
         d2.x += yd.x * yd.x + yd.y * yd.y;
       }
     }
   }
-  if (DOT) {
-    d1 = block_sum<TB>(d1);
-    d2 = block_sum<TB>(d2);
-    if (threadIdx.x == 0) {
-      part[lb] = d1;
-      part[G + lb] = d2;
-    }
-  }
+  if (EPI) reduce_finalize<TB>(rs, G, d1, d2);
 }
 
-// Generic complex CSR: y = A x with the same epilogue.
-template <typename R, int L, bool DOT, bool CONJ>
+// Generic complex CSR: y = A x with the same structure and epilogue.
+template <typename R, int L, int U, int EPI>
 __global__ __launch_bounds__(TB) void k_csr_spmv(int n, int G, const int32_t* __restrict__ rp,
                                                  const int32_t* __restrict__ col,
                                                  const typename C2<R>::T* __restrict__ val,
@@ -95,7 +105,7 @@ __global__ __launch_bounds__(TB) void k_csr_spmv(int n, int G, const int32_t* __
                                                  const typename C2<R>::T* __restrict__ x,
                                                  typename C2<R>::T* __restrict__ y,
                                                  const typename C2<R>::T* __restrict__ w,
-                                                 double2* part) {
+                                                 RedSite rs) {
   using C = typename C2<R>::T;
   constexpr int RPB = TB / L;
   const int lb = xcd_block(G);
@@ -110,10 +120,22 @@ __global__ __launch_bounds__(TB) void k_csr_spmv(int n, int G, const int32_t* __
     C acc = mk(R(0), R(0));
     if (r < end) {
       const int j1 = rp[r + 1];
-      for (int j = rp[r] + lane; j < j1; j += L) {
-        const C a = val[j], xc = x[col[j]];
-        acc.x += a.x * xc.x - a.y * xc.y;
-        acc.y += a.x * xc.y + a.y * xc.x;
+      for (int jb = rp[r] + lane; jb < j1; jb += L * U) {
+        int cc[U];
+        C aa[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int j = jb + u * L;
+          const bool ok = j < j1;
+          cc[u] = ok ? col[j] : -1;
+          aa[u] = ok ? val[j] : mk(R(0), R(0));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const C xc = cc[u] >= 0 ? x[cc[u]] : mk(R(0), R(0));
+          acc.x += aa[u].x * xc.x - aa[u].y * xc.y;
+          acc.y += aa[u].x * xc.y + aa[u].y * xc.x;
+        }
       }
     }
 #pragma unroll
@@ -124,24 +146,17 @@ __global__ __launch_bounds__(TB) void k_csr_spmv(int n, int G, const int32_t* __
     if (lane == 0 && r < end) {
       if (mask && mask[r]) acc = mk(R(0), R(0));
       y[r] = acc;
-      if (DOT) {
+      if (EPI) {
         const double2 yd = to_d(acc);
         double2 wd = to_d(w[r]);
-        if (CONJ) wd.y = -wd.y;
+        if (EPI == 2) wd.y = -wd.y;
         d1.x += wd.x * yd.x - wd.y * yd.y;
         d1.y += wd.x * yd.y + wd.y * yd.x;
         d2.x += yd.x * yd.x + yd.y * yd.y;
       }
     }
   }
-  if (DOT) {
-    d1 = block_sum<TB>(d1);
-    d2 = block_sum<TB>(d2);
-    if (threadIdx.x == 0) {
-      part[lb] = d1;
-      part[G + lb] = d2;
-    }
-  }
+  if (EPI) reduce_finalize<TB>(rs, G, d1, d2);
 }
 
 template <typename R>
@@ -292,9 +307,9 @@ __global__ __launch_bounds__(64) void k_rhs(int64_t nsrc, const double* pos, con
   }
 }
 
-template <typename R, int L>
+template <typename R, int L, int U>
 void launch_sys(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
-                void* y, const void* w, bool conj, double2* part, int G) {
+                void* y, const void* w, int epi, const RedSite& rs, int G) {
   const Space* s = sys->space;
   using C = typename C2<R>::T;
   const typename ST2<R>::T* st;
@@ -305,38 +320,42 @@ void launch_sys(Context* c, const System* sys, Coef k, const uint8_t* mask, cons
   const int32_t* s0p = has_s0 ? s->s0_ptr.p : nullptr;
   R cs = (R)k.s, ct = (R)k.t, c0 = (R)k.s0;
   int n = (int)s->ndofs;
-  if (!part) {
-    SUCEM_LAUNCH((k_sys_spmv<R, L, false, false>), G, TB, 0, c->stream, n, G, s->row_ptr.p,
-                 s->col.p, st, s0p, s->s0_col.p, s0v, cs, ct, c0, mask, (const C*)x, (C*)y,
-                 (const C*)w, part);
-  } else if (conj) {
-    SUCEM_LAUNCH((k_sys_spmv<R, L, true, true>), G, TB, 0, c->stream, n, G, s->row_ptr.p,
-                 s->col.p, st, s0p, s->s0_col.p, s0v, cs, ct, c0, mask, (const C*)x, (C*)y,
-                 (const C*)w, part);
-  } else {
-    SUCEM_LAUNCH((k_sys_spmv<R, L, true, false>), G, TB, 0, c->stream, n, G, s->row_ptr.p,
-                 s->col.p, st, s0p, s->s0_col.p, s0v, cs, ct, c0, mask, (const C*)x, (C*)y,
-                 (const C*)w, part);
-  }
+#define SYS_ARGS n, G, s->row_ptr.p, s->col.p, st, s0p, s->s0_col.p, s0v, cs, ct, c0, mask, \
+                 (const C*)x, (C*)y, (const C*)w, rs
+  if (epi == 0) SUCEM_LAUNCH((k_sys_spmv<R, L, U, 0>), G, TB, 0, c->stream, SYS_ARGS);
+  else if (epi == 1) SUCEM_LAUNCH((k_sys_spmv<R, L, U, 1>), G, TB, 0, c->stream, SYS_ARGS);
+  else SUCEM_LAUNCH((k_sys_spmv<R, L, U, 2>), G, TB, 0, c->stream, SYS_ARGS);
+#undef SYS_ARGS
 }
 
-template <typename R, int L>
+template <typename R, int L, int U>
 void launch_csr(Context* c, const Csr* A, const uint8_t* mask, const void* x, void* y,
-                const void* w, bool conj, double2* part, int G) {
+                const void* w, int epi, const RedSite& rs, int G) {
   using C = typename C2<R>::T;
   const C* val;
   if constexpr (sizeof(R) == 4) val = A->val32.p; else val = A->val64.p;
   int n = (int)A->n;
-  if (!part) {
-    SUCEM_LAUNCH((k_csr_spmv<R, L, false, false>), G, TB, 0, c->stream, n, G, A->row_ptr.p,
-                 A->col.p, val, mask, (const C*)x, (C*)y, (const C*)w, part);
-  } else if (conj) {
-    SUCEM_LAUNCH((k_csr_spmv<R, L, true, true>), G, TB, 0, c->stream, n, G, A->row_ptr.p,
-                 A->col.p, val, mask, (const C*)x, (C*)y, (const C*)w, part);
-  } else {
-    SUCEM_LAUNCH((k_csr_spmv<R, L, true, false>), G, TB, 0, c->stream, n, G, A->row_ptr.p,
-                 A->col.p, val, mask, (const C*)x, (C*)y, (const C*)w, part);
-  }
+#define CSR_ARGS n, G, A->row_ptr.p, A->col.p, val, mask, (const C*)x, (C*)y, (const C*)w, rs
+  if (epi == 0) SUCEM_LAUNCH((k_csr_spmv<R, L, U, 0>), G, TB, 0, c->stream, CSR_ARGS);
+  else if (epi == 1) SUCEM_LAUNCH((k_csr_spmv<R, L, U, 1>), G, TB, 0, c->stream, CSR_ARGS);
+  else SUCEM_LAUNCH((k_csr_spmv<R, L, U, 2>), G, TB, 0, c->stream, CSR_ARGS);
+#undef CSR_ARGS
+}
+
+template <typename R, int L>
+void sys_dispatch_u(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
+                    void* y, const void* w, int epi, const RedSite& rs, int G, int U) {
+  if (U <= 1) launch_sys<R, L, 1>(c, sys, k, mask, x, y, w, epi, rs, G);
+  else if (U == 2) launch_sys<R, L, 2>(c, sys, k, mask, x, y, w, epi, rs, G);
+  else launch_sys<R, L, 4>(c, sys, k, mask, x, y, w, epi, rs, G);
+}
+
+template <typename R, int L>
+void csr_dispatch_u(Context* c, const Csr* A, const uint8_t* mask, const void* x, void* y,
+                    const void* w, int epi, const RedSite& rs, int G, int U) {
+  if (U <= 1) launch_csr<R, L, 1>(c, A, mask, x, y, w, epi, rs, G);
+  else if (U == 2) launch_csr<R, L, 2>(c, A, mask, x, y, w, epi, rs, G);
+  else launch_csr<R, L, 4>(c, A, mask, x, y, w, epi, rs, G);
 }
 
 }  // namespace
@@ -349,48 +368,113 @@ int lanes_for(int64_t n, int64_t nnz) {
   return 32;
 }
 
-int op_grid(Context* c, int64_t n) {
-  // a multiple of 8 (one share per XCD); enough blocks to fill every CU
-  int64_t want = (int64_t)c->num_cus * 4;
-  int64_t rows_min = 256;  // at least a few rows per block
-  int64_t g = (n + rows_min - 1) / rows_min;
+int op_grid(Context* c, int64_t n, int per_cu) {
+  // a multiple of 8 (one share per XCD); at least a few rows per block
+  int64_t want = (int64_t)c->num_cus * per_cu;
+  int64_t g = (n + 63) / 64;
   if (g > want) g = want;
   g = (g + 7) / 8 * 8;
   if (g < 8) g = 8;
   return (int)g;
 }
 
+SpmvCfg default_cfg(Context* c, int64_t n, int64_t nnz) {
+  SpmvCfg f;
+  f.L = lanes_for(n, nnz);
+  double avg = n ? (double)nnz / (double)n : 1.0;
+  int u = (int)std::ceil(avg / f.L);
+  f.U = u <= 1 ? 1 : (u == 2 ? 2 : 4);
+  f.G = op_grid(c, n, 16);
+  return f;
+}
+
 template <typename R>
 void sys_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
-               void* y, const void* w, bool conj, double2* part, int G, int L) {
-  switch (L) {
-    case 4: launch_sys<R, 4>(c, sys, k, mask, x, y, w, conj, part, G); break;
-    case 8: launch_sys<R, 8>(c, sys, k, mask, x, y, w, conj, part, G); break;
-    case 16: launch_sys<R, 16>(c, sys, k, mask, x, y, w, conj, part, G); break;
-    case 32: launch_sys<R, 32>(c, sys, k, mask, x, y, w, conj, part, G); break;
-    default: launch_sys<R, 64>(c, sys, k, mask, x, y, w, conj, part, G); break;
+               void* y, const void* w, int epi, const RedSite& rs, const SpmvCfg& f) {
+  switch (f.L) {
+    case 4: sys_dispatch_u<R, 4>(c, sys, k, mask, x, y, w, epi, rs, f.G, f.U); break;
+    case 8: sys_dispatch_u<R, 8>(c, sys, k, mask, x, y, w, epi, rs, f.G, f.U); break;
+    case 16: sys_dispatch_u<R, 16>(c, sys, k, mask, x, y, w, epi, rs, f.G, f.U); break;
+    default: sys_dispatch_u<R, 32>(c, sys, k, mask, x, y, w, epi, rs, f.G, f.U); break;
   }
 }
 template void sys_apply<float>(Context*, const System*, Coef, const uint8_t*, const void*, void*,
-                               const void*, bool, double2*, int, int);
+                               const void*, int, const RedSite&, const SpmvCfg&);
 template void sys_apply<double>(Context*, const System*, Coef, const uint8_t*, const void*,
-                                void*, const void*, bool, double2*, int, int);
+                                void*, const void*, int, const RedSite&, const SpmvCfg&);
 
 template <typename R>
 void csr_apply(Context* c, const Csr* A, const uint8_t* mask, const void* x, void* y,
-               const void* w, bool conj, double2* part, int G, int L) {
-  switch (L) {
-    case 4: launch_csr<R, 4>(c, A, mask, x, y, w, conj, part, G); break;
-    case 8: launch_csr<R, 8>(c, A, mask, x, y, w, conj, part, G); break;
-    case 16: launch_csr<R, 16>(c, A, mask, x, y, w, conj, part, G); break;
-    case 32: launch_csr<R, 32>(c, A, mask, x, y, w, conj, part, G); break;
-    default: launch_csr<R, 64>(c, A, mask, x, y, w, conj, part, G); break;
+               const void* w, int epi, const RedSite& rs, const SpmvCfg& f) {
+  switch (f.L) {
+    case 4: csr_dispatch_u<R, 4>(c, A, mask, x, y, w, epi, rs, f.G, f.U); break;
+    case 8: csr_dispatch_u<R, 8>(c, A, mask, x, y, w, epi, rs, f.G, f.U); break;
+    case 16: csr_dispatch_u<R, 16>(c, A, mask, x, y, w, epi, rs, f.G, f.U); break;
+    default: csr_dispatch_u<R, 32>(c, A, mask, x, y, w, epi, rs, f.G, f.U); break;
   }
 }
 template void csr_apply<float>(Context*, const Csr*, const uint8_t*, const void*, void*,
-                               const void*, bool, double2*, int, int);
+                               const void*, int, const RedSite&, const SpmvCfg&);
 template void csr_apply<double>(Context*, const Csr*, const uint8_t*, const void*, void*,
-                                const void*, bool, double2*, int, int);
+                                const void*, int, const RedSite&, const SpmvCfg&);
+
+template <typename R>
+SpmvCfg sys_cfg(Context* c, const System* sys) {
+  const SpmvCfg& t = sys->cfg[sizeof(R) == 4 ? 0 : 1];
+  if (t.L > 0) return t;
+  return default_cfg(c, sys->space->ndofs, sys->space->nnz);
+}
+template SpmvCfg sys_cfg<float>(Context*, const System*);
+template SpmvCfg sys_cfg<double>(Context*, const System*);
+
+SpmvCfg csr_cfg(Context* c, const Csr* A) {
+  return A->cfg.L > 0 ? A->cfg : default_cfg(c, A->n, A->nnz);
+}
+
+// Autotuner: time every (lanes per row, loads per lane, blocks) candidate of the
+// fused operator with HIP events on the context's stream and keep the fastest.
+template <typename R>
+SpmvCfg tune_sys(Context* c, System* sys, int reps) {
+  const Space* s = sys->space;
+  using C = typename C2<R>::T;
+  DBuf<C> x(s->ndofs), y(s->ndofs);
+  SUCEM_HIP(hipMemsetAsync(x.p, 0, x.bytes(), c->stream));
+  hipEvent_t e0, e1;
+  SUCEM_HIP(hipEventCreate(&e0));
+  SUCEM_HIP(hipEventCreate(&e1));
+  Coef k;
+  k.s = 1.0; k.t = -1.0; k.s0 = 1.0;
+  RedSite none{};
+  SpmvCfg best = default_cfg(c, s->ndofs, s->nnz);
+  best.ms = 1e30;
+  const int Ls[] = {4, 8, 16, 32};
+  const int Us[] = {1, 2, 4};
+  const int per_cu[] = {4, 8, 16, 32};
+  const double avg = (double)s->nnz / (double)s->ndofs;
+  for (int L : Ls)
+    for (int U : Us) {
+      if (L * U > 4 * avg + 8 || L * U * 4 < avg) continue;
+      for (int pc : per_cu) {
+        SpmvCfg f;
+        f.L = L; f.U = U; f.G = op_grid(c, s->ndofs, pc);
+        sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);  // warm
+        SUCEM_HIP(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps; ++i) sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);
+        SUCEM_HIP(hipEventRecord(e1, c->stream));
+        SUCEM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SUCEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        f.ms = ms / reps;
+        if (f.ms < best.ms) best = f;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  sys->cfg[sizeof(R) == 4 ? 0 : 1] = best;
+  return best;
+}
+template SpmvCfg tune_sys<float>(Context*, System*, int);
+template SpmvCfg tune_sys<double>(Context*, System*, int);
 
 template <typename R>
 void sys_dinv(Context* c, const System* sys, Coef k, const uint8_t* mask, void* dinv) {
@@ -428,13 +512,13 @@ void system_apply(Context* c, const System* sys, Coef k, bool eliminate_pec, int
                   const void* x, void* y) {
   const Space* s = sys->space;
   SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S and T must be assembled");
-  const int G = op_grid(c, s->ndofs);
-  const int L = lanes_for(s->ndofs, s->nnz);
+  const SpmvCfg f = sys_cfg<R>(c, sys);
   const size_t bytes = sizeof(typename C2<R>::T);
+  RedSite none{};
   for (int64_t v = 0; v < nvec; ++v)
     sys_apply<R>(c, sys, k, eliminate_pec ? s->pec.p : nullptr,
                  (const char*)x + v * s->ndofs * bytes, (char*)y + v * s->ndofs * bytes, nullptr,
-                 false, nullptr, G, L);
+                 0, none, f);
 }
 template void system_apply<float>(Context*, const System*, Coef, bool, int64_t, const void*,
                                   void*);
@@ -442,8 +526,8 @@ template void system_apply<double>(Context*, const System*, Coef, bool, int64_t,
                                    void*);
 
 void csr_apply64(Context* c, const Csr* A, const double2* x, double2* y) {
-  csr_apply<double>(c, A, nullptr, x, y, nullptr, false, nullptr, op_grid(c, A->n),
-                    lanes_for(A->n, A->nnz));
+  RedSite none{};
+  csr_apply<double>(c, A, nullptr, x, y, nullptr, 0, none, csr_cfg(c, A));
 }
 
 void locate_points(Context* c, const Space* s, int64_t n, const double* pts_d, int32_t* cell_d) {

@@ -1,21 +1,27 @@
 // Host-side launchers of the operator kernels (used by the Krylov layer).
 #pragma once
 #include "internal.hpp"
+#include "kernels.hpp"
 
 namespace sucem {
 
 int lanes_for(int64_t n, int64_t nnz);
-int op_grid(Context* c, int64_t n);
+int op_grid(Context* c, int64_t n, int per_cu);
+SpmvCfg default_cfg(Context* c, int64_t n, int64_t nnz);
 
-// y = op(x) in precision R (float: float2 vectors / float values).  When `part`
-// is non-null the epilogue writes part[lb] = sum op(w_r) y_r (conjugated when
-// `conj`) and part[G + lb] = (sum |y_r|^2, 0) for the G blocks.
+// y = op(x) in precision R (float: float2 vectors / float values).  epi = 1 (or 2):
+// the grid also reduces sum_r w_r y_r (conj(w_r) y_r) and sum |y_r|^2 into rs.
 template <typename R>
 void sys_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
-               void* y, const void* w, bool conj, double2* part, int G, int L);
+               void* y, const void* w, int epi, const RedSite& rs, const SpmvCfg& f);
 template <typename R>
 void csr_apply(Context* c, const Csr* A, const uint8_t* mask, const void* x, void* y,
-               const void* w, bool conj, double2* part, int G, int L);
+               const void* w, int epi, const RedSite& rs, const SpmvCfg& f);
+template <typename R>
+SpmvCfg sys_cfg(Context* c, const System* sys);
+SpmvCfg csr_cfg(Context* c, const Csr* A);
+template <typename R>
+SpmvCfg tune_sys(Context* c, System* sys, int reps);
 // Jacobi inverse diagonal (zero on masked rows).
 template <typename R>
 void sys_dinv(Context* c, const System* sys, Coef k, const uint8_t* mask, void* dinv);

@@ -72,6 +72,14 @@ class DeviceProblem(object):
                   0 if hist is None else len(hist))
         return st.as_dict()
 
+    def tune(self, precision, reps=10):
+        """Autotune the fused SpMV launch for this system; returns ({L, U, G}, ms per SpMV)."""
+        cfg = np.zeros(3, np.int32)
+        ms = ctypes.c_double()
+        _hip.call("sucem_system_tune", self.ctx.handle, self.system.handle.h, int(precision),
+                  int(reps), _hip.ptr(cfg), ctypes.byref(ms))
+        return dict(lanes=int(cfg[0]), loads=int(cfg[1]), blocks=int(cfg[2])), ms.value
+
     def apply(self, k0, x, y, precision=32, nvec=1):
         _hip.call("sucem_system_apply", self.ctx.handle, self.system.handle.h, float(k0), precision,
                   0, nvec, _hip.ptr(x), _hip.ptr(y))

@@ -88,6 +88,7 @@ SIGNATURES = {
     "sucem_system_apply": (ctypes.c_int, [_P, _P, _D, ctypes.c_int, ctypes.c_int, _I64, _P, _P]),
     "sucem_system_apply_coef": (ctypes.c_int, [_P, _P, _D, _D, _D, ctypes.c_int, ctypes.c_int,
                                                 _I64, _P, _P]),
+    "sucem_system_tune": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, _P, _P]),
     "sucem_system_export_lhs": (ctypes.c_int, [_P, _D, _P]),
     "sucem_locate_points": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
     "sucem_basis_at_points": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
