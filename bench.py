@@ -34,11 +34,6 @@ def cpu_baseline(prob, fp64_iterations, budget_s=20.0):
     indptr, indices = prob.space.pattern()
     n = prob.ndofs
     A = scipy.sparse.csr_matrix((prob.system.lhs_values(prob.k0), indices, indptr), shape=(n, n))
-    tuned = {}
-    for prec in sorted({a.precision, 64}):
-        cfg, ms = prob.tune(prec)
-        tuned[prec] = dict(cfg, ms=ms)
-        log("tuned FP%d SpMV: %s %.4f ms" % (prec, cfg, ms))
     b = prob.rhs_dipoles(prob.source_pos, prob.source_moment, prob.k0).cpu().numpy()
     t0 = time.perf_counter()
     O.cocg_jacobi(A, b, tol=0.0, maxiter=2)
