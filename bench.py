@@ -105,8 +105,9 @@ def main():
     elapsed = time.perf_counter() - t0
     ok = all(s["status"] == 0 and s["rel_residual"] <= a.tol for s in stats)
 
-    # dominant kernel: the fused FP32 SpMV, timed with HIP events on the solve's stream
-    xs = torch.randn(prob.ndofs, dtype=torch.complex64, device="cuda")
+    # dominant kernel: the fused SpMV, timed with HIP events on the solve's stream
+    vdt = torch.complex64 if a.precision == 32 else torch.complex128
+    xs = torch.randn(prob.ndofs, dtype=vdt, device="cuda")
     ys = torch.zeros_like(xs)
     prob.apply(prob.k0, xs, ys, precision=a.precision)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -60,8 +60,22 @@ class DeviceProblem(object):
                   _hip.ptr(pos), _hip.ptr(val), _hip.ptr(idx), nrhs, 0.0, -k0 * Z0, _hip.ptr(b))
         return b
 
+    def _check_vec(self, t, dtype, nvec=1):
+        """Device vectors handed to the library must have the dtype and length it will use."""
+        import torch
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous():
+            raise ValueError("expected a contiguous device tensor")
+        if t.dtype != dtype or t.numel() < nvec * self.ndofs:
+            raise ValueError("expected %s with at least %d elements, got %s with %d"
+                             % (dtype, nvec * self.ndofs, t.dtype, t.numel()))
+
     def solve(self, k0, b, x, precision=64, tol=1e-8, max_iter=50000, method=_hip.METHOD_COCG,
               check_every=16, ru_delta=0.01, hist=None):
+        import torch
+        self._check_vec(b, torch.complex128)
+        self._check_vec(x, torch.complex128)
+        if hist is not None and (hist.dtype != np.float64 or not hist.flags["C_CONTIGUOUS"]):
+            raise ValueError("hist must be a contiguous float64 array")
         o = _hip.SolveOpts()
         _hip.load().sucem_solve_opts_default(ctypes.byref(o))
         o.method, o.precision, o.tol, o.max_iter = method, precision, tol, max_iter
@@ -81,6 +95,10 @@ class DeviceProblem(object):
         return dict(lanes=int(cfg[0]), loads=int(cfg[1]), blocks=int(cfg[2])), ms.value
 
     def apply(self, k0, x, y, precision=32, nvec=1):
+        import torch
+        dt = torch.complex64 if precision == 32 else torch.complex128
+        self._check_vec(x, dt, nvec)
+        self._check_vec(y, dt, nvec)
         _hip.call("sucem_system_apply", self.ctx.handle, self.system.handle.h, float(k0), precision,
                   0, nvec, _hip.ptr(x), _hip.ptr(y))
 
