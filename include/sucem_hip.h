@@ -42,6 +42,9 @@ enum {
 #define SUCEM_METHOD_BICGSTAB 1
 #define SUCEM_PRECOND_NONE 0
 #define SUCEM_PRECOND_JACOBI 1
+/* Jacobi plus the gradient-space correction G diag(G^T A G)^-1 G^T (driven / shifted
+ * Nedelec systems, COCG only) */
+#define SUCEM_PRECOND_GRADIENT 2
 
 typedef struct sucem_ctx sucem_ctx;
 typedef struct sucem_mesh sucem_mesh;

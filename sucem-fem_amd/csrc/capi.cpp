@@ -526,7 +526,8 @@ static SolveOpts convert(const sucem_solve_opts* o) {
   }
   SUCEM_CHECK(s.tol > 0 && s.ru_delta > 0 && s.ru_delta < 1, SUCEM_ERR_INVALID,
               "tol must be > 0 and ru_delta in (0, 1)");
-  SUCEM_CHECK(s.precond == SUCEM_PRECOND_NONE || s.precond == SUCEM_PRECOND_JACOBI,
+  SUCEM_CHECK(s.precond == SUCEM_PRECOND_NONE || s.precond == SUCEM_PRECOND_JACOBI ||
+                  s.precond == SUCEM_PRECOND_GRADIENT,
               SUCEM_ERR_UNSUPPORTED, "unknown preconditioner");
   return s;
 }

@@ -1,0 +1,286 @@
+// Gradient-space (Hiptmair-type) additive preconditioner for the curl-curl systems.
+//
+//   M^-1 r = D^-1 r + G W G^T r,   W = diag(G^T A G)^-1,
+//
+// with D = diag(A) and G the discrete gradient from the Lagrange space of the
+// same order (P1 vertices, or P2 vertices + edge midpoints) into the Nedelec
+// space: G[i][j] = nodal dual i applied to grad(phi_j).  Jacobi alone leaves the
+// gradient fields (the near-kernel of curl-curl, shifted to -k0^2 T by the mass
+// term) almost untouched; the second term acts on exactly that subspace.  M is
+// symmetric, so COCG stays applicable.  A = cs S + ct T + i c0 S0 and
+// G^T S G = 0, so its diagonal is formed per solve from three stored diagonals.
+#include <vector>
+
+#include "internal.hpp"
+#include "kernels.hpp"
+#include "ops.hpp"
+#include "refdata.hpp"
+
+namespace sucem {
+namespace {
+
+constexpr int TB = 256;
+
+// Contributions (ned dof, lag dof) of every cell's nonzero local gradient entries.
+__global__ void k_grad_keys(int64_t nc, int nloc, int nlagl, int npairs, const int32_t* pairs,
+                            const int32_t* cell_dofs, const int32_t* cells,
+                            const int32_t* cell_edges, int64_t nv, int B, int transpose,
+                            uint64_t* keys, uint32_t* vals) {
+  const int64_t n = nc * npairs;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t / npairs;
+    const int p = (int)(t % npairs);
+    const int i = pairs[2 * p], j = pairs[2 * p + 1];
+    const uint64_t ned = (uint64_t)cell_dofs[c * nloc + i];
+    const uint64_t lag = j < 4 ? (uint64_t)cells[4 * c + j]
+                               : (uint64_t)(nv + cell_edges[6 * c + (j - 4)]);
+    keys[t] = transpose ? ((lag << B) | ned) : ((ned << B) | lag);
+    vals[t] = (uint32_t)t;
+  }
+}
+
+// value of each unique entry = the local gradient entry of its first contribution
+__global__ void k_grad_vals(int64_t nnz, const int32_t* seg, const uint32_t* perm, int npairs,
+                            const double* pair_vals, double* v64, float* v32) {
+  for (int64_t z = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; z < nnz;
+       z += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t id = perm[seg[z]];
+    const double v = pair_vals[id % (uint32_t)npairs];
+    v64[z] = v;
+    v32[z] = (float)v;
+  }
+}
+
+__device__ __forceinline__ int find_col(const int32_t* col, int lo, int hi, int target) {
+  hi -= 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int cm = col[mid];
+    if (cm == target) return mid;
+    if (cm < target) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+// One wavefront per Lagrange node j: out[3j..3j+2] = g_j^T {S, T, S0} g_j over the
+// (unmasked) support of column j of G.
+__global__ __launch_bounds__(TB) void k_grad_diag(int64_t nlag, const int32_t* gt_ptr,
+                                                  const int32_t* gt_col, const double* gt_val,
+                                                  const int32_t* rp, const int32_t* col,
+                                                  const double2* st, const int32_t* s0p,
+                                                  const int32_t* s0c, const double* s0v,
+                                                  const uint8_t* mask, double* out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t j = wave; j < nlag; j += nwaves) {
+    const int a = gt_ptr[j], m = gt_ptr[j + 1] - a;
+    double sS = 0, sT = 0, s0 = 0;
+    for (int p = lane; p < m * m; p += 64) {
+      const int ii = p / m, kk = p - ii * m;
+      const int i = gt_col[a + ii], k = gt_col[a + kk];
+      if (mask && (mask[i] || mask[k])) continue;
+      const double w = gt_val[a + ii] * gt_val[a + kk];
+      const int pos = find_col(col, rp[i], rp[i + 1], k);
+      if (pos >= 0) {
+        sS += w * st[pos].x;
+        sT += w * st[pos].y;
+      }
+      if (s0p) {
+        const int q = find_col(s0c, s0p[i], s0p[i + 1], k);
+        if (q >= 0) s0 += w * s0v[q];
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      sS += __shfl_xor(sS, o);
+      sT += __shfl_xor(sT, o);
+      s0 += __shfl_xor(s0, o);
+    }
+    if (lane == 0) {
+      out[3 * j] = sS;
+      out[3 * j + 1] = sT;
+      out[3 * j + 2] = s0;
+    }
+  }
+}
+
+// winv[j] = 1 / (cs dS + ct dT + i c0 dS0), 0 where the diagonal vanishes
+template <typename C>
+__global__ void k_grad_winv(int64_t nlag, const double* d3, double cs, double ct, double c0,
+                            C* winv) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nlag;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const double re = cs * d3[3 * j] + ct * d3[3 * j + 1];
+    const double im = c0 * d3[3 * j + 2];
+    const double den = re * re + im * im;
+    winv[j] = den > 0.0 ? from_d<C>(make_double2(re / den, -im / den))
+                        : from_d<C>(make_double2(0.0, 0.0));
+  }
+}
+
+// w = winv .* (G^T r): L lanes per Lagrange row
+template <typename R, int L>
+__global__ __launch_bounds__(TB) void k_grad_t(int64_t nlag, const int32_t* __restrict__ gt_ptr,
+                                               const int32_t* __restrict__ gt_col,
+                                               const R* __restrict__ gt_val,
+                                               const typename C2<R>::T* __restrict__ r,
+                                               const typename C2<R>::T* __restrict__ winv,
+                                               typename C2<R>::T* __restrict__ w) {
+  using C = typename C2<R>::T;
+  const int lane = threadIdx.x % L;
+  const int64_t group = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / L;
+  const int64_t ngroups = ((int64_t)gridDim.x * blockDim.x) / L;
+  for (int64_t j = group; j < nlag; j += ngroups) {
+    C acc = mk(R(0), R(0));
+    for (int q = gt_ptr[j] + lane; q < gt_ptr[j + 1]; q += L) {
+      const R g = gt_val[q];
+      const C v = r[gt_col[q]];
+      acc.x += g * v.x;
+      acc.y += g * v.y;
+    }
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) {
+      acc.x += __shfl_xor(acc.x, o);
+      acc.y += __shfl_xor(acc.y, o);
+    }
+    if (lane == 0) w[j] = cmul(winv[j], acc);
+  }
+}
+
+// z = dinv .* r + G w (masked rows zero); optional reduction of r.z and |r|^2
+template <typename R>
+__global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __restrict__ g_ptr,
+                                               const int32_t* __restrict__ g_col,
+                                               const R* __restrict__ g_val,
+                                               const typename C2<R>::T* __restrict__ w,
+                                               const typename C2<R>::T* __restrict__ dinv,
+                                               const typename C2<R>::T* __restrict__ r,
+                                               typename C2<R>::T* __restrict__ z,
+                                               const uint8_t* __restrict__ mask, int reduce,
+                                               RedSite rs) {
+  using C = typename C2<R>::T;
+  const int lb = xcd_block(G);
+  const int chunk = (n + G - 1) / G;
+  const int start = lb * chunk, end = min(n, start + chunk);
+  double2 d1 = make_double2(0, 0), d2 = make_double2(0, 0);
+  for (int i = start + threadIdx.x; i < end; i += TB) {
+    const C ri = r[i];
+    C zi = cmul(dinv[i], ri);
+    for (int q = g_ptr[i]; q < g_ptr[i + 1]; ++q) {
+      const R g = g_val[q];
+      const C v = w[g_col[q]];
+      zi.x += g * v.x;
+      zi.y += g * v.y;
+    }
+    if (mask && mask[i]) zi = mk(R(0), R(0));
+    z[i] = zi;
+    if (reduce) {
+      const double2 rd = to_d(ri), zd = to_d(zi);
+      d1.x += rd.x * zd.x - rd.y * zd.y;
+      d1.y += rd.x * zd.y + rd.y * zd.x;
+      d2.x += rd.x * rd.x + rd.y * rd.y;
+    }
+  }
+  if (reduce) reduce_finalize<TB>(rs, G, d1, d2);
+}
+
+}  // namespace
+
+void ensure_gradient(Context* c, Space* s) {
+  if (s->g_ptr.p) return;
+  const RefData& R = ref_data(s->order);
+  const Mesh* m = s->mesh;
+  s->nlag = s->order == 1 ? m->nv : m->nv + s->nE;
+  std::vector<int32_t> pairs;
+  std::vector<double> pvals;
+  for (int i = 0; i < R.nloc; ++i)
+    for (int j = 0; j < R.nlag; ++j)
+      if (R.grad[i * R.nlag + j] != 0.0) {
+        pairs.push_back(i);
+        pairs.push_back(j);
+        pvals.push_back(R.grad[i * R.nlag + j]);
+      }
+  const int npairs = (int)pvals.size();
+  DBuf<int32_t> pairs_d(pairs.size());
+  DBuf<double> pvals_d(pvals.size());
+  h2d(c, pairs_d.p, pairs.data(), pairs.size());
+  h2d(c, pvals_d.p, pvals.data(), pvals.size());
+  const int64_t n = m->nc * npairs;
+  SUCEM_CHECK(n < (int64_t(1) << 31), SUCEM_ERR_UNSUPPORTED, "gradient too large");
+  const int B = key_bits(std::max<int64_t>(s->ndofs, s->nlag));
+  for (int transpose = 0; transpose < 2; ++transpose) {
+    DBuf<uint64_t> keys(n);
+    DBuf<uint32_t> vals(n);
+    SUCEM_LAUNCH(k_grad_keys, grid_for(n, TB), TB, 0, c->stream, m->nc, s->nloc, R.nlag, npairs,
+                 pairs_d.p, s->cell_dofs.p, m->cells.p, s->cell_edges.p, m->nv, B, transpose,
+                 keys.p, vals.p);
+    DBuf<int32_t> seg;
+    DBuf<uint32_t> perm;
+    DBuf<int32_t>& ptr = transpose ? s->gt_ptr : s->g_ptr;
+    DBuf<int32_t>& colb = transpose ? s->gt_col : s->g_col;
+    DBuf<double>& v64 = transpose ? s->gt_val : s->g_val;
+    DBuf<float>& v32 = transpose ? s->gt_val32 : s->g_val32;
+    const int64_t nnz = build_csr_from_pairs(c, n, keys, vals, B, transpose ? s->nlag : s->ndofs,
+                                             ptr, colb, seg, perm);
+    v64.alloc(nnz);
+    v32.alloc(nnz);
+    SUCEM_LAUNCH(k_grad_vals, grid_for(nnz, TB), TB, 0, c->stream, nnz, seg.p, perm.p, npairs,
+                 pvals_d.p, v64.p, v32.p);
+    s->nnz_g = nnz;
+    c->sync();
+  }
+}
+
+const double* gradient_diag(Context* c, System* sys, bool masked) {
+  Space* s = sys->space;
+  ensure_gradient(c, s);
+  DBuf<double>& d = sys->gdiag[masked ? 1 : 0];
+  if (!d.p) {
+    SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S and T must be assembled");
+    d.alloc(3 * s->nlag);
+    const bool has_s0 = (sys->which & SUCEM_MAT_S0) && s->nnz0 > 0;
+    const int64_t waves = s->nlag;
+    SUCEM_LAUNCH(k_grad_diag, grid_for(waves * 64, TB), TB, 0, c->stream, s->nlag, s->gt_ptr.p,
+                 s->gt_col.p, s->gt_val.p, s->row_ptr.p, s->col.p, sys->st64.p,
+                 has_s0 ? s->s0_ptr.p : nullptr, s->s0_col.p, sys->s0_64.p,
+                 masked ? s->pec.p : nullptr, d.p);
+    c->sync();
+  }
+  return d.p;
+}
+
+template <typename R>
+void grad_winv(Context* c, System* sys, Coef k, bool masked, void* winv) {
+  const double* d3 = gradient_diag(c, sys, masked);
+  using C = typename C2<R>::T;
+  SUCEM_LAUNCH((k_grad_winv<C>), grid_for(sys->space->nlag, TB), TB, 0, c->stream,
+               sys->space->nlag, d3, k.s, k.t, k.s0, (C*)winv);
+}
+template void grad_winv<float>(Context*, System*, Coef, bool, void*);
+template void grad_winv<double>(Context*, System*, Coef, bool, void*);
+
+// z = M^-1 r; with rs.mode != FIN_NONE the grid also reduces r.z and |r|^2.
+template <typename R>
+void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv, const void* r,
+                  void* w, void* z, const uint8_t* mask, int G, const RedSite& rs) {
+  using C = typename C2<R>::T;
+  const R* gtv;
+  const R* gv;
+  if constexpr (sizeof(R) == 4) { gtv = s->gt_val32.p; gv = s->g_val32.p; }
+  else { gtv = s->gt_val.p; gv = s->g_val.p; }
+  constexpr int L = 8;
+  const int64_t groups = s->nlag;
+  SUCEM_LAUNCH((k_grad_t<R, L>), grid_for(groups * L, TB, (int64_t)c->num_cus * 16), TB, 0,
+               c->stream, s->nlag, s->gt_ptr.p, s->gt_col.p, gtv, (const C*)r, (const C*)winv,
+               (C*)w);
+  SUCEM_LAUNCH((k_grad_z<R>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p, s->g_col.p, gv,
+               (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
+               rs.mode != FIN_NONE ? 1 : 0, rs);
+}
+template void grad_precond<float>(Context*, const Space*, const void*, const void*, const void*,
+                                  void*, void*, const uint8_t*, int, const RedSite&);
+template void grad_precond<double>(Context*, const Space*, const void*, const void*, const void*,
+                                   void*, void*, const uint8_t*, int, const RedSite&);
+
+}  // namespace sucem

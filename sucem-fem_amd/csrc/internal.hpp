@@ -46,6 +46,11 @@ struct Space {
   // reference tensors on device
   DBuf<double> ref_mass, ref_stiff, ref_face, ref_basis;
   DBuf<int32_t> ref_face_dofs;
+  // discrete gradient G (N x nlag) and its transpose, built on first use
+  int64_t nlag = 0, nnz_g = 0;
+  DBuf<int32_t> g_ptr, g_col, gt_ptr, gt_col;
+  DBuf<double> g_val, gt_val;
+  DBuf<float> g_val32, gt_val32;
 };
 
 // SpMV launch configuration: lanes per row, loads per lane, blocks (0 = heuristic).
@@ -57,6 +62,9 @@ struct SpmvCfg {
 struct System {
   Context* ctx = nullptr;
   SpmvCfg cfg[2];  // tuned FP32 / FP64 operator
+  // diag(G^T S G), diag(G^T T G), diag(G^T S0 G) per Lagrange node, without / with the
+  // PEC rows removed (built on first use by the gradient preconditioner)
+  DBuf<double> gdiag[2];
   Space* space = nullptr;
   unsigned which = 0;
   DBuf<double2> st64;
@@ -83,6 +91,11 @@ void mesh_sort_cells(Context* c, Mesh* m);
 int64_t mesh_set_materials(Context* c, Mesh* m, const int32_t* region_d, int64_t nreg,
                            const int64_t* ids_h, const double* eps_h, const double* mu_h);
 void space_build(Context* c, Space* s);
+
+int64_t build_csr_from_pairs(Context* c, int64_t n, DBuf<uint64_t>& keys, DBuf<uint32_t>& vals,
+                             int B, int64_t nrows, DBuf<int32_t>& row_ptr, DBuf<int32_t>& col,
+                             DBuf<int32_t>& seg, DBuf<uint32_t>& perm);
+int key_bits(int64_t n);
 
 // assemble.hip
 void assemble_system(Context* c, Space* s, System* sys, unsigned which);
@@ -127,6 +140,11 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
                   const SolveOpts& o, SolveStats& st, double* hist_h, int64_t hist_cap);
 void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const SolveOpts& o,
                SolveStats& st, double* hist_h, int64_t hist_cap);
+
+// hiptmair.hip: additive gradient-space preconditioner
+//   M^-1 r = D^-1 r + G (diag(G^T A G))^-1 G^T r
+void ensure_gradient(Context* c, Space* s);
+const double* gradient_diag(Context* c, System* sys, bool masked);
 
 // eigen.hip
 struct EigOpts {

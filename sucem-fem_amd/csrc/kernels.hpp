@@ -83,6 +83,7 @@ enum FinalizeMode {
   FIN_BICG_ALPHA,   // s0 = conj(rt).v      -> alpha = rho / s0
   FIN_BICG_OMEGA,   // s0 = conj(s).t, s1 = |t|^2 -> omega = conj(s0)/s1
   FIN_BICG_BETA,    // s0 = conj(rt).r, s1 = |r|^2 -> beta = (s0/rho)(alpha/omega), rho = s0
+  FIN_SET_RHO,      // s0 = r.z, s1 = |r|^2     -> rho = s0 (start / restart)
 };
 
 // One reduction site: per-block partial sums, a ticket counter (zero between launches)
@@ -119,6 +120,10 @@ __device__ __forceinline__ void finalize(const RedSite& rs, double2 s0, double2 
       break;
     case FIN_NORM:
       sc[S_BNORM] = s0.x;
+      break;
+    case FIN_SET_RHO:
+      sc[S_RHO] = s0.x; sc[S_RHO + 1] = s0.y;
+      sc[S_RR] = s1.x;
       break;
     case FIN_BICG_OMEGA:
       out = s1.x == 0.0 ? make_double2(0.0, 0.0) : make_double2(s0.x / s1.x, -s0.y / s1.x);

@@ -140,6 +140,33 @@ __global__ __launch_bounds__(TB) void k_cocg_xr(int n, int G, const double* scal
   reduce_finalize<TB>(rs, G, nr, n2);
 }
 
+// x += alpha p ; r -= alpha q  (the preconditioner kernel reduces afterwards)
+template <typename C>
+__global__ void k_cocg_xr_plain(int n, int G, const double* scal, C* x, C* r, const C* p,
+                                const C* q) {
+  const C a = from_d<C>(scal2(scal, S_ALPHA));
+  Range rr = block_rows(n, G);
+  for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
+    const C ap = cmul(a, p[i]), aq = cmul(a, q[i]);
+    C xi = x[i], ri = r[i];
+    xi.x += ap.x; xi.y += ap.y;
+    ri.x -= aq.x; ri.y -= aq.y;
+    x[i] = xi;
+    r[i] = ri;
+  }
+}
+
+// p = z + beta p
+template <typename C>
+__global__ void k_cocg_pz(int n, int G, const double* scal, const C* z, C* p) {
+  const C b = from_d<C>(scal2(scal, S_BETA));
+  Range rr = block_rows(n, G);
+  for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
+    const C bp = cmul(b, p[i]), zi = z[i];
+    p[i] = mk(zi.x + bp.x, zi.y + bp.y);
+  }
+}
+
 // p = dinv r + beta p
 template <typename C>
 __global__ void k_cocg_p(int n, int G, const double* scal, const C* dinv, const C* r, C* p) {
@@ -222,6 +249,11 @@ struct Op {
   virtual void apply_lo(const void* x, void* y, const void* w, int epi, const RedSite& rs) = 0;
   virtual void apply_hi(const double2* x, double2* y) = 0;
   virtual void dinv_lo(void* d) = 0;
+  // gradient-space preconditioner (driven / shifted systems only)
+  virtual int64_t nlag() { return 0; }
+  virtual void winv_lo(void* winv) {}
+  virtual void precond_lo(const void* winv, const void* dinv, const void* r, void* w, void* z,
+                          int G, const RedSite& rs) {}
   virtual ~Op() {}
 };
 
@@ -240,6 +272,17 @@ struct SysOp : Op {
   void dinv_lo(void* d) override {
     if (identity_prec) csr_dinv<R>(c, nullptr, mask, true, d, n);
     else sys_dinv<R>(c, sys, k, mask, d);
+  }
+  int64_t nlag() override {
+    ensure_gradient(c, sys->space);
+    return sys->space->nlag;
+  }
+  void winv_lo(void* winv) override {
+    grad_winv<R>(c, const_cast<System*>(sys), k, mask != nullptr, winv);
+  }
+  void precond_lo(const void* winv, const void* dinv, const void* r, void* w, void* z, int G,
+                  const RedSite& rs) override {
+    grad_precond<R>(c, sys->space, winv, dinv, r, w, z, mask, G, rs);
   }
 };
 
@@ -276,6 +319,12 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   const int Gm = std::max(G, op.flo.G);      // partial buffer size
   const bool lo = sizeof(R) == 4;
   const bool cocg = o.method == SUCEM_METHOD_COCG;
+  const bool grad = o.precond == SUCEM_PRECOND_GRADIENT;
+  SUCEM_CHECK(!grad || cocg, SUCEM_ERR_UNSUPPORTED,
+              "the gradient preconditioner is available with COCG");
+  const int64_t nl = grad ? op.nlag() : 0;
+  SUCEM_CHECK(!grad || nl > 0, SUCEM_ERR_UNSUPPORTED,
+              "the gradient preconditioner needs an assembled Nedelec system");
   auto t0 = std::chrono::steady_clock::now();
   st = SolveStats();
 
@@ -302,8 +351,14 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   DBuf<C> rt(cocg ? 0 : n), v(cocg ? 0 : n), phat(cocg ? 0 : n), s(cocg ? 0 : n),
       shat(cocg ? 0 : n), t(cocg ? 0 : n);
   C* xlo = lo ? xlo_buf.p : (C*)x;
+  DBuf<C> z(grad ? n : 0), w(nl), winv(nl);
+  // z = M^-1 r, reducing r.z / |r|^2 into the site's scalars
+  auto precondition = [&](const RedSite& rs) {
+    op.precond_lo(winv.p, dinv.p, r.p, w.p, z.p, G, rs);
+  };
 
   op.dinv_lo(dinv.p);
+  if (grad) op.winv_lo(winv.p);
   SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, x);
   if (lo) SUCEM_LAUNCH((k_zero_c<C>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, xlo);
   SUCEM_LAUNCH(k_norm2, G, TB, 0, c->stream, n, G, b, op.mask, site(SITE_NORM, FIN_NORM));
@@ -318,15 +373,27 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, ax.p);
   SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p,
                cocg ? 1 : 0, site(SITE_RESID, FIN_RESID));
-  if (cocg) SUCEM_LAUNCH((k_cocg_start<C>), G, TB, 0, c->stream, n, G, dinv.p, r.p, p.p);
-  else SUCEM_LAUNCH((k_copy<C>), G, TB, 0, c->stream, n, G, r.p, rt.p);
+  if (grad) {
+    precondition(site(SITE_UPDATE, FIN_SET_RHO));
+    SUCEM_LAUNCH((k_copy<C>), G, TB, 0, c->stream, n, G, z.p, p.p);
+  } else if (cocg) {
+    SUCEM_LAUNCH((k_cocg_start<C>), G, TB, 0, c->stream, n, G, dinv.p, r.p, p.p);
+  } else {
+    SUCEM_LAUNCH((k_copy<C>), G, TB, 0, c->stream, n, G, r.p, rt.p);
+  }
   double last_true = 1.0;
   int it = 0;
   int restart = 1;
   st.status = o.max_iter;
   double rel = 1.0;
   for (it = 0; it < o.max_iter; ++it) {
-    if (cocg) {
+    if (cocg && grad) {
+      op.apply_lo(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
+      SUCEM_LAUNCH((k_cocg_xr_plain<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, p.p, q.p);
+      precondition(site(SITE_UPDATE, FIN_COCG_BETA, it));
+      SUCEM_LAUNCH((k_cocg_pz<C>), G, TB, 0, c->stream, n, G, scal.p, z.p, p.p);
+      st.spmv_lo += 1;
+    } else if (cocg) {
       op.apply_lo(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
       SUCEM_LAUNCH((k_cocg_xr<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, p.p, q.p, dinv.p,
                    site(SITE_UPDATE, FIN_COCG_BETA, it));
@@ -369,7 +436,10 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     }
     // restart from the replaced residual (rho was recomputed by k_residual): keeping
     // the old direction after a residual replacement stalls these indefinite systems
-    if (cocg) {
+    if (grad) {
+      precondition(site(SITE_UPDATE, FIN_SET_RHO));
+      SUCEM_LAUNCH((k_copy<C>), G, TB, 0, c->stream, n, G, z.p, p.p);
+    } else if (cocg) {
       SUCEM_LAUNCH((k_cocg_start<C>), G, TB, 0, c->stream, n, G, dinv.p, r.p, p.p);
     } else {
       SUCEM_LAUNCH((k_copy<C>), G, TB, 0, c->stream, n, G, r.p, rt.p);

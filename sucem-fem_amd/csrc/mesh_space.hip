@@ -466,6 +466,14 @@ int64_t build_pattern(Context* c, int64_t n, DBuf<uint64_t>& keys, DBuf<uint32_t
 
 }  // namespace
 
+int64_t build_csr_from_pairs(Context* c, int64_t n, DBuf<uint64_t>& keys, DBuf<uint32_t>& vals,
+                             int B, int64_t nrows, DBuf<int32_t>& row_ptr, DBuf<int32_t>& col,
+                             DBuf<int32_t>& seg, DBuf<uint32_t>& perm) {
+  return build_pattern(c, n, keys, vals, B, nrows, false, row_ptr, col, seg, perm);
+}
+
+int key_bits(int64_t n) { return bits_for(n); }
+
 void mesh_create_box(Context* c, Mesh* m, int64_t nx, int64_t ny, int64_t nz,
                      const double scale[3], const double shift[3]) {
   SUCEM_CHECK(nx > 0 && ny > 0 && nz > 0, SUCEM_ERR_INVALID, "box divisions must be positive");

@@ -22,6 +22,13 @@ SpmvCfg sys_cfg(Context* c, const System* sys);
 SpmvCfg csr_cfg(Context* c, const Csr* A);
 template <typename R>
 SpmvCfg tune_sys(Context* c, System* sys, int reps);
+
+// gradient-space preconditioner (hiptmair.hip)
+template <typename R>
+void grad_winv(Context* c, System* sys, Coef k, bool masked, void* winv);
+template <typename R>
+void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv, const void* r,
+                  void* w, void* z, const uint8_t* mask, int G, const RedSite& rs);
 // Jacobi inverse diagonal (zero on masked rows).
 template <typename R>
 void sys_dinv(Context* c, const System* sys, Coef k, const uint8_t* mask, void* dinv);

@@ -48,6 +48,19 @@ Poly deriv(const Poly& p, int axis) {
   return o;
 }
 
+Poly mul(const Poly& a, const Poly& b) {
+  Poly o{};
+  for (int i = 0; i < kMono; ++i) {
+    if (a[i] == 0) continue;
+    for (int j = 0; j < kMono; ++j) {
+      if (b[j] == 0) continue;
+      o[mono_index(kExp[i][0] + kExp[j][0], kExp[i][1] + kExp[j][1], kExp[i][2] + kExp[j][2])] +=
+          a[i] * b[j];
+    }
+  }
+  return o;
+}
+
 Poly sub(const Poly& a, const Poly& b) {
   Poly o;
   for (int m = 0; m < kMono; ++m) o[m] = a[m] - b[m];
@@ -284,6 +297,38 @@ RefData build(int order) {
             }
         }
       }
+  }
+  // Discrete gradient of the Lagrange P_order basis.
+  {
+    Poly lam[4] = {};
+    lam[0][0] = 1; lam[0][1] = -1; lam[0][2] = -1; lam[0][3] = -1;
+    lam[1][1] = 1; lam[2][2] = 1; lam[3][3] = 1;
+    std::vector<Poly> lag;
+    if (order == 1) {
+      for (int v = 0; v < 4; ++v) lag.push_back(lam[v]);
+    } else {
+      for (int v = 0; v < 4; ++v) {
+        Poly two = lam[v];
+        for (auto& c : two) c *= 2;
+        two[0] -= 1;
+        lag.push_back(mul(lam[v], two));        // lambda (2 lambda - 1)
+      }
+      for (int e = 0; e < 6; ++e) {
+        Poly p = mul(lam[kEdges[e][0]], lam[kEdges[e][1]]);
+        for (auto& c : p) c *= 4;
+        lag.push_back(p);                       // 4 lambda_a lambda_b
+      }
+    }
+    R.nlag = (int)lag.size();
+    R.grad.assign(n * R.nlag, 0.0);
+    for (int j = 0; j < R.nlag; ++j) {
+      Poly g[3] = {deriv(lag[j], 0), deriv(lag[j], 1), deriv(lag[j], 2)};
+      for (int i = 0; i < n; ++i) {
+        double s = 0;
+        for (int k = 0; k < 3; ++k) s += eval(g[k], duals[i].p) * duals[i].t[k];
+        R.grad[i * R.nlag + j] = std::fabs(s) < 1e-13 ? 0.0 : s;
+      }
+    }
   }
   return R;
 }

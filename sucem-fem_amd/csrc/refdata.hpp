@@ -35,6 +35,11 @@ struct RefData {
   // face_dofs[f][a]: local DOF index of face-DOF a of local face f (4 * nfd)
   std::vector<int> face_dofs;
   double ref_face_area[4];
+  // Discrete gradient: grad[i][j] = nodal dual i applied to grad(phi_j) for the
+  // Lagrange P_order basis phi_j (vertices, then (order 2) edge midpoints in UFC
+  // edge order); the same on every cell (nloc x nlag).
+  int nlag = 0;
+  std::vector<double> grad;
 };
 
 const RefData& ref_data(int order);

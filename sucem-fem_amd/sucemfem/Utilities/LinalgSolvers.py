@@ -4,8 +4,10 @@ Same classes, logging data and call pattern as the reference.  The Krylov
 iterations run in libsucem_hip: a matrix produced by
 ``DrivenProblemABC.get_LHS_matrix`` is solved with the fused S/T/S0 operator,
 any other matrix is uploaded as a complex CSR.  Preconditioners: ``None``,
-``'diagonal'`` (Jacobi) and ``'ilu'``; the device has no incomplete-LU
-factorisation, so ``'ilu'`` warns and uses Jacobi.  Tolerances default to
+``'diagonal'`` (Jacobi), ``'gradient'`` (Jacobi plus the discrete-gradient
+correction, COCG on a driven system only) and ``'ilu'``.  The device has no
+incomplete-LU factorisation: ``'ilu'`` selects the gradient preconditioner for a
+driven system solved by COCG and Jacobi otherwise, with a warning.  Tolerances default to
 SciPy's (relative residual 1e-5, at most 10*N iterations) as in the reference,
 which passes none.
 """
@@ -89,9 +91,18 @@ class SystemSolverBase(object):
             self._precond = _hip.PRECOND_NONE
         elif M_type.lower() == 'diagonal':
             self._precond = _hip.PRECOND_JACOBI
+        elif M_type.lower() == 'gradient':
+            self._precond = _hip.PRECOND_GRADIENT
         elif M_type.lower() == 'ilu':
-            warnings.warn("incomplete LU is not available on the device; using Jacobi ('diagonal')")
-            self._precond = _hip.PRECOND_JACOBI
+            driven = getattr(self._A, "device_operator", None) is not None
+            if driven and self.method == _hip.METHOD_COCG:
+                warnings.warn("incomplete LU is not available on the device; using the gradient "
+                              "preconditioner")
+                self._precond = _hip.PRECOND_GRADIENT
+            else:
+                warnings.warn("incomplete LU is not available on the device; using Jacobi "
+                              "('diagonal')")
+                self._precond = _hip.PRECOND_JACOBI
         else:
             warnings.warn("Preconditioner type '%s' not recognised. Not using preconditioner." % M_type)
             self._precond = _hip.PRECOND_NONE

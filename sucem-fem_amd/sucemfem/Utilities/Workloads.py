@@ -70,7 +70,7 @@ class DeviceProblem(object):
                              % (dtype, nvec * self.ndofs, t.dtype, t.numel()))
 
     def solve(self, k0, b, x, precision=64, tol=1e-8, max_iter=50000, method=_hip.METHOD_COCG,
-              check_every=16, ru_delta=0.01, hist=None):
+              check_every=16, ru_delta=0.01, hist=None, precond=_hip.PRECOND_GRADIENT):
         import torch
         self._check_vec(b, torch.complex128)
         self._check_vec(x, torch.complex128)
@@ -79,7 +79,7 @@ class DeviceProblem(object):
         o = _hip.SolveOpts()
         _hip.load().sucem_solve_opts_default(ctypes.byref(o))
         o.method, o.precision, o.tol, o.max_iter = method, precision, tol, max_iter
-        o.check_every, o.ru_delta = check_every, ru_delta
+        o.check_every, o.ru_delta, o.precond = check_every, ru_delta, precond
         st = _hip.SolveStats()
         _hip.call("sucem_solve_driven", self.ctx.handle, self.system.handle.h, float(k0),
                   _hip.ptr(b), _hip.ptr(x), ctypes.byref(o), ctypes.byref(st), _hip.ptr(hist),

@@ -15,7 +15,7 @@ SUCEM_OK = 0
 ERR_INVALID, ERR_HIP, ERR_OOM, ERR_OUTSIDE, ERR_MATERIAL, ERR_UNSUPPORTED = 1, 2, 3, 4, 5, 6
 MAT_S, MAT_T, MAT_S0 = 1, 2, 4
 METHOD_COCG, METHOD_BICGSTAB = 0, 1
-PRECOND_NONE, PRECOND_JACOBI = 0, 1
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_GRADIENT = 0, 1, 2
 
 
 class SolveOpts(ctypes.Structure):
