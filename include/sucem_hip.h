@@ -66,7 +66,7 @@ typedef struct sucem_solve_opts {
 typedef struct sucem_eig_opts {
   int ncv;              /* Lanczos basis size (0: max(2 nev + 1, 20)) */
   int max_restarts;
-  int inner_precision;  /* 64 or 32 (FP32 inner solves with FP64 reliable updates) */
+  int inner_precision;  /* 64 (32 is rejected: FP32 inner solves of S - sigma T stall) */
   int inner_max_iter;
   double tol;           /* relative Ritz residual of nu = 1/(k^2 - sigma) */
   double inner_tol;     /* relative residual of the inner (S - sigma T) solves */

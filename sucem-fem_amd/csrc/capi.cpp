@@ -607,6 +607,9 @@ int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,
   }
   SUCEM_CHECK(o.inner_precision == 32 || o.inner_precision == 64, SUCEM_ERR_INVALID,
               "inner precision must be 32 or 64");
+  SUCEM_CHECK(o.inner_precision == 64, SUCEM_ERR_UNSUPPORTED,
+              "FP32 inner solves of the shifted operator S - sigma T do not converge to the "
+              "accuracy the Lanczos recursion needs; use inner_precision 64");
   SUCEM_CHECK(o.tol > 0 && o.inner_tol > 0 && o.max_restarts >= 0 && o.inner_max_iter > 0,
               SUCEM_ERR_INVALID, "bad eigen options");
   EigStats st;
