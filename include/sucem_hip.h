@@ -184,6 +184,15 @@ int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const doubl
                        double* x_d, const sucem_solve_opts* opts, sucem_solve_stats* stats,
                        double* res_hist_h, int64_t hist_cap);
 
+/* nb driven systems on one assembled system, system j at wavenumber k0_h[j]:
+ * (S - k0_j^2 T + i k0_j S0) x_j = b_j.  B_d, X_d: nb x N complex128 (system-major).  The
+ * matrix is read once per iteration for a batch of up to 16 systems (vectors interleaved
+ * internally).  FP64 COCG with SUCEM_PRECOND_GRADIENT only.  stats_h (nullable): nb entries.
+ * (Frequency sweeps of the reference re-form and re-solve the LHS per frequency,
+ * sandbox/patch/patch_driver.py:64-129.) */
+int sucem_solve_driven_batch(sucem_ctx* ctx, sucem_system* sys, int64_t nb, const double* k0_h,
+                             const double* B_d, double* X_d, const sucem_solve_opts* opts,
+                             sucem_solve_stats* stats_h);
 /* Solve (cs S + ct T + i c0 S0) x = b (e.g. the shifted S - sigma T of the eigen solver). */
 int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
                        const double* b_d, double* x_d, const sucem_solve_opts* opts,

@@ -558,6 +558,23 @@ int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const doubl
   API_END
 }
 
+int sucem_solve_driven_batch(sucem_ctx* ctx, sucem_system* sys, int64_t nb, const double* k0_h,
+                             const double* B_d, double* X_d, const sucem_solve_opts* opts,
+                             sucem_solve_stats* stats_h) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys); NOT_NULL(k0_h); NOT_NULL(B_d); NOT_NULL(X_d);
+  SUCEM_CHECK(nb > 0, SUCEM_ERR_INVALID, "nb must be positive");
+  Context* c = &ctx->c;
+  c->use();
+  std::vector<SolveStats> st(nb);
+  SolveOpts o = convert(opts);
+  if (!opts) o.precond = SUCEM_PRECOND_GRADIENT;
+  solve_system_batch(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, st.data());
+  if (stats_h)
+    for (int64_t j = 0; j < nb; ++j) convert_stats(st[j], stats_h + j);
+  API_END
+}
+
 int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
                        const double* b_d, double* x_d, const sucem_solve_opts* opts,
                        sucem_solve_stats* stats, double* res_hist_h, int64_t hist_cap) {

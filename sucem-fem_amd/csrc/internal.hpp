@@ -146,6 +146,10 @@ void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const Sol
 void ensure_gradient(Context* c, Space* s);
 const double* gradient_diag(Context* c, System* sys, bool masked);
 
+// batch.hip: nb driven systems (one k0 each) on one assembled system
+void solve_system_batch(Context* c, System* sys, int64_t nb, const double* k0,
+                        const double2* B, double2* X, const SolveOpts& o, SolveStats* st);
+
 // eigen.hip
 struct EigOpts {
   int ncv = 0;               // Lanczos basis size (0: max(2 nev + 1, 20))

@@ -86,6 +86,22 @@ class DeviceProblem(object):
                   0 if hist is None else len(hist))
         return st.as_dict()
 
+    def solve_batch(self, k0s, B, X, tol=1e-8, max_iter=50000, check_every=16):
+        """Solve len(k0s) systems at once: B, X are (nb, N) complex128 device tensors."""
+        import torch
+        nb = len(k0s)
+        self._check_vec(B, torch.complex128, nb)
+        self._check_vec(X, torch.complex128, nb)
+        o = _hip.SolveOpts()
+        _hip.load().sucem_solve_opts_default(ctypes.byref(o))
+        o.method, o.precision, o.tol, o.max_iter = _hip.METHOD_COCG, 64, tol, max_iter
+        o.check_every, o.precond = check_every, _hip.PRECOND_GRADIENT
+        st = (_hip.SolveStats * nb)()
+        k0 = np.ascontiguousarray(k0s, np.float64)
+        _hip.call("sucem_solve_driven_batch", self.ctx.handle, self.system.handle.h, nb,
+                  _hip.ptr(k0), _hip.ptr(B), _hip.ptr(X), ctypes.byref(o), st)
+        return [s.as_dict() for s in st]
+
     def tune(self, precision, reps=10):
         """Autotune the fused SpMV launch for this system; returns ({L, U, G}, ms per SpMV)."""
         cfg = np.zeros(3, np.int32)

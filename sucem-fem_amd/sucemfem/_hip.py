@@ -95,6 +95,7 @@ SIGNATURES = {
     "sucem_rhs_point_sources": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _I64, _D, _D, _P]),
     "sucem_solve_opts_default": (None, [_P]),
     "sucem_solve_driven": (ctypes.c_int, [_P, _P, _D, _P, _P, _P, _P, _P, _I64]),
+    "sucem_solve_driven_batch": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _P, _P]),
     "sucem_solve_system": (ctypes.c_int, [_P, _P, _D, _D, _D, _P, _P, _P, _P, _P, _I64]),
     "sucem_eig_opts_default": (None, [_P]),
     "sucem_eigs": (ctypes.c_int, [_P, _P, _D, ctypes.c_int, _P, _P, _P, _P, _P]),

@@ -285,3 +285,55 @@ def test_fused_operator_matches_materialised(dev):
     _hip.call("sucem_system_apply", V.ctx.handle, sysm.handle.h, k0, 32, 0, 3, _hip.ptr(xs),
               _hip.ptr(ys))
     assert np.abs(ys.cpu().numpy() - y_ref).max() <= 2e-5 * np.abs(y_ref).max()
+
+
+def test_batched_frequency_sweep_matches_single_solves(dev):
+    """Five frequencies solved as one batch agree with five separate solves and with the
+    oracle's direct solves (relative residual 1e-10)."""
+    import torch
+    from sucemfem.Utilities.Workloads import DeviceProblem
+    from sucemfem import _hip
+    _say("batched sweep")
+    prob = DeviceProblem(3, 0.3, 2, _hip.MAT_S | _hip.MAT_T | _hip.MAT_S0, (1.0, 2.2, 4.4), seed=3)
+    freqs = np.linspace(0.5e9, 1.2e9, 5)
+    k0s = 2 * np.pi * freqs / O.c0
+    pos = np.array([[0.151, 0.149, 0.152]])
+    mom = np.array([[0, 0, 1e-3]], np.complex128)
+    B = torch.stack([prob.rhs_dipoles(pos, mom, k) for k in k0s]).contiguous()
+    X = torch.zeros_like(B)
+    stats = prob.solve_batch(k0s, B, X, tol=1e-10, max_iter=20000, check_every=8)
+    assert all(s["status"] == 0 and s["rel_residual"] <= 1e-10 for s in stats), stats
+    co, ce = O.unit_cube(3, 3, 3)
+    eps = np.array([1.0, 2.2, 4.4])[prob.region]
+    osp = O.Space(co * 0.3, ce, 2)
+    mats = O.assemble_fast(osp, eps_r=eps)
+    for j, f in enumerate(freqs):
+        A = O.driven_lhs(mats, f)
+        xo = O.solve_direct(A, B[j].cpu().numpy())
+        xb = X[j].cpu().numpy()
+        assert np.linalg.norm(xb - xo) / np.linalg.norm(xo) < 1e-7
+        xs = torch.zeros_like(B[j])
+        st = prob.solve(k0s[j], B[j].contiguous(), xs, precision=64, tol=1e-10, max_iter=20000)
+        assert st["status"] == 0
+        assert np.linalg.norm(xs.cpu().numpy() - xo) / np.linalg.norm(xo) < 1e-7
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_gradient_preconditioner_solve_matches_direct(dev, precision):
+    """COCG with the gradient preconditioner (FP64, and FP32 with FP64 reliable updates)
+    reaches the oracle's direct solution on a heterogeneous order-2 driven cube."""
+    import torch
+    from sucemfem.Utilities.Workloads import DeviceProblem
+    from sucemfem import _hip
+    _say("gradient preconditioner precision %d" % precision)
+    prob = DeviceProblem(4, 0.1, 2, _hip.MAT_S | _hip.MAT_T | _hip.MAT_S0, (1.0, 4.4), seed=1)
+    k0 = 2 * np.pi * 1e9 / O.c0
+    b = prob.rhs_dipoles(np.array([[0.051, 0.049, 0.05]]), np.array([[0, 1e-3, 1e-3]]), k0)
+    x = torch.zeros_like(b)
+    st = prob.solve(k0, b, x, precision=precision, tol=1e-9, max_iter=20000)
+    assert st["status"] == 0 and st["rel_residual"] <= 1e-9, st
+    co, ce = O.unit_cube(4, 4, 4)
+    osp = O.Space(co * 0.1, ce, 2)
+    A = O.driven_lhs(O.assemble_fast(osp, eps_r=np.array([1.0, 4.4])[prob.region]), 1e9)
+    xo = O.solve_direct(A, b.cpu().numpy())
+    assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) < 1e-6
