@@ -102,6 +102,21 @@ class DeviceProblem(object):
                   _hip.ptr(k0), _hip.ptr(B), _hip.ptr(X), ctypes.byref(o), st)
         return [s.as_dict() for s in st]
 
+    def eigs(self, sigma, nev, ncv=0, tol=1e-10, inner_tol=1e-11, vectors=False):
+        """Lowest nev k^2 > sigma of the PEC cavity: (k2, relative residuals, stats[, modes])."""
+        o = _hip.EigOpts()
+        _hip.load().sucem_eig_opts_default(ctypes.byref(o))
+        o.ncv, o.tol, o.inner_tol = int(ncv), tol, inner_tol
+        k2 = np.zeros(nev)
+        resid = np.zeros(nev)
+        vecs = self.ctx.tensor(nev * self.ndofs) if vectors else None
+        st = _hip.EigStats()
+        _hip.call("sucem_eigs", self.ctx.handle, self.system.handle.h, float(sigma), int(nev),
+                  ctypes.byref(o), _hip.ptr(k2), _hip.ptr(resid), _hip.ptr(vecs), ctypes.byref(st))
+        if vectors:
+            return k2, resid, st.as_dict(), vecs.reshape(nev, self.ndofs)
+        return k2, resid, st.as_dict()
+
     def tune(self, precision, reps=10):
         """Autotune the fused SpMV launch for this system; returns ({L, U, G}, ms per SpMV)."""
         cfg = np.zeros(3, np.int32)
