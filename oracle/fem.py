@@ -703,18 +703,14 @@ def discrete_gradient(space):
     grads = np.array([np.stack([_deriv(f, 0), _deriv(f, 1), _deriv(f, 2)]) for f in p2])
     Gloc = apply_duals(2, grads).T                    # (20 nedelec, 10 lagrange)
     ne = len(edges)
-    rows, cols, vals = [], [], []
-    for c in range(len(space.cells)):
-        ldofs = np.concatenate([space.cells[c], nv + space.topo["cell_edges"][c]])
-        rows.append(np.repeat(space.cell_dofs[c], 10))
-        cols.append(np.tile(ldofs, 20))
-        vals.append(Gloc.ravel())
-    G = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                      shape=(space.ndofs, nv + ne)).tocsr()
+    nc = len(space.cells)
+    ldofs = np.concatenate([space.cells, nv + space.topo["cell_edges"]], axis=1)   # (nc, 10)
+    rows = np.repeat(space.cell_dofs, 10, axis=1).ravel()
+    cols = np.tile(ldofs, (1, 20)).ravel()
+    vals = np.tile(Gloc.ravel(), nc)
+    G = sp.coo_matrix((vals, (rows, cols)), shape=(space.ndofs, nv + ne)).tocsr()
     # every cell writes the same value for a shared entry; undo the summation
-    cnt = sp.coo_matrix((np.ones(sum(len(v) for v in vals)),
-                         (np.concatenate(rows), np.concatenate(cols))),
-                        shape=(space.ndofs, nv + ne)).tocsr()
+    cnt = sp.coo_matrix((np.ones(len(vals)), (rows, cols)), shape=(space.ndofs, nv + ne)).tocsr()
     G.data = G.data / cnt.data
     return G
 
@@ -751,3 +747,200 @@ def cocg_jacobi(A, b, tol=1e-8, maxiter=10000):
         if hist[-1] <= tol:
             break
     return x, it + 1, np.array(hist)
+
+
+def gradient_diag(A, G):
+    """diag(G^T A G), column by column: sum_i G_ij (A G)_ij."""
+    AG = (A @ G).tocsr()
+    return np.asarray(G.multiply(AG).sum(axis=0)).ravel()
+
+
+def cocg_gradient(A, b, G, tol=1e-8, maxiter=10000, gdiag=None):
+    """COCG with the additive gradient-space preconditioner, FP64 (scipy CSR).
+
+    M^-1 r = D^-1 r + G diag(G^T A G)^-1 G^T r, D = diag(A); Lagrange nodes whose
+    diagonal is zero (none on a mesh without PEC walls) are dropped. Same
+    recurrences as ``cocg_jacobi`` with z = M^-1 r. Returns ``(x, iterations, history)``.
+    """
+    A = A.tocsr()
+    if gdiag is None:
+        gdiag = gradient_diag(A, G)
+    G = G.tocsr().astype(np.complex128)        # complex once, not per product
+    Gt = G.T.tocsr()
+    D = 1.0 / A.diagonal()
+    W = np.zeros(len(gdiag), np.complex128)
+    nz = gdiag != 0
+    W[nz] = 1.0 / gdiag[nz]
+
+    def prec(r):
+        return D * r + G @ (W * (Gt @ r))
+
+    nb = np.linalg.norm(b)
+    x = np.zeros_like(b, dtype=np.complex128)
+    r = np.asarray(b, np.complex128).copy()
+    z = prec(r)
+    p = z.copy()
+    rho = r @ z
+    hist = []
+    for it in range(maxiter):
+        q = A @ p
+        alpha = rho / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        z = prec(r)
+        rho_new = r @ z
+        p = z + (rho_new / rho) * p
+        rho = rho_new
+        hist.append(np.linalg.norm(r) / nb)
+        if hist[-1] <= tol:
+            break
+    return x, it + 1, np.array(hist)
+
+
+# ---------------------------------------------------------------------------
+# Surface post-processing: power flux and near-to-far-field transform
+# ---------------------------------------------------------------------------
+def boundary_face_samples(space, x, n=6):
+    """Per boundary face: physical quadrature points, weights (area-scaled), the
+    outward unit normal, and E, curl E of the DOF vector ``x`` at the points.
+
+    Yields ``(pts (q,3), w (q,), nrm (3,), E (q,3), curlE (q,3))``.  The rule is the
+    collapsed Gauss rule with ``n``^2 points (exact to degree 2n-2 in each direction).
+    """
+    tp, tw = tri_rule(n)
+    x = np.asarray(x)
+    for c, lf in zip(space.boundary_face_cell, space.boundary_face_local):
+        X = space.coords[space.cells[c]]
+        p, q, r = (X[i] for i in LOCAL_FACES[lf])
+        nrm = np.cross(q - p, r - p)
+        area2 = np.linalg.norm(nrm)
+        nrm = nrm / area2
+        if nrm @ (X[lf] - p) > 0:            # local face lf is opposite vertex lf
+            nrm = -nrm
+        pts = p + np.outer(tp[:, 0], q - p) + np.outer(tp[:, 1], r - p)
+        vals, curls = eval_basis(space.order, X, pts)
+        xl = x[space.cell_dofs[c]]
+        yield pts, tw * area2, nrm, np.einsum("pid,i->pd", vals, xl), np.einsum("pid,i->pd", curls, xl)
+
+
+def surface_flux(space, x, k0, mu_r=1.0, n=6):
+    """Real power flux through the mesh boundary, ``PostProcessing/power_flux.py:52-63``:
+    (1/(k0 Z0)) int n . (E_r x (-curl E_i / mu_r) + E_i x (curl E_r / mu_r)) dS."""
+    total = 0.0
+    for _, w, nrm, E, cE in boundary_face_samples(space, x, n):
+        f = np.cross(E.real, -cE.imag / mu_r) + np.cross(E.imag, cE.real / mu_r)
+        total += w @ (f @ nrm)
+    return total / k0 / Z0
+
+
+def ntff_directions(theta, phi):
+    st, ct, sp_, cp = np.sin(theta), np.cos(theta), np.sin(phi), np.cos(phi)
+    r_hat = np.array([st * cp, st * sp_, ct])
+    theta_hat = np.array([ct * cp, ct * sp_, -st])
+    phi_hat = np.array([-sp_, cp, 0.0])
+    return r_hat, theta_hat, phi_hat
+
+
+def surface_ntff(space, x, frequency, theta_deg, phi_deg, n=6):
+    """Far field (E_theta, E_phi) at unit radius from the equivalent surface currents,
+    ``PostProcessing/surface_ntff.py:36-155`` and ``ntff_expressions.py:22-37``.
+
+    J = n x H with H = j curl E / (k0 Z0), M = -n x E, both weighted by
+    exp(j k0 r'.r_hat); E_theta = -f (L_phi + Z0 N_theta), E_phi = f (L_theta - Z0 N_phi),
+    f = j k0 exp(-j k0) / (4 pi).  Returns complex (len(theta), 2).
+    """
+    k0 = frequency * 2 * np.pi / c0
+    samples = list(boundary_face_samples(space, x, n))
+    out = []
+    for th, ph in zip(np.deg2rad(np.atleast_1d(theta_deg)), np.deg2rad(np.atleast_1d(phi_deg))):
+        r_hat, theta_hat, phi_hat = ntff_directions(th, ph)
+        Nv = np.zeros(3, complex)
+        Lv = np.zeros(3, complex)
+        for pts, w, nrm, E, cE in samples:
+            e = np.exp(1j * k0 * (pts @ r_hat))
+            H = 1j * cE / (k0 * Z0)
+            Nv += (w * e) @ np.cross(nrm, H)
+            Lv += (w * e) @ (-np.cross(nrm, E))
+        f = 1j * k0 * np.exp(-1j * k0) / (4 * np.pi)
+        out.append((-f * (Lv @ phi_hat + Z0 * (Nv @ theta_hat)),
+                    f * (Lv @ theta_hat - Z0 * (Nv @ phi_hat))))
+    return np.array(out)
+
+
+def boundary_edge_cells(space):
+    """Cells with at least one edge on the boundary, ``Geometry.py:76-104``."""
+    on_b = np.zeros(len(space.topo["edges"]), bool)
+    for c, lf in zip(space.boundary_face_cell, space.boundary_face_local):
+        fv = set(LOCAL_FACES[lf])
+        for le, (a, b) in enumerate(LOCAL_EDGES):
+            if a in fv and b in fv:
+                on_b[space.topo["cell_edges"][c, le]] = True
+    return np.flatnonzero(on_b[space.topo["cell_edges"]].any(axis=1))
+
+
+def interpolate_boundary(space, f):
+    """DOF values of ``f`` (points (q,3) -> complex (q,3)) on the boundary DOFs, zero
+    elsewhere: the ``DirichletBC(V, expr, on_boundary).apply`` of
+    ``Interpolation/Interpolation.py:71-83``.  Each DOF is its nodal functional,
+    f(x_i) . (J t_i), with the dual point and tangent mapped from the reference."""
+    duals = _dual_functionals(space.order)
+    P = np.array([p for p, _ in duals])
+    Tt = np.array([t for _, t in duals])
+    mask = space.pec_mask()
+    g = np.zeros(space.ndofs, np.complex128)
+    for c in np.unique(space.boundary_face_cell):
+        X = space.coords[space.cells[c]]
+        J = np.stack([X[1] - X[0], X[2] - X[0], X[3] - X[0]], axis=1)
+        pts = X[0] + P @ J.T
+        vals = np.einsum("qd,qd->q", f(pts), Tt @ J.T)
+        d = space.cell_dofs[c]
+        m = mask[d]
+        g[d[m]] = vals[m]
+    return g
+
+
+def em_functional(space, E, g, k0, cells):
+    """sum over ``cells`` of g_c^T (S_e - k0^2 T_e) E_c with eps_r = mu_r = 1,
+    ``PostProcessing/PostProcessing.py:107-172`` (bilinear, no conjugation)."""
+    total = 0.0 + 0.0j
+    for c in cells:
+        T, S = element_matrices(space.order, space.coords[space.cells[c]])
+        d = space.cell_dofs[c]
+        total += g[d] @ ((S - k0 ** 2 * T) @ E[d])
+    return total
+
+
+def variational_flux(space, x, k0):
+    """``power_flux.VariationalSurfaceFlux.calc_flux`` (``power_flux.py:65-133``):
+    conj(F(E, g)) with g = j conj(E on the boundary DOFs) / (k0 Z0), over the
+    cells that touch the boundary through an edge."""
+    x = np.asarray(x, np.complex128)
+    g = 1j * np.where(space.pec_mask(), x, 0).conjugate() / k0 / Z0
+    return em_functional(space, x, g, k0, boundary_edge_cells(space)).conjugate()
+
+
+def ntff_testing_field(rhat, ahat, k0):
+    """(j/k0) rhat x (ahat x rhat) exp(j k0 rhat . x), ``variational_ntff.py:31-61``."""
+    cf = np.cross(rhat, np.cross(ahat, rhat)) / k0
+    return lambda pts: 1j * cf[None, :] * np.exp(1j * k0 * (pts @ rhat))[:, None]
+
+
+def variational_ntff(space, x, frequency, theta_deg, phi_deg, n=6):
+    """``variational_ntff.NTFF.calc_pt`` (``variational_ntff.py:64-131``): the H
+    contribution from F(E, g_a) with the boundary interpolant g_a of the testing
+    field, the E contribution from the surface L potential."""
+    k0 = frequency * 2 * np.pi / c0
+    x = np.asarray(x, np.complex128)
+    cells = boundary_edge_cells(space)
+    samples = list(boundary_face_samples(space, x, n))
+    out = []
+    for th, ph in zip(np.deg2rad(np.atleast_1d(theta_deg)), np.deg2rad(np.atleast_1d(phi_deg))):
+        r_hat, theta_hat, phi_hat = ntff_directions(th, ph)
+        EH = [em_functional(space, x, interpolate_boundary(space, ntff_testing_field(r_hat, a, k0)),
+                            k0, cells) for a in (theta_hat, phi_hat)]
+        Lv = np.zeros(3, complex)
+        for pts, w, nrm, E, _ in samples:
+            Lv += (w * np.exp(1j * k0 * (pts @ r_hat))) @ (-np.cross(nrm, E))
+        f = 1j * k0 * np.exp(-1j * k0) / (4 * np.pi)
+        out.append((f * (-(Lv @ phi_hat) + EH[0]), f * (Lv @ theta_hat + EH[1])))
+    return np.array(out)

@@ -25,9 +25,14 @@ Fixtures written (data only: inputs and expected outputs):
   ``PostProcessing/tests/data/ntff_generate_dofs.py:40-95``).
 * ``driven_order2_filament_x.npz`` -- ``power_flux_reference_dofs_...pickle`` (1314 DOFs,
   generator ``PostProcessing/tests/data/power_flux_generate_dofs.py:26-102``) plus the
-  cells of its DOLFIN mesh XML.
+  cells and the (6-digit) vertex coordinates of its DOLFIN mesh XML, which
+  ``test_power_flux.py:42`` loads.
 * ``driven_order2_filament9_x.npz`` -- ``examples/filament_source/data/dofs-2-...-100.pickle``
   (30186 DOFs).
+* ``ntff_surface_order2.npz``, ``ntff_variational_order2.npz`` -- far fields of the
+  solution in ``driven_order2_dipole_x.npz`` at six angles
+  (``PostProcessing/tests/test_ntff.py:58-103``).
+* ``ntff_interpolant.npz`` -- ``interpolant_test_data.pickle`` (``test_ntff.py:36-55``).
 * ``unitcube1_topology.npz`` -- the boundary-edge count of UnitCube(1,1,1)
   asserted by ``sucemfem/tests/test_Geometry.py:31-35``.
 """
@@ -256,6 +261,14 @@ def read_dolfin_mesh_cells(path):
     return nv, cells
 
 
+def read_dolfin_mesh_vertices(path):
+    """Vertex coordinates as written (6 significant digits) in a DOLFIN mesh XML file."""
+    txt = open(path).read()
+    v = re.findall(r'<vertex index="(\d+)" x="([^"]+)" y="([^"]+)" z="([^"]+)"', txt)
+    return np.array([[float(c) for c in r[1:]] for r in sorted(v, key=lambda r: int(r[0]))],
+                    dtype=np.float64)
+
+
 def main(ref):
     out = HERE
     # 1. order-1 driven LHS/RHS on InscribedTet
@@ -289,15 +302,34 @@ def main(ref):
     d = read_pickle(os.path.join(ref, base + ".pickle"))
     nv, cells = read_dolfin_mesh_cells(os.path.join(ref, base + "_mesh.xml"))
     keep = {k: v for k, v in d.items() if isinstance(v, (np.ndarray, float, int, np.floating))}
+    coords = read_dolfin_mesh_vertices(os.path.join(ref, base + "_mesh.xml"))
     np.savez_compressed(os.path.join(out, "driven_order2_filament_x.npz"),
-                        mesh_nv=nv, mesh_cells=cells, **{k: np.asarray(v) for k, v in keep.items()})
+                        mesh_nv=nv, mesh_cells=cells, mesh_coords=coords,
+                        **{k: np.asarray(v) for k, v in keep.items()})
     # 5. larger order-2 driven solution (UnitCube(9), lambda/4 filament)
     d = read_pickle(os.path.join(ref, "examples/filament_source/data/"
                                  "dofs-2-0.299792458-0.0499654096667-0.0749481145-100.pickle"))
     keep = {k: v for k, v in d.items() if isinstance(v, (np.ndarray, float, int, np.floating))}
     np.savez_compressed(os.path.join(out, "driven_order2_filament9_x.npz"),
                         **{k: np.asarray(v) for k, v in keep.items()})
-    # 6. UnitCube(1,1,1) boundary-edge count
+    # 6. far fields of the solution of 3. (surface and variational NTFF) at six angles
+    for kind in ("surface", "variational"):
+        d = read_pickle(os.path.join(ref, "sucemfem/PostProcessing/tests/data/reference_%s_ntff"
+                                     "-2-0.149896229-0.0499654096667.pickle" % kind))
+        ff = d["ff_result_data"]
+        np.savez_compressed(os.path.join(out, "ntff_%s_order2.npz" % kind),
+                            theta_deg=np.asarray(ff["theta_deg"], np.float64),
+                            phi_deg=np.asarray(ff["phi_deg"], np.float64),
+                            E_ff=np.asarray(ff["E_ff"], np.complex128),
+                            E_theta_analytical=np.asarray(ff["E_theta_analytical"], np.complex128))
+    # 7. variational NTFF testing-function interpolant (test_ntff.py:36-55)
+    d = read_pickle(os.path.join(ref, "sucemfem/PostProcessing/tests/data/interpolant_test_data.pickle"))
+    np.savez_compressed(os.path.join(out, "ntff_interpolant.npz"), k0=float(d["k0"]),
+                        rhat=np.asarray(d["rhat"], np.float64),
+                        ahats=np.asarray(d["ahats"], np.float64),
+                        coords=np.asarray(d["coords"], np.float64),
+                        vals=np.asarray(d["vals"], np.complex128))
+    # 8. UnitCube(1,1,1) boundary-edge count
     np.savez_compressed(os.path.join(out, "unitcube1_topology.npz"), n_boundary_edges=18)
     for f in sorted(os.listdir(out)):
         if f.endswith(".npz"):

@@ -337,3 +337,27 @@ def test_gradient_preconditioner_solve_matches_direct(dev, precision):
     A = O.driven_lhs(O.assemble_fast(osp, eps_r=np.array([1.0, 4.4])[prob.region]), 1e9)
     xo = O.solve_direct(A, b.cpu().numpy())
     assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) < 1e-6
+
+
+def test_gradient_preconditioner_iterations_match_oracle(dev):
+    """The FP64 device solve with the gradient preconditioner follows the oracle's
+    gradient-preconditioned COCG: same iteration count up to the convergence-check
+    interval (a wrong diag(G^T A G), G or D would change the count)."""
+    import torch
+    from sucemfem.Utilities.Workloads import DeviceProblem
+    from sucemfem import _hip
+    _say("gradient preconditioner iteration count vs oracle")
+    prob = DeviceProblem(4, 0.1, 2, _hip.MAT_S | _hip.MAT_T | _hip.MAT_S0, (1.0, 4.4), seed=1)
+    k0 = 2 * np.pi * 1e9 / O.c0
+    b = prob.rhs_dipoles(np.array([[0.051, 0.049, 0.05]]), np.array([[0, 1e-3, 1e-3]]), k0)
+    x = torch.zeros_like(b)
+    st = prob.solve(k0, b, x, precision=64, tol=1e-9, max_iter=20000, check_every=1)
+    co, ce = O.unit_cube(4, 4, 4)
+    osp = O.Space(co * 0.1, ce, 2)
+    A = O.driven_lhs(O.assemble_fast(osp, eps_r=np.array([1.0, 4.4])[prob.region]), 1e9)
+    xo, its, _ = O.cocg_gradient(A, b.cpu().numpy(), O.discrete_gradient(osp), tol=1e-9,
+                                 maxiter=20000)
+    assert st["status"] == 0
+    # equal up to rounding; a true-residual check that misses tol by a hair may add a few
+    assert -2 <= st["iterations"] - its <= max(4, its // 20), (st["iterations"], its)
+    assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) < 1e-7

@@ -215,3 +215,102 @@ def test_face_matrix_planted_error_is_caught(golden):
     m["S_0"] = O.fem._scatter(sp_, wrong)
     A = O.driven_lhs(m, float(z["frequency"])).toarray()
     assert not np.allclose(A, z["lhs"], rtol=1e-10, atol=3e-15)
+
+
+def test_cocg_gradient_matches_direct_and_cuts_iterations():
+    """Gradient-space preconditioned COCG reaches the direct solution of a heterogeneous
+    order-2 driven cube and needs far fewer iterations than Jacobi-COCG; the
+    preconditioner is complex symmetric (required by COCG)."""
+    co, ce = O.box_mesh([0.1, 0.1, 0.1], 3)
+    sp_ = O.Space(co, ce, 2)
+    rng = np.random.default_rng(3)
+    eps = np.array([1.0, 4.4])[rng.integers(0, 2, len(ce))]
+    A = O.driven_lhs(O.assemble_fast(sp_, eps_r=eps), 1e9)
+    b = rng.standard_normal(sp_.ndofs) + 1j * rng.standard_normal(sp_.ndofs)
+    xd = O.solve_direct(A, b)
+    G = O.discrete_gradient(sp_)
+    x, its, hist = O.cocg_gradient(A, b, G, tol=1e-10, maxiter=5000)
+    assert hist[-1] <= 1e-10
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-7
+    _, its_j, _ = O.cocg_jacobi(A, b, tol=1e-10, maxiter=20000)
+    assert its * 3 < its_j, (its, its_j)
+    gd = O.gradient_diag(A, G)
+    assert np.allclose(gd, (G.T @ A @ G).diagonal())
+    Minv = (scipy.sparse.diags(1 / A.diagonal()) + G @ scipy.sparse.diags(1 / gd) @ G.T).toarray()
+    assert np.allclose(Minv, Minv.T)
+
+
+# --- post-processing: power flux and near-to-far-field (PostProcessing/tests) ---------
+def _flux_space(golden):
+    d = golden("driven_order2_filament_x.npz")
+    # test_power_flux.py:42 loads the DOLFIN XML mesh, whose coordinates carry 6 digits
+    return O.Space(d["mesh_coords"], d["mesh_cells"], 2), d["x"], 2 * np.pi * float(d["freq"]) / O.c0
+
+
+def test_surface_flux_fixture(golden):
+    """``test_power_flux.py:53-54``: SurfaceFlux.calc_flux() == 26.3586862457 (7 places)."""
+    sp_, x, k0 = _flux_space(golden)
+    assert round(O.surface_flux(sp_, x, k0) - 26.3586862457, 7) == 0
+
+
+def test_variational_flux_fixture(golden):
+    """``test_power_flux.py:72-73``: VariationalSurfaceFlux == 26.9015957282-7.85747240242e-08j."""
+    sp_, x, k0 = _flux_space(golden)
+    assert round(abs(O.variational_flux(sp_, x, k0) - (26.9015957282 - 7.85747240242e-08j)), 7) == 0
+
+
+def test_flux_planted_error_is_caught(golden):
+    """Dropping the E_i x curl E_r term (or flipping the normal) moves the flux off the fixture."""
+    sp_, x, k0 = _flux_space(golden)
+    total = 0.0
+    for _, w, nrm, E, cE in O.boundary_face_samples(sp_, x):
+        total += w @ (np.cross(E.real, -cE.imag) @ nrm)
+    assert abs(total / k0 / O.Z0 - 26.3586862457) > 1e-3
+
+
+def _ntff_space(golden):
+    d = golden("driven_order2_dipole_x.npz")
+    co, ce = O.centred_cube(d["domain_size"], float(d["max_edge_len"]))
+    return O.Space(co, ce, 2), d["x"], float(d["freq"])
+
+
+def test_surface_ntff_fixture(golden):
+    """``test_ntff.py:84-95``: six far-field points, rtol 1e-12, atol 1e-7."""
+    sp_, x, f = _ntff_space(golden)
+    ff = golden("ntff_surface_order2.npz")
+    E = O.surface_ntff(sp_, x, f, ff["theta_deg"], ff["phi_deg"])
+    assert np.allclose(E, ff["E_ff"], rtol=1e-12, atol=1e-7)
+
+
+def test_variational_ntff_fixture(golden):
+    """``test_ntff.py:98-103``: variational NTFF, same tolerances."""
+    sp_, x, f = _ntff_space(golden)
+    ff = golden("ntff_variational_order2.npz")
+    E = O.variational_ntff(sp_, x, f, ff["theta_deg"], ff["phi_deg"])
+    assert np.allclose(E, ff["E_ff"], rtol=1e-12, atol=1e-7)
+
+
+def test_ntff_testing_field_fixture(golden):
+    """``test_ntff.py:36-55``: (j/k0) rhat x (ahat x rhat) exp(j k0 rhat.x) at 8 points."""
+    d = golden("ntff_interpolant.npz")
+    for a, want in zip(d["ahats"], d["vals"]):
+        got = O.ntff_testing_field(d["rhat"], a, float(d["k0"]))(d["coords"])
+        assert np.allclose(got, want, rtol=1e-10, atol=1e-7)
+
+
+def test_boundary_edge_cells_unit_cube():
+    """Cells touching the boundary through an edge: all but the fully interior boxes'
+    cells whose edges avoid the boundary; on UnitCube(1) every cell qualifies, and on
+    UnitCube(3) none of the central box's cells is missed."""
+    co, ce = O.unit_cube(1, 1, 1)
+    assert len(O.boundary_edge_cells(O.Space(co, ce, 1))) == 6
+    co, ce = O.unit_cube(3, 3, 3)
+    sp_ = O.Space(co, ce, 1)
+    marked = set(O.boundary_edge_cells(sp_).tolist())
+    on_b = lambda p: np.any((np.abs(p) < 1e-12) | (np.abs(p - 1) < 1e-12))  # noqa: E731
+    for c in range(len(ce)):
+        X = co[ce[c]]
+        touches = any(on_b(X[a]) and on_b(X[b]) and
+                      np.any((np.abs(X[a] - X[b]) < 1e-12) & ((np.abs(X[a]) < 1e-12) | (np.abs(X[a] - 1) < 1e-12)))
+                      for a, b in O.LOCAL_EDGES)
+        assert (c in marked) == touches
