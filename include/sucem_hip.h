@@ -208,6 +208,25 @@ int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,
                const sucem_eig_opts* opts, double* k2_h, double* resid_h, double* vecs_d,
                sucem_eig_stats* stats);
 
+/* ---- post-processing of a solution x_d (complex128[N] on the device) ----
+ * Real power flux through the mesh boundary,
+ *   (1/(k0 Z0)) int n . (E_r x (-curl E_i / mu_r) + E_i x (curl E_r / mu_r)) dS,
+ * replacing SurfaceFlux.calc_flux (PostProcessing/power_flux.py:52-63).  use_mu = 0 takes
+ * mu_r = 1 (the reference's default), 1 the mesh's per-cell mu_r. */
+int sucem_surface_flux(sucem_ctx* ctx, sucem_space* V, const double* x_d, double k0, int use_mu,
+                       double* flux);
+/* Variational power flux, replacing VariationalSurfaceFlux.calc_flux
+ * (power_flux.py:65-133): conj of the free-space functional <curl E, curl g> - k0^2 <E, g>
+ * over the cells touching the boundary through an edge, g = j conj(E)/(k0 Z0) on the boundary
+ * DOFs.  flux: complex (2 doubles). */
+int sucem_variational_flux(sucem_ctx* ctx, sucem_space* V, const double* x_d, double k0,
+                           double* flux);
+/* Far field at unit radius, (E_theta, E_phi) per direction (theta_h, phi_h in radians):
+ * NTFF.calc_pt of PostProcessing/surface_ntff.py:125-155 (variational = 0) or
+ * PostProcessing/variational_ntff.py:91-131 (variational = 1).  E_ff_h: complex128[n][2]. */
+int sucem_ntff(sucem_ctx* ctx, sucem_space* V, const double* x_d, double k0, int variational,
+               int64_t n, const double* theta_h, const double* phi_h, double* E_ff_h);
+
 /* ---- generic complex CSR (LinalgSolvers on a user scipy matrix, LinalgSolvers.py:30-322) ---- */
 int sucem_csr_create(sucem_ctx* ctx, int64_t n, int64_t nnz, const int64_t* indptr_h,
                      const int32_t* indices_h, const double* values_h, sucem_csr** out);

@@ -591,6 +591,39 @@ int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, 
   API_END
 }
 
+int sucem_surface_flux(sucem_ctx* ctx, sucem_space* V, const double* x_d, double k0, int use_mu,
+                       double* flux) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(V); NOT_NULL(x_d); NOT_NULL(flux);
+  SUCEM_CHECK(k0 > 0, SUCEM_ERR_INVALID, "k0 must be positive");
+  ctx->c.use();
+  *flux = surface_flux(&ctx->c, &V->s, x_d, k0, use_mu != 0);
+  API_END
+}
+
+int sucem_variational_flux(sucem_ctx* ctx, sucem_space* V, const double* x_d, double k0,
+                           double* flux) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(V); NOT_NULL(x_d); NOT_NULL(flux);
+  SUCEM_CHECK(k0 > 0, SUCEM_ERR_INVALID, "k0 must be positive");
+  ctx->c.use();
+  variational_flux(&ctx->c, &V->s, x_d, k0, flux);
+  API_END
+}
+
+int sucem_ntff(sucem_ctx* ctx, sucem_space* V, const double* x_d, double k0, int variational,
+               int64_t n, const double* theta_h, const double* phi_h, double* E_ff_h) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(V); NOT_NULL(x_d);
+  SUCEM_CHECK(k0 > 0, SUCEM_ERR_INVALID, "k0 must be positive");
+  SUCEM_CHECK(n >= 0 && n <= 65535, SUCEM_ERR_INVALID, "direction count must be in [0, 65535]");
+  if (n == 0) return SUCEM_OK;
+  NOT_NULL(theta_h); NOT_NULL(phi_h); NOT_NULL(E_ff_h);
+  ctx->c.use();
+  ntff(&ctx->c, &V->s, x_d, k0, variational != 0, n, theta_h, phi_h, E_ff_h);
+  API_END
+}
+
 void sucem_eig_opts_default(sucem_eig_opts* o) {
   if (!o) return;
   EigOpts d;

@@ -168,6 +168,17 @@ struct EigStats {
 void eigs_system(Context* c, const System* sys, double sigma, int nev, const EigOpts& o,
                  double* lam_h, double* resid_h, double2* vecs_d, EigStats& st);
 
+// postproc.hip -- field post-processing on the boundary
+// Free-space constants as the reference defines them (Consts.py:24-33):
+// mu0 = pi*4*1e-7 evaluated left to right, Z0 = mu0*c0.
+constexpr double kC0 = 299792458.0;
+constexpr double kMu0 = M_PI * 4 * 1e-7;
+constexpr double kZ0 = kMu0 * kC0;
+double surface_flux(Context* c, const Space* s, const void* x, double k0, bool use_mu);
+void variational_flux(Context* c, const Space* s, const void* x, double k0, double out[2]);
+void ntff(Context* c, const Space* s, const void* x, double k0, bool variational, int64_t n,
+          const double* theta, const double* phi, double* E_ff);
+
 }  // namespace sucem
 
 struct sucem_mesh { sucem::Mesh m; };

@@ -99,6 +99,9 @@ SIGNATURES = {
     "sucem_solve_system": (ctypes.c_int, [_P, _P, _D, _D, _D, _P, _P, _P, _P, _P, _I64]),
     "sucem_eig_opts_default": (None, [_P]),
     "sucem_eigs": (ctypes.c_int, [_P, _P, _D, ctypes.c_int, _P, _P, _P, _P, _P]),
+    "sucem_surface_flux": (ctypes.c_int, [_P, _P, _P, _D, ctypes.c_int, _P]),
+    "sucem_variational_flux": (ctypes.c_int, [_P, _P, _P, _D, _P]),
+    "sucem_ntff": (ctypes.c_int, [_P, _P, _P, _D, ctypes.c_int, _I64, _P, _P, _P]),
     "sucem_csr_create": (ctypes.c_int, [_P, _I64, _I64, _P, _P, _P, _PP]),
     "sucem_csr_apply": (ctypes.c_int, [_P, _P, _P, _P]),
     "sucem_csr_solve": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _I64]),
