@@ -134,8 +134,15 @@ int sucem_space_info(const sucem_space* s, int64_t* info8_h);
 int sucem_space_export_pattern(sucem_space* s, int64_t* indptr_h, int32_t* indices_h);
 int sucem_space_export_dofmap(sucem_space* s, int64_t* cell_dofs_h);
 int sucem_space_export_entities(sucem_space* s, int64_t* edges_h, int64_t* faces_h);
-/* DOFs constrained by PEC walls on every boundary facet (essential.py:86-99). */
+/* PEC (essential) DOFs: by default the closure of every boundary facet, the PEC walls of
+ * essential.py:86-99. */
 int sucem_space_export_pec(sucem_space* s, uint8_t* mask_h);
+/* Set the PEC DOFs to the closure of the facets flagged in facet_flag_h (nfaces bytes, in
+ * the space's facet numbering; null = every boundary facet), replacing the current set or,
+ * with union_mode != 0, adding to it: the DirichletBC(V, 0, mesh_function, region) of
+ * essential.py:65-81 with facet_flag_h[f] = (mesh_function[f] == region). */
+int sucem_space_mark_pec(sucem_ctx* ctx, sucem_space* s, const uint8_t* facet_flag_h,
+                         int union_mode);
 int sucem_space_destroy(sucem_space* s);
 
 /* ---- assembly (replaces SystemMatrices.calc_system_matrices, SystemMatrices.py:46-58) ---- */

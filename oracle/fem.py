@@ -469,14 +469,22 @@ class Space(object):
             loc += [12 + 2 * local_face, 13 + 2 * local_face]
         return np.array(loc, np.int64)
 
-    def pec_mask(self):
-        """DOFs on boundary facets (edges and faces of boundary faces).
+    def pec_mask(self, facet_flags=None):
+        """DOFs on the closure (edges and faces) of the flagged facets; default: the
+        boundary facets.
 
-        ``BoundaryConditions/essential.py:86-99`` marks all boundary facets; the
-        topological DirichletBC constrains the DOFs of their closure.
+        ``BoundaryConditions/essential.py:86-99`` marks all boundary facets, and
+        ``essential.py:65-81`` takes a facet MeshFunction and region number; the
+        topological DirichletBC constrains the DOFs of the closure.
         """
         mask = np.zeros(self.ndofs, bool)
-        for c, lf in zip(self.boundary_face_cell, self.boundary_face_local):
+        if facet_flags is None:
+            pairs = zip(self.boundary_face_cell, self.boundary_face_local)
+        else:
+            flagged = np.flatnonzero(np.asarray(facet_flags))
+            fc, fl = self.topo["face_cells"], self.topo["face_local"]
+            pairs = [(fc[f, s], fl[f, s]) for f in flagged for s in (0, 1) if fc[f, s] >= 0]
+        for c, lf in pairs:
             mask[self.cell_dofs[c, self.face_dofs_local(lf)]] = True
         return mask
 
@@ -625,7 +633,8 @@ def solve_direct(A, b):
 # ---------------------------------------------------------------------------
 # Eigen (resonant cavity, PEC walls)
 # ---------------------------------------------------------------------------
-def cavity_eigs(space, nev, sigma=1.1, eps_r=None, inv_mu_r=None, dense_limit=4000):
+def cavity_eigs(space, nev, sigma=1.1, eps_r=None, inv_mu_r=None, dense_limit=4000,
+                facet_flags=None):
     """Lowest ``nev`` nonzero k^2 of the PEC-eliminated pencil (S_II, T_II).
 
     Follows ``EMVectorWaveEigenproblem.py:61-105`` with the spectrum-shift
@@ -635,7 +644,7 @@ def cavity_eigs(space, nev, sigma=1.1, eps_r=None, inv_mu_r=None, dense_limit=40
     (largest algebraic), which excludes the gradient null space (nu = -1/sigma).
     """
     mats = assemble_fast(space, eps_r, inv_mu_r, abc=False)
-    free = np.flatnonzero(~space.pec_mask())
+    free = np.flatnonzero(~space.pec_mask(facet_flags))
     S = mats["S"][free][:, free]
     T = mats["M"][free][:, free]
     if len(free) <= dense_limit:

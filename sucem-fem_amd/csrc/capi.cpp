@@ -304,6 +304,15 @@ int sucem_space_export_pec(sucem_space* s, uint8_t* mask_h) {
   API_END
 }
 
+int sucem_space_mark_pec(sucem_ctx* ctx, sucem_space* s, const uint8_t* facet_flag_h,
+                         int union_mode) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(s);
+  ctx->c.use();
+  space_mark_pec(&ctx->c, &s->s, facet_flag_h, union_mode != 0);
+  API_END
+}
+
 int sucem_space_destroy(sucem_space* s) {
   API_BEGIN
   if (!s) return SUCEM_OK;

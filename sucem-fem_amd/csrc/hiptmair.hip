@@ -236,7 +236,9 @@ const double* gradient_diag(Context* c, System* sys, bool masked) {
   Space* s = sys->space;
   ensure_gradient(c, s);
   DBuf<double>& d = sys->gdiag[masked ? 1 : 0];
+  if (masked && sys->gdiag_pec_version != s->pec_version) d.release();   // PEC mask changed
   if (!d.p) {
+    if (masked) sys->gdiag_pec_version = s->pec_version;
     SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S and T must be assembled");
     d.alloc(3 * s->nlag);
     const bool has_s0 = (sys->which & SUCEM_MAT_S0) && s->nnz0 > 0;

@@ -37,7 +37,9 @@ struct Space {
   DBuf<int32_t> edges;            // nE*2
   DBuf<int32_t> faces;            // nF*3
   DBuf<int32_t> bf_cell, bf_local;
-  DBuf<uint8_t> pec;              // ndofs
+  DBuf<uint8_t> pec;              // ndofs: essential (PEC) DOFs, default = bdof
+  DBuf<uint8_t> bdof;             // ndofs: DOFs of the closure of the boundary faces
+  int pec_version = 0;            // bumped whenever pec changes
   DBuf<int32_t> row_ptr, col, diag;
   DBuf<int32_t> seg;
   DBuf<uint32_t> perm;
@@ -65,6 +67,7 @@ struct System {
   // diag(G^T S G), diag(G^T T G), diag(G^T S0 G) per Lagrange node, without / with the
   // PEC rows removed (built on first use by the gradient preconditioner)
   DBuf<double> gdiag[2];
+  int gdiag_pec_version = -1;     // Space::pec_version gdiag[1] was built with
   Space* space = nullptr;
   unsigned which = 0;
   DBuf<double2> st64;
@@ -96,6 +99,9 @@ int64_t build_csr_from_pairs(Context* c, int64_t n, DBuf<uint64_t>& keys, DBuf<u
                              int B, int64_t nrows, DBuf<int32_t>& row_ptr, DBuf<int32_t>& col,
                              DBuf<int32_t>& seg, DBuf<uint32_t>& perm);
 int key_bits(int64_t n);
+// PEC DOFs = closure of the facets with facet_flag_h[f] != 0 (nF flags; null = the
+// boundary facets); union_mode ORs into the current mask instead of replacing it.
+void space_mark_pec(Context* c, Space* s, const uint8_t* facet_flag_h, bool union_mode);
 
 // assemble.hip
 void assemble_system(Context* c, Space* s, System* sys, unsigned which);

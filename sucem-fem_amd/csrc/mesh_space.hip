@@ -266,6 +266,19 @@ __global__ void k_pec(int64_t nbf, int nloc, int nfd, const int32_t* bf_cell,
   }
 }
 
+// Closure of the flagged facets: every (cell, local face) whose global face is flagged.
+__global__ void k_pec_facets(int64_t nc, int nloc, int nfd, const int32_t* cell_faces,
+                             const uint8_t* flag, const int32_t* cd, const int32_t* face_dofs,
+                             uint8_t* pec) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nc * 4;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (!flag[cell_faces[i]]) continue;
+    const int64_t c = i / 4;
+    const int lf = (int)(i % 4);
+    for (int a = 0; a < nfd; ++a) pec[cd[c * nloc + face_dofs[lf * nfd + a]]] = 1;
+  }
+}
+
 __global__ void k_cell_pair_keys(int64_t nc, int nloc, int B, const int32_t* cd, uint64_t* keys,
                                  uint32_t* vals) {
   int64_t kk = (int64_t)nloc * nloc;
@@ -358,6 +371,12 @@ __global__ void k_set_u8(int64_t n, uint8_t* p, uint8_t v) {
     p[i] = v;
 }
 
+__global__ void k_or_u8(int64_t n, const uint8_t* a, uint8_t* b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] |= a[i];
+}
+
 __global__ void k_count_u8(int64_t n, const uint8_t* p, unsigned long long* out) {
   unsigned long long s = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -368,6 +387,14 @@ __global__ void k_count_u8(int64_t n, const uint8_t* p, unsigned long long* out)
 }
 
 constexpr int TB = 256;
+
+int64_t count_flags(Context* c, const uint8_t* p, int64_t n) {
+  DBuf<unsigned long long> cnt(1);
+  unsigned long long z = 0;
+  SUCEM_HIP(hipMemcpyAsync(cnt.p, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+  SUCEM_LAUNCH(k_count_u8, grid_for(n, TB, 4096), TB, 0, c->stream, n, p, cnt.p);
+  return (int64_t)read_scalar(c, cnt.p);
+}
 
 // Number the entities with NV vertices (2 = edges, 3 = faces) that each cell has NL of.
 template <int NV, int NL>
@@ -567,14 +594,9 @@ void space_build(Context* c, Space* s) {
   if (s->nbf)
     SUCEM_LAUNCH(k_pec, grid_for(s->nbf * s->nfd, TB), TB, 0, c->stream, s->nbf, s->nloc, s->nfd,
                  s->bf_cell.p, s->bf_local.p, s->cell_dofs.p, s->ref_face_dofs.p, s->pec.p);
-  {
-    DBuf<unsigned long long> cnt(1);
-    unsigned long long z = 0;
-    SUCEM_HIP(hipMemcpyAsync(cnt.p, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
-    SUCEM_LAUNCH(k_count_u8, grid_for(s->ndofs, TB, 4096), TB, 0, c->stream, s->ndofs, s->pec.p,
-                 cnt.p);
-    s->npec = (int64_t)read_scalar(c, cnt.p);
-  }
+  s->bdof.alloc(s->ndofs);
+  SUCEM_HIP(hipMemcpyAsync(s->bdof.p, s->pec.p, s->ndofs, hipMemcpyDeviceToDevice, c->stream));
+  s->npec = count_flags(c, s->pec.p, s->ndofs);
   // 4. CSR pattern of all cell-local pairs + the assembly gather map
   const int B = bits_for(s->ndofs);
   {
@@ -607,6 +629,29 @@ void space_build(Context* c, Space* s) {
                  s->s0_col.p, s->s0_diag.p);
   }
   c->sync();
+}
+
+void space_mark_pec(Context* c, Space* s, const uint8_t* facet_flag_h, bool union_mode) {
+  if (!facet_flag_h) {
+    if (union_mode) {
+      SUCEM_LAUNCH(k_or_u8, grid_for(s->ndofs, TB, 4096), TB, 0, c->stream, s->ndofs, s->bdof.p,
+                   s->pec.p);
+    } else {
+      SUCEM_HIP(hipMemcpyAsync(s->pec.p, s->bdof.p, s->ndofs, hipMemcpyDeviceToDevice, c->stream));
+    }
+  } else {
+    DBuf<uint8_t> flag(s->nF);
+    h2d(c, flag.p, facet_flag_h, s->nF);
+    if (!union_mode)
+      SUCEM_LAUNCH(k_set_u8, grid_for(s->ndofs, TB), TB, 0, c->stream, s->ndofs, s->pec.p,
+                   (uint8_t)0);
+    const int64_t nc = s->mesh->nc;
+    SUCEM_LAUNCH(k_pec_facets, grid_for(nc * 4, TB), TB, 0, c->stream, nc, s->nloc, s->nfd,
+                 s->cell_faces.p, flag.p, s->cell_dofs.p, s->ref_face_dofs.p, s->pec.p);
+    c->sync();   // flag is released on return
+  }
+  s->npec = count_flags(c, s->pec.p, s->ndofs);
+  s->pec_version += 1;
 }
 
 }  // namespace sucem

@@ -417,7 +417,7 @@ void variational_flux(Context* c, const Space* s, const void* x, double k0, doub
   const unsigned G = grid_for(total, TB, 1024);
   DBuf<double> part(2 * (size_t)G);
   SUCEM_LAUNCH(k_flux_variational, G, TB, 0, c->stream, s->mesh->nc, sk.cell_on_b.p,
-               s->cell_dofs.p, s->pec.p, s->nloc, (const double2*)x, sk.W.p, part.p);
+               s->cell_dofs.p, s->bdof.p, s->nloc, (const double2*)x, sk.W.p, part.p);
   std::vector<double> h(2 * (size_t)G);
   d2h(c, h.data(), part.p, h.size());
   c->sync();
@@ -449,7 +449,7 @@ void ntff(Context* c, const Space* s, const void* x, double k0, bool variational
     make_skin(c, s, x, k0, sk);
     DBuf<double> pv((size_t)n * P * 4);
     SUCEM_LAUNCH(k_ntff_variational, dim3(P, (unsigned)n), TB, 0, c->stream, s->mesh->nc,
-                 sk.cell_on_b.p, s->mesh->xyz.p, s->mesh->cells.p, s->cell_dofs.p, s->pec.p,
+                 sk.cell_on_b.p, s->mesh->xyz.p, s->mesh->cells.p, s->cell_dofs.p, s->bdof.p,
                  s->nloc, s->order, sk.W.p, ddirs.p, k0, pv.p);
     hv.resize((size_t)n * P * 4);
     d2h(c, hv.data(), pv.p, hv.size());

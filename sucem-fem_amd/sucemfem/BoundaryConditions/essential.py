@@ -8,6 +8,8 @@ zero-row / unit-diagonal form.
 """
 from __future__ import division
 
+import numpy as np
+
 from .base_boundary_condition import BoundaryCondition
 
 
@@ -28,12 +30,32 @@ class EssentialBoundaryCondition(BoundaryCondition):
             raise NotImplementedError("only homogeneous (PEC) essential conditions are supported")
 
         def apply(A, b=None):
+            self._mark_space()
             if hasattr(A, "constrain_pec"):
                 A.constrain_pec()
             if b is not None:
                 mask = self.function_space.pec_mask()
                 b[mask] = 0
         return apply
+
+
+    def _mark_space(self):
+        """Set the space's PEC DOFs from this condition's facet marks (once per space);
+        several essential conditions on one space add up."""
+        V = self.function_space
+        if getattr(V, "_pec_marked_by", None) is None:
+            V._pec_marked_by = []
+        if self in V._pec_marked_by:
+            return
+        union = len(V._pec_marked_by) > 0
+        mf = self.mesh_function
+        if mf is None:
+            V.mark_pec(None, union=union)
+        else:
+            if getattr(mf, "dim", 2) != 2:
+                raise ValueError("essential conditions need a facet (dim 2) mesh function")
+            V.mark_pec(np.asarray(mf.array()) == self.region_number, union=union)
+        V._pec_marked_by.append(self)
 
 
 class PECWallsBoundaryCondition(EssentialBoundaryCondition):

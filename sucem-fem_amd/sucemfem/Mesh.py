@@ -8,6 +8,7 @@ Stands in for the DOLFIN objects the reference passes around: ``dolfin.Mesh`` /
 pattern and PEC DOFs are computed by libsucem_hip on the device.
 """
 import ctypes
+import xml.etree.ElementTree as ET
 
 import numpy as np
 
@@ -15,10 +16,65 @@ from . import _hip
 from .runtime import Handle, get_context
 
 
+def _tag(el):
+    return el.tag.rsplit('}', 1)[-1]
+
+
+def _read_dolfin_xml_mesh(filename):
+    coords = cells = None
+    for ev, el in ET.iterparse(filename, events=("start", "end")):
+        t = _tag(el)
+        if ev == "start":
+            if t == "mesh" and el.get("celltype", "tetrahedron") != "tetrahedron":
+                raise NotImplementedError("only tetrahedral meshes are supported")
+            if t == "vertices":
+                coords = np.zeros((int(el.get("size")), 3))
+            elif t == "cells":
+                cells = np.zeros((int(el.get("size")), 4), np.int64)
+            continue
+        if t == "vertex":
+            coords[int(el.get("index"))] = (float(el.get("x")), float(el.get("y")),
+                                            float(el.get("z", 0.0)))
+            el.clear()
+        elif t == "tetrahedron":
+            cells[int(el.get("index"))] = [int(el.get("v%d" % k)) for k in range(4)]
+            el.clear()
+    if coords is None or cells is None:
+        raise ValueError("%s holds no DOLFIN tetrahedral mesh" % filename)
+    return coords, cells
+
+
+def _read_dolfin_xml_meshfunction(filename):
+    """(value_type, dim, values) of a DOLFIN XML MeshFunction."""
+    vals = None
+    vtype, dim = "uint", None
+    for ev, el in ET.iterparse(filename, events=("start", "end")):
+        t = _tag(el)
+        if ev == "start":
+            if t == "meshfunction":
+                vtype = el.get("type", "uint")
+                dim = int(el.get("dim"))
+                vals = np.zeros(int(el.get("size")), np.float64 if vtype == "double" else np.int64)
+            continue
+        if t == "entity":
+            v = el.get("value")
+            vals[int(el.get("index"))] = float(v) if vtype == "double" else int(v)
+            el.clear()
+    if vals is None:
+        raise ValueError("%s holds no DOLFIN MeshFunction" % filename)
+    return vtype, dim, vals
+
+
 class Mesh(object):
-    """Tetrahedral mesh; cell vertex lists are kept sorted (DOLFIN ordering)."""
+    """Tetrahedral mesh; cell vertex lists are kept sorted (DOLFIN ordering).
+
+    ``Mesh(filename)`` reads a DOLFIN XML mesh (as ``dolfin.Mesh(filename)``,
+    e.g. ``examples/albani_bernardi74/driver.py:55``)."""
 
     def __init__(self, coordinates=None, cells=None):
+        if isinstance(coordinates, str):
+            coordinates, cells = _read_dolfin_xml_mesh(coordinates)
+        self._nfacets = None
         self._coords = None if coordinates is None else np.array(coordinates, np.float64).reshape(-1, 3)
         self._cells = None if cells is None else np.sort(np.array(cells, np.int64).reshape(-1, 4), axis=1)
         self._device = None          # (ctx, Handle) of a device copy
@@ -43,6 +99,12 @@ class Mesh(object):
 
     def topology_dim(self):
         return 3
+
+    def num_facets(self):
+        """Number of triangular faces (DOLFIN facet numbering: first encounter)."""
+        if self._nfacets is None:
+            self._nfacets = int(FunctionSpace(self, "Nedelec 1st kind H(curl)", 1).info()[3])
+        return self._nfacets
 
     # -- device copy ---------------------------------------------------------------
     def _info(self):
@@ -98,13 +160,28 @@ def UnitCube(nx, ny, nz):
 
 
 class MeshFunction(object):
-    """Minimal ``dolfin.MeshFunction``: one value per entity of dimension ``dim``."""
+    """``dolfin.MeshFunction``: one value per cell (dim 3) or facet (dim 2, in the
+    first-encounter facet numbering).  ``MeshFunction(type, mesh, filename)`` reads a
+    DOLFIN XML mesh function (``examples/albani_bernardi74/driver.py:57``)."""
 
     def __init__(self, value_type, mesh, dim):
         self.mesh = mesh
+        if isinstance(dim, str):
+            _, self.dim, values = _read_dolfin_xml_meshfunction(dim)
+            want = mesh.num_cells() if self.dim == 3 else mesh.num_facets()
+            if self.dim not in (2, 3) or len(values) != want:
+                raise ValueError("mesh function of dim %d with %d values does not fit the mesh"
+                                 % (self.dim, len(values)))
+            self._values = values
+            return
         self.dim = dim
-        n = mesh.num_cells() if dim == 3 else 0
-        self._values = np.zeros(n, np.int64)
+        if dim == 3:
+            n = mesh.num_cells()
+        elif dim == 2:
+            n = mesh.num_facets()
+        else:
+            raise NotImplementedError("mesh functions on cells (3) and facets (2) only")
+        self._values = np.zeros(n, np.float64 if value_type == "double" else np.int64)
 
     def set_all(self, value):
         self._values[:] = value
@@ -202,3 +279,31 @@ class FunctionSpace(object):
             _hip.call("sucem_space_export_pec", self.handle.h, _hip.ptr(m))
             self._pec = m.astype(bool)
         return self._pec
+
+    def mark_pec(self, facet_flags=None, union=False):
+        """PEC DOFs = closure of the flagged facets (None: every boundary facet);
+        ``union`` adds to the current set instead of replacing it."""
+        flags = None
+        if facet_flags is not None:
+            flags = np.ascontiguousarray(np.asarray(facet_flags) != 0, np.uint8)
+            if len(flags) != int(self.info()[3]):
+                raise ValueError("expected %d facet flags, got %d" % (int(self.info()[3]), len(flags)))
+        _hip.call("sucem_space_mark_pec", self.ctx.handle, self.handle.h, _hip.ptr(flags),
+                  int(bool(union)))
+        self._pec = None
+        self._info = None
+
+
+def boundary_facets(mesh):
+    """Indices of the facets on the mesh boundary (facets of exactly one cell), in the
+    first-encounter facet numbering; marks what ``dolfin.DomainBoundary`` marks."""
+    V = FunctionSpace(mesh, "Nedelec 1st kind H(curl)", 1)
+    _, faces = V.entities()
+    cells = np.sort(mesh.cells(), axis=1)
+    loc = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
+    tri = np.concatenate([cells[:, list(l)] for l in loc])
+    nv = np.int64(mesh.num_vertices())
+    key = lambda t: (t[:, 0] * nv + t[:, 1]) * nv + t[:, 2]          # noqa: E731
+    uk, cnt = np.unique(key(tri), return_counts=True)
+    fk = key(faces)
+    return np.flatnonzero(cnt[np.searchsorted(uk, fk)] == 1)

@@ -33,6 +33,12 @@ Fixtures written (data only: inputs and expected outputs):
   solution in ``driven_order2_dipole_x.npz`` at six angles
   (``PostProcessing/tests/test_ntff.py:58-103``).
 * ``ntff_interpolant.npz`` -- ``interpolant_test_data.pickle`` (``test_ntff.py:36-55``).
+* ``mesh_rectangular_prism.npz`` -- ``functional_tests/data/meshes/rectangular_prism.xml`` and
+  its facet regions (PEC walls = 1; ``functional_tests/test_EigenProblems.py:69-84``).
+* ``mesh_albani_bernardi74.npz`` -- ``examples/albani_bernardi74/mesh/*fig2VII*.xml`` with
+  the cell regions (1000: eps_r 16, 1001: eps_r 1; ``examples/albani_bernardi74/driver.py:53-61``).
+* ``mesh_sphere_r1m_4.npz`` -- ``sandbox/solvers/meshes/sphere-r1m-4.femmesh`` (nodes, tets,
+  property numbers), the input format of ``Utilities/MeshIO.py:39-117``.
 * ``unitcube1_topology.npz`` -- the boundary-edge count of UnitCube(1,1,1)
   asserted by ``sucemfem/tests/test_Geometry.py:31-35``.
 """
@@ -269,6 +275,43 @@ def read_dolfin_mesh_vertices(path):
                     dtype=np.float64)
 
 
+def read_dolfin_meshfunction(path):
+    """Values (by entity index) of a DOLFIN XML MeshFunction."""
+    txt = open(path).read()
+    ent = re.findall(r'<entity index="(\d+)" value="(-?\d+)"', txt)
+    vals = np.zeros(int(re.search(r'size="(\d+)"', txt).group(1)), np.int64)
+    for i, v in ent:
+        vals[int(i)] = int(v)
+    return vals
+
+
+def read_femmesh(path):
+    """Nodes, tets (file order, 0-based node ids) and property numbers of a .femmesh file."""
+    lines = open(path).read().splitlines()
+    nodes = tets = props = None
+    i = 0
+    while i < len(lines):
+        if lines[i].startswith("BLOCK nodes"):
+            n = int(lines[i + 1].split()[0])
+            nodes = np.zeros((n, 3))
+            for l in lines[i + 2:i + 2 + n]:
+                sp = l.split()
+                nodes[int(sp[0]) - 1] = [float(v) for v in sp[1:4]]
+            i += 2 + n
+        elif lines[i].startswith("BLOCK tets"):
+            n = int(lines[i + 1].split()[0])
+            tets = np.zeros((n, 4), np.int64)
+            props = np.zeros(n, np.int64)
+            for l in lines[i + 2:i + 2 + n]:
+                sp = l.split()
+                tets[int(sp[0]) - 1] = [int(v) - 1 for v in sp[2:6]]
+                props[int(sp[0]) - 1] = int(sp[1])
+            i += 2 + n
+        else:
+            i += 1
+    return nodes, tets, props
+
+
 def main(ref):
     out = HERE
     # 1. order-1 driven LHS/RHS on InscribedTet
@@ -329,7 +372,23 @@ def main(ref):
                         ahats=np.asarray(d["ahats"], np.float64),
                         coords=np.asarray(d["coords"], np.float64),
                         vals=np.asarray(d["vals"], np.complex128))
-    # 8. UnitCube(1,1,1) boundary-edge count
+    # 8. meshes read from files by the reference's drivers and functional tests
+    base = "functional_tests/data/meshes/rectangular_prism"
+    np.savez_compressed(os.path.join(out, "mesh_rectangular_prism.npz"),
+                        coords=read_dolfin_mesh_vertices(os.path.join(ref, base + ".xml")),
+                        cells=read_dolfin_mesh_cells(os.path.join(ref, base + ".xml"))[1],
+                        facet_region=read_dolfin_meshfunction(
+                            os.path.join(ref, base + "_facet_region.xml")))
+    base = "examples/albani_bernardi74/mesh/albani_bernardi74_fig2VII"
+    np.savez_compressed(os.path.join(out, "mesh_albani_bernardi74.npz"),
+                        coords=read_dolfin_mesh_vertices(os.path.join(ref, base + ".xml")),
+                        cells=read_dolfin_mesh_cells(os.path.join(ref, base + ".xml"))[1],
+                        physical_region=read_dolfin_meshfunction(
+                            os.path.join(ref, base + "_physical_region.xml")))
+    nodes, tets, props = read_femmesh(os.path.join(ref, "sandbox/solvers/meshes/sphere-r1m-4.femmesh"))
+    np.savez_compressed(os.path.join(out, "mesh_sphere_r1m_4.npz"), nodes=nodes, tets=tets,
+                        property_nos=props)
+    # 9. UnitCube(1,1,1) boundary-edge count
     np.savez_compressed(os.path.join(out, "unitcube1_topology.npz"), n_boundary_edges=18)
     for f in sorted(os.listdir(out)):
         if f.endswith(".npz"):

@@ -314,3 +314,31 @@ def test_boundary_edge_cells_unit_cube():
                       np.any((np.abs(X[a] - X[b]) < 1e-12) & ((np.abs(X[a]) < 1e-12) | (np.abs(X[a] - 1) < 1e-12)))
                       for a, b in O.LOCAL_EDGES)
         assert (c in marked) == touches
+
+
+# --- meshes read from the reference's files ----------------------------------------
+def test_facet_region_numbering_matches_dolfin(golden):
+    """rectangular_prism_facet_region.xml marks the PEC walls (value 1) by DOLFIN facet
+    index; on this unstructured gmsh mesh the first-encounter facet numbering must put
+    exactly the 92 boundary facets under those indices."""
+    d = golden("mesh_rectangular_prism.npz")
+    sp_ = O.Space(d["coords"], d["cells"], 2)
+    fr = d["facet_region"]
+    assert len(fr) == len(sp_.topo["faces"])
+    assert set(np.flatnonzero(fr == 1)) == set(sp_.boundary_faces.tolist())
+    assert np.array_equal(sp_.pec_mask(fr == 1), sp_.pec_mask())
+    # an interior facet set gives its own closure: 3 edges x 2 + 2 face DOFs each
+    inner = np.zeros(len(fr), bool)
+    inner[np.flatnonzero(fr == 0)[0]] = True
+    assert sp_.pec_mask(inner).sum() == 8
+
+
+def test_rectangular_prism_cavity_modes(golden):
+    """functional_tests/test_EigenProblems.py:65-134 on its own mesh at order 2 (the
+    reference uses order 4): the lowest modes of the 1 x 0.5 x 0.25 cavity within the
+    order-2 discretisation error of the analytic values."""
+    d = golden("mesh_rectangular_prism.npz")
+    sp_ = O.Space(d["coords"], d["cells"], 2)
+    k2, _, _ = O.cavity_eigs(sp_, 4, 1.1, facet_flags=d["facet_region"] == 1)
+    ana = O.analytic_cavity_k2([1.0, 0.5, 0.25], 4)
+    assert np.all(np.abs(k2 - ana) / ana < 0.05), (k2, ana)
