@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=200)
+    ap.add_argument("--copy-probe-mb", type=int, default=256,
+                    help="one plain device copy of this size after the SpMV timing: the "
+                         "known-size kernel that scales HBM byte counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d"
@@ -126,6 +129,12 @@ def main():
     torch.cuda.synchronize()
     spmv_ms = e0.elapsed_time(e1) / a.spmv_reps
     spmv_bytes = prob.spmv_bytes(a.precision)
+    if a.copy_probe_mb > 0:
+        src = torch.ones(a.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     log("spmv %.4f ms, %.1f GB/s" % (spmv_ms, achieved))
 

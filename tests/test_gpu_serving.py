@@ -1,0 +1,34 @@
+"""The request loop on the device: requests run concurrently on per-slot streams give the
+same results as the same requests run alone (every kernel reduces in a fixed order)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_requests_in_the_mix_match_alone():
+    import torch
+    torch.cuda.set_device(0)
+    from sucemfem.Serving import Request, Server, device_executor, summarize
+    print("serving: mix vs alone", flush=True)
+    dims = [(0.3, 0.2, 0.25), (0.5, 0.4, 0.3), (0.25, 0.25, 0.4)]
+    trace = []
+    for i in range(8):
+        d = dims[i % 3]
+        kind = "eigen" if i % 2 == 0 else "driven"
+        order = 1 + (i // 2) % 2
+        n = 3 + i % 3
+        if kind == "eigen":
+            lo = (np.pi / max(d)) ** 2 + (np.pi / sorted(d)[1]) ** 2
+            trace.append(Request(i, kind, d, n, order, 0.0, nev=3, sigma=0.5 * lo))
+        else:
+            trace.append(Request(i, kind, d, n, order, 0.0, freq=3e8 / (10 * max(d) / n)))
+    ex = device_executor(tol=1e-8, max_iter=20000)
+    results, makespan = Server(ex, slots=3).run(trace, realtime=False)
+    s = summarize(results, makespan)
+    assert s["done"] == len(trace), [r.error for r in results if r.status != "done"]
+    alone = Server(ex, slots=1)
+    for r, req in zip(results, trace):
+        (ra,), _ = alone.run([req], realtime=False)
+        assert ra.status == "done", ra.error
+        assert np.array_equal(np.asarray(r.value), np.asarray(ra.value)), req
