@@ -161,12 +161,18 @@ int sucem_system_apply(sucem_ctx* ctx, sucem_system* sys, double k0, int precisi
 int sucem_system_apply_coef(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
                             int precision, int eliminate_pec, int64_t nvec, const void* x_d,
                             void* y_d);
-/* Autotune the fused SpMV launch (lanes per row, loads per lane, blocks) for precision 32 or
- * 64 by timing every candidate with HIP events; the fastest is kept on the system and used by
- * apply and solve.  cfg3_h (nullable) receives {lanes, loads, blocks}, ms_h the time per SpMV.
+/* Autotune the fused operator for precision 32 or 64 by timing every candidate with HIP
+ * events: the assembled CSR operator over (lanes per row, loads per lane, blocks) and the
+ * matrix-free operator (element products + row gather) over its gather grid.  The fastest
+ * is kept on the system and used by apply and solve.  cfg4_h (nullable) receives
+ * {matrix_free, lanes, loads, blocks}, ms_h the time per application.
  * (No reference counterpart: launch tuning of the device operator.) */
-int sucem_system_tune(sucem_ctx* ctx, sucem_system* sys, int precision, int reps, int* cfg3_h,
+int sucem_system_tune(sucem_ctx* ctx, sucem_system* sys, int precision, int reps, int* cfg4_h,
                       double* ms_h);
+/* Set the operator explicitly: cfg4_h = {matrix_free (0/1), lanes (4, 8, 16, 32), loads
+ * (1, 2, 4), blocks (multiple of 8)}; lanes and loads are ignored when matrix_free is 1.
+ * Calls on one system must not overlap (the matrix-free element buffer is per system). */
+int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, const int* cfg4_h);
 /* Materialised A values (complex128 interleaved) on the full pattern. */
 int sucem_system_export_lhs(sucem_system* sys, double k0, double* values_h);
 

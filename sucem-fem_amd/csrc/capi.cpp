@@ -387,7 +387,7 @@ int sucem_system_apply(sucem_ctx* ctx, sucem_system* sys, double k0, int precisi
   API_END
 }
 
-int sucem_system_tune(sucem_ctx* ctx, sucem_system* sys, int precision, int reps, int* cfg3_h,
+int sucem_system_tune(sucem_ctx* ctx, sucem_system* sys, int precision, int reps, int* cfg4_h,
                       double* ms_h) {
   API_BEGIN
   NOT_NULL(ctx); NOT_NULL(sys);
@@ -398,8 +398,33 @@ int sucem_system_tune(sucem_ctx* ctx, sucem_system* sys, int precision, int reps
   if (precision == 32) f = tune_sys<float>(c, &sys->s, reps);
   else if (precision == 64) f = tune_sys<double>(c, &sys->s, reps);
   else throw Error(SUCEM_ERR_INVALID, "precision must be 32 or 64");
-  if (cfg3_h) { cfg3_h[0] = f.L; cfg3_h[1] = f.U; cfg3_h[2] = f.G; }
+  if (cfg4_h) { cfg4_h[0] = f.mf; cfg4_h[1] = f.L; cfg4_h[2] = f.U; cfg4_h[3] = f.G; }
   if (ms_h) *ms_h = f.ms;
+  API_END
+}
+
+int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, const int* cfg4_h) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys); NOT_NULL(cfg4_h);
+  SUCEM_CHECK(precision == 32 || precision == 64, SUCEM_ERR_INVALID, "precision must be 32 or 64");
+  SpmvCfg f;
+  f.mf = cfg4_h[0]; f.L = cfg4_h[1]; f.U = cfg4_h[2]; f.G = cfg4_h[3];
+  SUCEM_CHECK(f.G > 0 && f.G % 8 == 0 && f.G <= (1 << 20), SUCEM_ERR_INVALID,
+              "blocks must be a positive multiple of 8");
+  Context* c = &ctx->c;
+  c->use();
+  if (f.mf == 1) {
+    SUCEM_CHECK(sys->s.space->nloc == 6 || sys->s.space->nloc == 20, SUCEM_ERR_UNSUPPORTED,
+                "matrix-free operator: orders 1 and 2 only");
+    f.L = 0; f.U = 0;
+    mf_prepare(c, &sys->s);
+  } else {
+    SUCEM_CHECK(f.mf == 0, SUCEM_ERR_INVALID, "matrix_free must be 0 or 1");
+    SUCEM_CHECK(f.L == 4 || f.L == 8 || f.L == 16 || f.L == 32, SUCEM_ERR_INVALID,
+                "lanes must be 4, 8, 16 or 32");
+    SUCEM_CHECK(f.U == 1 || f.U == 2 || f.U == 4, SUCEM_ERR_INVALID, "loads must be 1, 2 or 4");
+  }
+  sys->s.cfg[precision == 32 ? 0 : 1] = f;
   API_END
 }
 

@@ -53,11 +53,17 @@ struct Space {
   DBuf<int32_t> g_ptr, g_col, gt_ptr, gt_col;
   DBuf<double> g_val, gt_val;
   DBuf<float> g_val32, gt_val32;
+  // matrix-free operator (built on first use): DOF ids per cell as SoA, DOF -> element
+  // slot (cell * nloc + local) map sorted by DOF, packed reference tensors
+  DBuf<int32_t> mf_cdT, mf_dptr, mf_slot;
+  DBuf<double> mf_ref64;
+  DBuf<float> mf_ref32;
 };
 
 // SpMV launch configuration: lanes per row, loads per lane, blocks (0 = heuristic).
 struct SpmvCfg {
   int L = 0, U = 0, G = 0;
+  int mf = 0;      // 1: matrix-free operator (G = grid of its gather kernel)
   double ms = 0;
 };
 
@@ -74,6 +80,11 @@ struct System {
   DBuf<float2> st32;
   DBuf<double> s0_64;
   DBuf<float> s0_32;
+  // matrix-free operator: per-cell coefficients (SoA, 12 per cell) and the element buffer
+  bool mf_ready = false;
+  DBuf<double> mf_coef64;
+  DBuf<float> mf_coef32;
+  DBuf<double2> mf_yl;
 };
 
 // Generic complex CSR matrix (user-supplied, e.g. from scipy).

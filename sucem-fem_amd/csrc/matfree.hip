@@ -1,0 +1,284 @@
+// Matrix-free fused operator y = (cs S + ct T + i c0 S0) x.
+//
+// The assembled operator streams one (S, T) pair and one column index per nonzero:
+// 12 B (FP32) or 20 B (FP64) per nonzero, ~43 nonzeros per order-2 row.  Here the
+// volume part is recomputed from 12 geometric coefficients per cell instead:
+//
+//  1. k_mf_cells: one thread per cell forms the dense element matrix
+//     K = sum_k (ct cT_k) M_k + (cs cS_k) C_k (symmetric, nloc x nloc) with the
+//     reference tensors M_k, C_k read as wave-uniform (scalar) operands, and writes
+//     the element product K x_c (nloc complex values) to a per-cell buffer;
+//  2. k_mf_gather: one thread per DOF row sums its element contributions in cell
+//     order (a DOF -> (cell, local) map sorted once per space), adds the compact ABC
+//     row i c0 (S0 x)_r, applies the PEC row mask and the optional Krylov epilogue.
+//
+// HBM traffic per application: 12 coefficients + nloc DOF ids per cell, x and the
+// element buffer, the DOF map, y -- about a fifth of the assembled operator's
+// bytes for order 2 -- for ~3.3k FMAs per order-2 cell.  The order of every sum is
+// fixed, so results are deterministic.  The element buffer belongs to the System:
+// concurrent applications of one System on different streams are not supported.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include "geom.hpp"
+#include "internal.hpp"
+#include "kernels.hpp"
+#include "ops.hpp"
+#include "refdata.hpp"
+
+namespace sucem {
+namespace {
+
+constexpr int TB = 256;
+
+__constant__ int c_msym[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+
+// Per cell, SoA: coef[k * nc + c], k < 6: eps_r |det J| (J^-1 J^-T)_k (pairs with M_k),
+// k >= 6: (1/mu_r) / |det J| (J^T J)_{k-6} (pairs with C_{k-6}).
+template <typename R>
+__global__ void k_mf_coefs(int64_t nc, const double* xyz, const int32_t* cells, const double* eps,
+                           const double* imu, R* coef) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    double J[3][3], Ji[3][3], det;
+    cell_jacobian(xyz, cells + 4 * c, J, det, Ji);
+    const double adet = fabs(det);
+    const double e = eps[c] * adet, m = imu[c] / adet;
+    for (int k = 0; k < 6; ++k) {
+      const int a = c_msym[k][0], b = c_msym[k][1];
+      const double g = Ji[a][0] * Ji[b][0] + Ji[a][1] * Ji[b][1] + Ji[a][2] * Ji[b][2];
+      const double q = J[0][a] * J[0][b] + J[1][a] * J[1][b] + J[2][a] * J[2][b];
+      coef[k * nc + c] = (R)(e * g);
+      coef[(6 + k) * nc + c] = (R)(m * q);
+    }
+  }
+}
+
+__global__ void k_mf_transpose_dofs(int64_t nc, int nloc, const int32_t* cd, int32_t* cdT) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nc * nloc;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = i / nloc;
+    const int l = (int)(i % nloc);
+    cdT[(int64_t)l * nc + c] = cd[i];
+  }
+}
+
+__global__ void k_mf_iota(int64_t n, uint32_t* v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = (uint32_t)i;
+}
+
+// dptr[d] = first position of DOF d in the sorted key list (every DOF occurs).
+__global__ void k_mf_heads(int64_t n, int64_t ndofs, const uint32_t* keys, int32_t* dptr) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i == 0 || keys[i] != keys[i - 1]) dptr[keys[i]] = (int32_t)i;
+    if (i == n - 1) dptr[ndofs] = (int32_t)n;
+  }
+}
+
+template <typename R, int NLOC>
+struct RefPacked {
+  static constexpr int NP = NLOC * (NLOC + 1) / 2;
+};
+
+// Element products, one thread per cell, cells split into one contiguous share per XCD.
+template <typename R, int NLOC>
+__global__ __launch_bounds__(TB) void k_mf_cells(int64_t nc, int G, const R* __restrict__ coef,
+                                                 const int32_t* __restrict__ cdT,
+                                                 const R* __restrict__ ref, R cs, R ct,
+                                                 const typename C2<R>::T* __restrict__ x,
+                                                 typename C2<R>::T* __restrict__ yl) {
+  using C = typename C2<R>::T;
+  constexpr int NP = RefPacked<R, NLOC>::NP;
+  // reference tensors through the constant address space: uniform scalar loads
+  const __attribute__((address_space(4))) R* cref =
+      (const __attribute__((address_space(4))) R*)ref;
+  const int lb = xcd_block(G);
+  int64_t chunk = (nc + G - 1) / G;
+  chunk = (chunk + TB - 1) / TB * TB;
+  const int64_t start = (int64_t)lb * chunk;
+  const int64_t end = min(nc, start + chunk);
+  for (int64_t c = start + threadIdx.x; c < end; c += TB) {
+    R a[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) a[k] = coef[k * nc + c] * (k < 6 ? ct : cs);
+    C xl[NLOC], acc[NLOC];
+#pragma unroll
+    for (int i = 0; i < NLOC; ++i) {
+      xl[i] = x[cdT[(int64_t)i * nc + c]];
+      acc[i] = mk(R(0), R(0));
+    }
+    int p = 0;
+#pragma unroll
+    for (int i = 0; i < NLOC; ++i) {
+#pragma unroll
+      for (int j = i; j < NLOC; ++j, ++p) {
+        R kij = a[0] * cref[p];
+#pragma unroll
+        for (int k = 1; k < 12; ++k) kij += a[k] * cref[k * NP + p];
+        acc[i].x += kij * xl[j].x;
+        acc[i].y += kij * xl[j].y;
+        if (j != i) {
+          acc[j].x += kij * xl[i].x;
+          acc[j].y += kij * xl[i].y;
+        }
+      }
+    }
+    C* out = yl + c * NLOC;
+#pragma unroll
+    for (int i = 0; i < NLOC; ++i) out[i] = acc[i];
+  }
+}
+
+template <typename R, int EPI>
+__global__ __launch_bounds__(TB) void k_mf_gather(
+    int n, int G, const int32_t* __restrict__ dptr, const int32_t* __restrict__ slot,
+    const typename C2<R>::T* __restrict__ yl, const int32_t* __restrict__ s0p,
+    const int32_t* __restrict__ s0c, const R* __restrict__ s0v, R c0,
+    const uint8_t* __restrict__ mask, const typename C2<R>::T* __restrict__ x,
+    typename C2<R>::T* __restrict__ y, const typename C2<R>::T* __restrict__ w, RedSite rs) {
+  using C = typename C2<R>::T;
+  const int lb = xcd_block(G);
+  int chunk = (n + G - 1) / G;
+  chunk = (chunk + TB - 1) / TB * TB;
+  const int start = lb * chunk;
+  const int end = min(n, start + chunk);
+  double2 d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0);
+  for (int r = start + threadIdx.x; r < end; r += TB) {
+    C acc = mk(R(0), R(0));
+    const int j1 = dptr[r + 1];
+    for (int j = dptr[r]; j < j1; ++j) {
+      const C v = yl[slot[j]];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    if (s0p) {
+      C a0 = mk(R(0), R(0));
+      const int k1 = s0p[r + 1];
+      for (int k = s0p[r]; k < k1; ++k) {
+        const R v = s0v[k];
+        const C xc = x[s0c[k]];
+        a0.x += v * xc.x;
+        a0.y += v * xc.y;
+      }
+      acc.x -= c0 * a0.y;
+      acc.y += c0 * a0.x;
+    }
+    if (mask && mask[r]) acc = mk(R(0), R(0));
+    y[r] = acc;
+    if (EPI) {
+      const double2 yd = to_d(acc);
+      double2 wd = to_d(w[r]);
+      if (EPI == 2) wd.y = -wd.y;
+      d1.x += wd.x * yd.x - wd.y * yd.y;
+      d1.y += wd.x * yd.y + wd.y * yd.x;
+      d2.x += yd.x * yd.x + yd.y * yd.y;
+    }
+  }
+  if (EPI) reduce_finalize<TB>(rs, G, d1, d2);
+}
+
+// Reference tensors packed as [k][upper-triangle pair p], k < 6: M_k, k >= 6: C_{k-6}.
+template <typename R>
+void pack_ref(const RefData& R_, std::vector<R>& out) {
+  const int n = R_.nloc, np = n * (n + 1) / 2;
+  out.assign((size_t)12 * np, R(0));
+  for (int k = 0; k < 12; ++k) {
+    const std::vector<double>& src = k < 6 ? R_.mass : R_.stiff;
+    const int kk = k % 6;
+    int p = 0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i; j < n; ++j, ++p) out[(size_t)k * np + p] = (R)src[((size_t)kk * n + i) * n + j];
+  }
+}
+
+}  // namespace
+
+void mf_prepare(Context* c, System* sys) {
+  if (sys->mf_ready) return;
+  Space* s = sys->space;
+  const Mesh* m = s->mesh;
+  const int64_t nc = m->nc;
+  const int nloc = s->nloc;
+  if (!s->mf_cdT.p) {
+    // DOF ids per cell (SoA) and the DOF -> element-slot map, sorted by (DOF, slot)
+    s->mf_cdT.alloc((size_t)nc * nloc);
+    SUCEM_LAUNCH(k_mf_transpose_dofs, grid_for(nc * nloc, TB), TB, 0, c->stream, nc, nloc,
+                 s->cell_dofs.p, s->mf_cdT.p);
+    const int64_t n = nc * nloc;
+    DBuf<uint32_t> keys_out(n);
+    DBuf<uint32_t> vin(n);
+    s->mf_slot.alloc(n);
+    SUCEM_LAUNCH(k_mf_iota, grid_for(n, TB), TB, 0, c->stream, n, vin.p);
+    const unsigned bits = (unsigned)key_bits(s->ndofs);
+    size_t bytes = 0;
+    SUCEM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)s->cell_dofs.p, keys_out.p, vin.p,
+                                        (uint32_t*)s->mf_slot.p, (size_t)n, 0u, bits, c->stream));
+    DBuf<char> tmp(bytes);
+    SUCEM_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, (const uint32_t*)s->cell_dofs.p, keys_out.p, vin.p,
+                                        (uint32_t*)s->mf_slot.p, (size_t)n, 0u, bits, c->stream));
+    s->mf_dptr.alloc(s->ndofs + 1);
+    SUCEM_LAUNCH(k_mf_heads, grid_for(n, TB), TB, 0, c->stream, n, s->ndofs, keys_out.p,
+                 s->mf_dptr.p);
+    const RefData& R_ = ref_data(s->order);
+    std::vector<double> r64;
+    std::vector<float> r32;
+    pack_ref<double>(R_, r64);
+    pack_ref<float>(R_, r32);
+    s->mf_ref64.alloc(r64.size());
+    s->mf_ref32.alloc(r32.size());
+    h2d(c, s->mf_ref64.p, r64.data(), r64.size());
+    h2d(c, s->mf_ref32.p, r32.data(), r32.size());
+    c->sync();   // temporaries and host vectors are released on return
+  }
+  sys->mf_coef64.alloc((size_t)12 * nc);
+  sys->mf_coef32.alloc((size_t)12 * nc);
+  SUCEM_LAUNCH((k_mf_coefs<double>), grid_for(nc, TB), TB, 0, c->stream, nc, m->xyz.p, m->cells.p,
+               m->eps_r.p, m->inv_mu_r.p, sys->mf_coef64.p);
+  SUCEM_LAUNCH((k_mf_coefs<float>), grid_for(nc, TB), TB, 0, c->stream, nc, m->xyz.p, m->cells.p,
+               m->eps_r.p, m->inv_mu_r.p, sys->mf_coef32.p);
+  sys->mf_yl.alloc((size_t)nc * nloc * sizeof(double2));
+  c->sync();
+  sys->mf_ready = true;
+}
+
+template <typename R>
+void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x, void* y,
+              const void* w, int epi, const RedSite& rs, const SpmvCfg& f) {
+  using C = typename C2<R>::T;
+  const Space* s = sys->space;
+  const int64_t nc = s->mesh->nc;
+  const R* coef;
+  const R* ref;
+  const R* s0v;
+  if constexpr (sizeof(R) == 4) {
+    coef = sys->mf_coef32.p; ref = s->mf_ref32.p; s0v = sys->s0_32.p;
+  } else {
+    coef = sys->mf_coef64.p; ref = s->mf_ref64.p; s0v = sys->s0_64.p;
+  }
+  C* yl = (C*)sys->mf_yl.p;
+  const int Gc = op_grid(c, (nc + 3) / 4, 8);   // cell blocks, a multiple of 8
+  if (s->nloc == 20)
+    SUCEM_LAUNCH((k_mf_cells<R, 20>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p, ref,
+                 (R)k.s, (R)k.t, (const C*)x, yl);
+  else
+    SUCEM_LAUNCH((k_mf_cells<R, 6>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p, ref,
+                 (R)k.s, (R)k.t, (const C*)x, yl);
+  const bool has_s0 = (sys->which & SUCEM_MAT_S0) && s->nnz0 > 0;
+  const int32_t* s0p = has_s0 ? s->s0_ptr.p : nullptr;
+  const int n = (int)s->ndofs;
+#define MF_ARGS n, f.G, s->mf_dptr.p, s->mf_slot.p, (const C*)yl, s0p, s->s0_col.p, s0v, \
+                (R)k.s0, mask, (const C*)x, (C*)y, (const C*)w, rs
+  if (epi == 0) SUCEM_LAUNCH((k_mf_gather<R, 0>), f.G, TB, 0, c->stream, MF_ARGS);
+  else if (epi == 1) SUCEM_LAUNCH((k_mf_gather<R, 1>), f.G, TB, 0, c->stream, MF_ARGS);
+  else SUCEM_LAUNCH((k_mf_gather<R, 2>), f.G, TB, 0, c->stream, MF_ARGS);
+#undef MF_ARGS
+}
+template void mf_apply<float>(Context*, const System*, Coef, const uint8_t*, const void*, void*,
+                              const void*, int, const RedSite&, const SpmvCfg&);
+template void mf_apply<double>(Context*, const System*, Coef, const uint8_t*, const void*, void*,
+                               const void*, int, const RedSite&, const SpmvCfg&);
+
+}  // namespace sucem

@@ -391,6 +391,11 @@ SpmvCfg default_cfg(Context* c, int64_t n, int64_t nnz) {
 template <typename R>
 void sys_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
                void* y, const void* w, int epi, const RedSite& rs, const SpmvCfg& f) {
+  if (f.mf) {
+    SUCEM_CHECK(sys->mf_ready, SUCEM_ERR_INVALID, "matrix-free operator not prepared");
+    mf_apply<R>(c, sys, k, mask, x, y, w, epi, rs, f);
+    return;
+  }
   switch (f.L) {
     case 4: sys_dispatch_u<R, 4>(c, sys, k, mask, x, y, w, epi, rs, f.G, f.U); break;
     case 8: sys_dispatch_u<R, 8>(c, sys, k, mask, x, y, w, epi, rs, f.G, f.U); break;
@@ -468,6 +473,23 @@ SpmvCfg tune_sys(Context* c, System* sys, int reps) {
         if (f.ms < best.ms) best = f;
       }
     }
+  // matrix-free candidates: element products + row gather
+  if (s->nloc == 6 || s->nloc == 20) {
+    mf_prepare(c, sys);
+    for (int pc : per_cu) {
+      SpmvCfg f;
+      f.mf = 1; f.G = op_grid(c, s->ndofs, pc);
+      sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);  // warm
+      SUCEM_HIP(hipEventRecord(e0, c->stream));
+      for (int i = 0; i < reps; ++i) sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);
+      SUCEM_HIP(hipEventRecord(e1, c->stream));
+      SUCEM_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      SUCEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      f.ms = ms / reps;
+      if (f.ms < best.ms) best = f;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   sys->cfg[sizeof(R) == 4 ? 0 : 1] = best;

@@ -22,6 +22,11 @@ SpmvCfg sys_cfg(Context* c, const System* sys);
 SpmvCfg csr_cfg(Context* c, const Csr* A);
 template <typename R>
 SpmvCfg tune_sys(Context* c, System* sys, int reps);
+// matrix-free operator (matfree.hip): mf_prepare before the first mf_apply on a system
+void mf_prepare(Context* c, System* sys);
+template <typename R>
+void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x, void* y,
+              const void* w, int epi, const RedSite& rs, const SpmvCfg& f);
 
 // gradient-space preconditioner (hiptmair.hip)
 template <typename R>

@@ -118,12 +118,20 @@ class DeviceProblem(object):
         return k2, resid, st.as_dict()
 
     def tune(self, precision, reps=10):
-        """Autotune the fused SpMV launch for this system; returns ({L, U, G}, ms per SpMV)."""
-        cfg = np.zeros(3, np.int32)
+        """Autotune the fused operator for this system (assembled launch shapes and the
+        matrix-free operator); returns ({matrix_free, lanes, loads, blocks}, ms per apply)."""
+        cfg = np.zeros(4, np.int32)
         ms = ctypes.c_double()
         _hip.call("sucem_system_tune", self.ctx.handle, self.system.handle.h, int(precision),
                   int(reps), _hip.ptr(cfg), ctypes.byref(ms))
-        return dict(lanes=int(cfg[0]), loads=int(cfg[1]), blocks=int(cfg[2])), ms.value
+        return dict(matrix_free=int(cfg[0]), lanes=int(cfg[1]), loads=int(cfg[2]),
+                    blocks=int(cfg[3])), ms.value
+
+    def set_spmv(self, precision, matrix_free=0, lanes=16, loads=4, blocks=2048):
+        """Select the operator explicitly (see ``tune``)."""
+        cfg = np.array([matrix_free, lanes, loads, blocks], np.int32)
+        _hip.call("sucem_system_set_spmv", self.ctx.handle, self.system.handle.h, int(precision),
+                  _hip.ptr(cfg))
 
     def apply(self, k0, x, y, precision=32, nvec=1):
         import torch

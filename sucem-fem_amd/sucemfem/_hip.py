@@ -82,6 +82,7 @@ SIGNATURES = {
     "sucem_space_export_entities": (ctypes.c_int, [_P, _P, _P]),
     "sucem_space_export_pec": (ctypes.c_int, [_P, _P]),
     "sucem_space_mark_pec": (ctypes.c_int, [_P, _P, _P, ctypes.c_int]),
+    "sucem_system_set_spmv": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "sucem_space_destroy": (ctypes.c_int, [_P]),
     "sucem_assemble": (ctypes.c_int, [_P, _P, ctypes.c_uint, _PP]),
     "sucem_system_export": (ctypes.c_int, [_P, ctypes.c_uint, _P]),
