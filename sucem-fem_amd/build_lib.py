@@ -22,7 +22,7 @@ SOURCES = ["refdata.cpp", "capi.cpp", "mesh_space.hip", "assemble.hip", "ops.hip
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SUCEM_OFFLOAD_ARCH", "gfx950")
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", INCLUDE, "-I", CSRC, "-Wall",
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", INCLUDE, "-I", CSRC, "-I", OBJ_DIR, "-Wall",
           "-Wno-unused-function", "-Wno-unused-variable"]
 # No packed-FP32 instructions in device code (v_pk_*_f32).
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -40,9 +40,26 @@ def _compile(src):
     return src, obj, r.returncode, r.stdout + r.stderr, cmd
 
 
+def _gen_tables():
+    """Reference-tensor tables for the matrix-free kernel, generated from refdata.cpp."""
+    exe = os.path.join(OBJ_DIR, "gen_mf_tables")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", CSRC, os.path.join(CSRC, "gen_mf_tables.cpp"),
+           os.path.join(CSRC, "refdata.cpp"), "-o", exe, "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("table generator build failed")
+    out = os.path.join(OBJ_DIR, "mf_tables.inc")
+    r = subprocess.run([exe, out], capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("table generation failed")
+
+
 def build(jobs=None, verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
     os.makedirs(OBJ_DIR, exist_ok=True)
+    _gen_tables()
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
     objs = []
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:

@@ -54,10 +54,8 @@ struct Space {
   DBuf<double> g_val, gt_val;
   DBuf<float> g_val32, gt_val32;
   // matrix-free operator (built on first use): DOF ids per cell as SoA, DOF -> element
-  // slot (cell * nloc + local) map sorted by DOF, packed reference tensors
+  // slot (cell * nloc + local) map sorted by DOF
   DBuf<int32_t> mf_cdT, mf_dptr, mf_slot;
-  DBuf<double> mf_ref64;
-  DBuf<float> mf_ref32;
 };
 
 // SpMV launch configuration: lanes per row, loads per lane, blocks (0 = heuristic).

@@ -6,7 +6,8 @@
 //
 //  1. k_mf_cells: one thread per cell forms the dense element matrix
 //     K = sum_k (ct cT_k) M_k + (cs cS_k) C_k (symmetric, nloc x nloc) with the
-//     reference tensors M_k, C_k read as wave-uniform (scalar) operands, and writes
+//     reference tensors M_k, C_k compiled into the instructions (generated tables;
+//     zero entries generate nothing), and writes
 //     the element product K x_c (nloc complex values) to a per-cell buffer;
 //  2. k_mf_gather: one thread per DOF row sums its element contributions in cell
 //     order (a DOF -> (cell, local) map sorted once per space), adds the compact ABC
@@ -18,13 +19,14 @@
 // fixed, so results are deterministic.  The element buffer belongs to the System:
 // concurrent applications of one System on different streams are not supported.
 #include <cstring>
+#include <utility>
 #include <rocprim/rocprim.hpp>
 
 #include "geom.hpp"
 #include "internal.hpp"
 #include "kernels.hpp"
 #include "ops.hpp"
-#include "refdata.hpp"
+#include "mf_tables.inc"
 
 namespace sucem {
 namespace {
@@ -78,29 +80,86 @@ __global__ void k_mf_heads(int64_t n, int64_t ndofs, const uint32_t* keys, int32
   }
 }
 
-template <typename R, int NLOC>
-struct RefPacked {
-  static constexpr int NP = NLOC * (NLOC + 1) / 2;
+// ---- element matrix with the reference tensors folded into the instructions ----------
+// kMfRef<order>[k][p]: k < 6 mass M_k, k >= 6 curl-curl C_{k-6}; p runs over the upper
+// triangle (i <= j) row by row.  Zero entries (~40 % for order 2) generate no code.
+template <int ORDER>
+struct MfTab;
+template <>
+struct MfTab<1> {
+  static constexpr int NLOC = 6, NP = kMfNp1;
+  static constexpr double v(int k, int p) { return kMfRef1[k][p]; }
+};
+template <>
+struct MfTab<2> {
+  static constexpr int NLOC = 20, NP = kMfNp2;
+  static constexpr double v(int k, int p) { return kMfRef2[k][p]; }
 };
 
-// Element products, one thread per cell, cells split into one contiguous share per XCD.
-template <typename R, int NLOC>
+// (i, j) of upper-triangle pair p
+template <int NLOC>
+constexpr int pair_i(int p) {
+  int i = 0;
+  while (p >= NLOC - i) { p -= NLOC - i; ++i; }
+  return i;
+}
+template <int NLOC>
+constexpr int pair_j(int p) {
+  int i = 0;
+  while (p >= NLOC - i) { p -= NLOC - i; ++i; }
+  return i + p;
+}
+
+template <typename R, int ORDER, int P, int K>
+__device__ __forceinline__ void k_term(R& kij, const R* a) {
+  constexpr double v = MfTab<ORDER>::v(K, P);
+  if constexpr (v != 0.0) kij += a[K] * (R)v;
+}
+
+template <int ORDER, int P, int... K>
+constexpr bool pair_nonzero(std::integer_sequence<int, K...>) {
+  return ((MfTab<ORDER>::v(K, P) != 0.0) || ...);
+}
+
+template <typename R, int ORDER, int P, int... K>
+__device__ __forceinline__ void pair_step(const R* a, const typename C2<R>::T* xl,
+                                          typename C2<R>::T* acc,
+                                          std::integer_sequence<int, K...> ks) {
+  if constexpr (pair_nonzero<ORDER, P>(ks)) {
+    constexpr int N = MfTab<ORDER>::NLOC;
+    constexpr int i = pair_i<N>(P), j = pair_j<N>(P);
+    R kij = R(0);
+    (k_term<R, ORDER, P, K>(kij, a), ...);
+    acc[i].x += kij * xl[j].x;
+    acc[i].y += kij * xl[j].y;
+    if constexpr (i != j) {
+      acc[j].x += kij * xl[i].x;
+      acc[j].y += kij * xl[i].y;
+    }
+    // FP64 constants live in scalar register pairs: stop the scheduler from gathering
+    // the constants of many pairs at once (they would spill)
+    if constexpr (sizeof(R) == 8) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <typename R, int ORDER, int... P>
+__device__ __forceinline__ void element_product(const R* a, const typename C2<R>::T* xl,
+                                                typename C2<R>::T* acc,
+                                                std::integer_sequence<int, P...>) {
+  (pair_step<R, ORDER, P>(a, xl, acc, std::make_integer_sequence<int, 12>{}), ...);
+}
+
+// Element products, one thread per cell; consecutive blocks of cells go to one XCD.
+template <typename R, int ORDER>
 __global__ __launch_bounds__(TB) void k_mf_cells(int64_t nc, int G, const R* __restrict__ coef,
-                                                 const int32_t* __restrict__ cdT,
-                                                 const R* __restrict__ ref, R cs, R ct,
+                                                 const int32_t* __restrict__ cdT, R cs, R ct,
                                                  const typename C2<R>::T* __restrict__ x,
                                                  typename C2<R>::T* __restrict__ yl) {
   using C = typename C2<R>::T;
-  constexpr int NP = RefPacked<R, NLOC>::NP;
-  // reference tensors through the constant address space: uniform scalar loads
-  const __attribute__((address_space(4))) R* cref =
-      (const __attribute__((address_space(4))) R*)ref;
-  const int lb = xcd_block(G);
-  int64_t chunk = (nc + G - 1) / G;
-  chunk = (chunk + TB - 1) / TB * TB;
-  const int64_t start = (int64_t)lb * chunk;
-  const int64_t end = min(nc, start + chunk);
-  for (int64_t c = start + threadIdx.x; c < end; c += TB) {
+  constexpr int NLOC = MfTab<ORDER>::NLOC;
+  // one cell per thread and no loop, so that no constant is hoisted into a register
+  const int64_t c = (int64_t)xcd_block(G) * TB + threadIdx.x;
+  if (c < nc) {
     R a[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) a[k] = coef[k * nc + c] * (k < 6 ? ct : cs);
@@ -110,22 +169,7 @@ __global__ __launch_bounds__(TB) void k_mf_cells(int64_t nc, int G, const R* __r
       xl[i] = x[cdT[(int64_t)i * nc + c]];
       acc[i] = mk(R(0), R(0));
     }
-    int p = 0;
-#pragma unroll
-    for (int i = 0; i < NLOC; ++i) {
-#pragma unroll
-      for (int j = i; j < NLOC; ++j, ++p) {
-        R kij = a[0] * cref[p];
-#pragma unroll
-        for (int k = 1; k < 12; ++k) kij += a[k] * cref[k * NP + p];
-        acc[i].x += kij * xl[j].x;
-        acc[i].y += kij * xl[j].y;
-        if (j != i) {
-          acc[j].x += kij * xl[i].x;
-          acc[j].y += kij * xl[i].y;
-        }
-      }
-    }
+    element_product<R, ORDER>(a, xl, acc, std::make_integer_sequence<int, MfTab<ORDER>::NP>{});
     C* out = yl + c * NLOC;
 #pragma unroll
     for (int i = 0; i < NLOC; ++i) out[i] = acc[i];
@@ -180,20 +224,6 @@ __global__ __launch_bounds__(TB) void k_mf_gather(
   if (EPI) reduce_finalize<TB>(rs, G, d1, d2);
 }
 
-// Reference tensors packed as [k][upper-triangle pair p], k < 6: M_k, k >= 6: C_{k-6}.
-template <typename R>
-void pack_ref(const RefData& R_, std::vector<R>& out) {
-  const int n = R_.nloc, np = n * (n + 1) / 2;
-  out.assign((size_t)12 * np, R(0));
-  for (int k = 0; k < 12; ++k) {
-    const std::vector<double>& src = k < 6 ? R_.mass : R_.stiff;
-    const int kk = k % 6;
-    int p = 0;
-    for (int i = 0; i < n; ++i)
-      for (int j = i; j < n; ++j, ++p) out[(size_t)k * np + p] = (R)src[((size_t)kk * n + i) * n + j];
-  }
-}
-
 }  // namespace
 
 void mf_prepare(Context* c, System* sys) {
@@ -222,16 +252,7 @@ void mf_prepare(Context* c, System* sys) {
     s->mf_dptr.alloc(s->ndofs + 1);
     SUCEM_LAUNCH(k_mf_heads, grid_for(n, TB), TB, 0, c->stream, n, s->ndofs, keys_out.p,
                  s->mf_dptr.p);
-    const RefData& R_ = ref_data(s->order);
-    std::vector<double> r64;
-    std::vector<float> r32;
-    pack_ref<double>(R_, r64);
-    pack_ref<float>(R_, r32);
-    s->mf_ref64.alloc(r64.size());
-    s->mf_ref32.alloc(r32.size());
-    h2d(c, s->mf_ref64.p, r64.data(), r64.size());
-    h2d(c, s->mf_ref32.p, r32.data(), r32.size());
-    c->sync();   // temporaries and host vectors are released on return
+    c->sync();   // temporaries are released on return
   }
   sys->mf_coef64.alloc((size_t)12 * nc);
   sys->mf_coef32.alloc((size_t)12 * nc);
@@ -251,21 +272,20 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
   const Space* s = sys->space;
   const int64_t nc = s->mesh->nc;
   const R* coef;
-  const R* ref;
   const R* s0v;
   if constexpr (sizeof(R) == 4) {
-    coef = sys->mf_coef32.p; ref = s->mf_ref32.p; s0v = sys->s0_32.p;
+    coef = sys->mf_coef32.p; s0v = sys->s0_32.p;
   } else {
-    coef = sys->mf_coef64.p; ref = s->mf_ref64.p; s0v = sys->s0_64.p;
+    coef = sys->mf_coef64.p; s0v = sys->s0_64.p;
   }
   C* yl = (C*)sys->mf_yl.p;
-  const int Gc = op_grid(c, (nc + 3) / 4, 8);   // cell blocks, a multiple of 8
-  if (s->nloc == 20)
-    SUCEM_LAUNCH((k_mf_cells<R, 20>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p, ref,
-                 (R)k.s, (R)k.t, (const C*)x, yl);
+  const int Gc = (int)((nc + 8 * TB - 1) / (8 * TB) * 8);   // TB cells per block, multiple of 8
+  if (s->order == 2)
+    SUCEM_LAUNCH((k_mf_cells<R, 2>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p, (R)k.s,
+                 (R)k.t, (const C*)x, yl);
   else
-    SUCEM_LAUNCH((k_mf_cells<R, 6>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p, ref,
-                 (R)k.s, (R)k.t, (const C*)x, yl);
+    SUCEM_LAUNCH((k_mf_cells<R, 1>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p, (R)k.s,
+                 (R)k.t, (const C*)x, yl);
   const bool has_s0 = (sys->which & SUCEM_MAT_S0) && s->nnz0 > 0;
   const int32_t* s0p = has_s0 ? s->s0_ptr.p : nullptr;
   const int n = (int)s->ndofs;
