@@ -5,8 +5,8 @@
 // reads a contiguous segment holding that entry of every system, and the
 // matrix (column indices and (S, T) pairs) is read once per iteration for the
 // whole batch.  Each system runs its own preconditioned COCG (Jacobi + gradient
-// correction) with its own scalars; per-system reductions are finished by the
-// last block, as in the single-system solver.  FP64 throughout.
+// correction) with its own scalars; per-system reductions are finished by a
+// one-block kernel, as in the single-system solver.  FP64 throughout.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -33,12 +33,12 @@ __device__ __forceinline__ void finalize_sys(const RedSite& rs, int NB, int j, d
 }
 
 // Every thread holds values for system j = threadIdx.x % NB.  Block sums per system
-// -> partials part[lb*NB + j] (first G*NB) and part[(G + lb)*NB + j]; the last block
-// sums them per system in block order and finalises each system's scalars.
+// -> partials part[lb*NB + j] (first G*NB) and part[(G + lb)*NB + j]; k_finalize_nb,
+// launched after the producing kernel, sums them per system in block order and
+// finalises each system's scalars (stream order, no device-scope fences).
 template <int NB>
 __device__ void reduce_finalize_nb(const RedSite& rs, int G, double2 v0, double2 v1) {
   __shared__ double2 s0[TB], s1[TB];
-  __shared__ int is_last;
   const int t = threadIdx.x;
   s0[t] = v0;
   s1[t] = v1;
@@ -54,16 +54,13 @@ __device__ void reduce_finalize_nb(const RedSite& rs, int G, double2 v0, double2
   if (t < NB) {
     rs.part[(int64_t)lb * NB + t] = s0[t];
     rs.part[(int64_t)(G + lb) * NB + t] = s1[t];
-    __threadfence();
   }
-  __syncthreads();
-  if (t == 0) {
-    const unsigned ticket = atomicAdd(rs.counter, 1u);
-    is_last = ticket == (unsigned)(G - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
+}
+
+template <int NB>
+__global__ __launch_bounds__(TB) void k_finalize_nb(RedSite rs, int G) {
+  __shared__ double2 s0[TB], s1[TB];
+  const int t = threadIdx.x;
   const int j = t % NB;
   double2 a = make_double2(0, 0), b = make_double2(0, 0);
   for (int g = t / NB; g < G; g += TB / NB) {
@@ -82,7 +79,6 @@ __device__ void reduce_finalize_nb(const RedSite& rs, int G, double2 v0, double2
     __syncthreads();
   }
   if (t < NB) finalize_sys(rs, NB, t, s0[t], s1[t]);
-  if (t == 0) *rs.counter = 0u;
 }
 
 struct Rows { int start, end; };
@@ -382,18 +378,23 @@ void solve_chunk(Context* c, System* sys, int nb, const double* k0, const double
   SUCEM_LAUNCH((k_bdinv<NB>), grid_for(ne, TB), TB, 0, c->stream, (int64_t)n, s->diag.p, sys->st64.p,
                has_s0 ? s->s0_diag.p : nullptr, sys->s0_64.p, coef_d.p, mask, dinv.p);
   SUCEM_LAUNCH((k_bwinv<NB>), grid_for(nl * NB, TB), TB, 0, c->stream, nl, d3, coef_d.p, winv.p);
+  auto finish = [&](const RedSite& rs) {
+    SUCEM_LAUNCH((k_finalize_nb<NB>), 1, TB, 0, c->stream, rs, G);
+  };
   auto precondition = [&](const RedSite& rs) {
     SUCEM_LAUNCH((k_bgrad_t<NB, LR>), grid_for(nl * NB * LR, TB, (int64_t)c->num_cus * 16), TB, 0,
                  c->stream, nl, s->gt_ptr.p, s->gt_col.p, s->gt_val.p, r.p, winv.p, w.p);
     SUCEM_LAUNCH((k_bgrad_z<NB>), G, TB, 0, c->stream, n, G, s->g_ptr.p, s->g_col.p, s->g_val.p,
                  w.p, dinv.p, r.p, z.p, mask, rs);
+    finish(rs);
   };
   auto spmm = [&](const C* xin, C* yout, const C* wv, int epi, const RedSite& rs) {
-    if (epi)
+    if (epi) {
       SUCEM_LAUNCH((k_spmm<NB, LR, 1>), G, TB, 0, c->stream, n, G, s->row_ptr.p, s->col.p,
                    sys->st64.p, has_s0 ? s->s0_ptr.p : nullptr, s->s0_col.p, sys->s0_64.p,
                    coef_d.p, mask, xin, yout, wv, rs);
-    else
+      finish(rs);
+    } else
       SUCEM_LAUNCH((k_spmm<NB, LR, 0>), G, TB, 0, c->stream, n, G, s->row_ptr.p, s->col.p,
                    sys->st64.p, has_s0 ? s->s0_ptr.p : nullptr, s->s0_col.p, sys->s0_64.p,
                    coef_d.p, mask, xin, yout, wv, rs);
@@ -412,6 +413,7 @@ void solve_chunk(Context* c, System* sys, int nb, const double* k0, const double
   };
 
   SUCEM_LAUNCH((k_bnorm<NB>), G, TB, 0, c->stream, n, G, b.p, mask, site(0, FIN_NORM));
+  finish(site(0, FIN_NORM));
   std::vector<double> bn = scal_slot(S_BNORM);
   for (auto& v : bn) v = std::sqrt(v);
   // r = b (x0 = 0) ; z = M r ; p = z
@@ -447,6 +449,7 @@ void solve_chunk(Context* c, System* sys, int nb, const double* k0, const double
     spmm(x.p, ax.p, nullptr, 0, RedSite{});
     SUCEM_LAUNCH((k_bresid<NB>), G, TB, 0, c->stream, n, G, b.p, ax.p, mask, r.p,
                  site(0, FIN_NORM));
+    finish(site(0, FIN_NORM));
     std::vector<double> tr = scal_slot(S_BNORM);
     all_done = true;
     for (int j = 0; j < NB; ++j) {

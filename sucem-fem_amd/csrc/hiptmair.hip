@@ -279,6 +279,7 @@ void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv
   SUCEM_LAUNCH((k_grad_z<R>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p, s->g_col.p, gv,
                (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
                rs.mode != FIN_NONE ? 1 : 0, rs);
+  finish_reduction(c, rs, G);
 }
 template void grad_precond<float>(Context*, const Space*, const void*, const void*, const void*,
                                   void*, void*, const uint8_t*, int, const RedSite&);

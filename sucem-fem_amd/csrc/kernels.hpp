@@ -86,8 +86,8 @@ enum FinalizeMode {
   FIN_SET_RHO,      // s0 = r.z, s1 = |r|^2     -> rho = s0 (start / restart)
 };
 
-// One reduction site: per-block partial sums, a ticket counter (zero between launches)
-// and the scalar array the last block finalises into.
+// One reduction site: per-block partial sums and the scalar array they are finalised
+// into (counter: unused, kept for layout).
 struct RedSite {
   double2* part;      // 2*G entries
   unsigned* counter;  // one per site
@@ -144,37 +144,19 @@ __device__ __forceinline__ void finalize(const RedSite& rs, double2 s0, double2 
   }
 }
 
-// Block sums of (v0, v1) -> partials; the last block to arrive sums all partials in
-// block order (deterministic) and finalises the scalars.  Must be called by every
-// thread of every block.
+// Block sums of (v0, v1) -> partials part[lb] and part[G + lb]; k_finalize, launched
+// after the producing kernel (finish_reduction), sums them in block order
+// (deterministic) and finalises the scalars.  Ordering between the two kernels comes
+// from the stream, so no block needs a device-scope fence (on a multi-XCD part each
+// such fence writes back the XCD's L2).  Must be called by every thread of every block.
 template <int TB>
 __device__ __forceinline__ void reduce_finalize(const RedSite& rs, int G, double2 v0, double2 v1) {
   v0 = block_sum<TB>(v0);
   v1 = block_sum<TB>(v1);
-  __shared__ int is_last;
   if (threadIdx.x == 0) {
     const int lb = xcd_block(G);
     rs.part[lb] = v0;
     rs.part[G + lb] = v1;
-    __threadfence();
-    const unsigned ticket = atomicAdd(rs.counter, 1u);
-    is_last = ticket == (unsigned)(G - 1);
-  }
-  __syncthreads();
-  if (is_last) {
-    __threadfence();
-    double2 a = make_double2(0, 0), b = make_double2(0, 0);
-    for (int i = threadIdx.x; i < G; i += TB) {
-      const double2 p = rs.part[i], q = rs.part[G + i];
-      a.x += p.x; a.y += p.y;
-      b.x += q.x; b.y += q.y;
-    }
-    a = block_sum<TB>(a);
-    b = block_sum<TB>(b);
-    if (threadIdx.x == 0) {
-      finalize(rs, a, b);
-      *rs.counter = 0u;
-    }
   }
 }
 

@@ -12,9 +12,11 @@
 // products accumulate in FP64.
 //
 // Scalars (alpha, beta, rho, omega) live on the device.  Every reducing kernel
-// writes per-block partials; the last block to finish sums them in block order
-// (deterministic) and updates the scalars, so an iteration is a fixed sequence of
-// launches with no host round trip.  The host polls the residual history every
+// writes per-block partials; a one-block kernel launched behind it sums them in
+// block order (deterministic) and updates the scalars, so an iteration is a fixed
+// sequence of launches with no host round trip.  (Letting the last block of the
+// producing kernel finish the sum needs a device-scope fence in every block, which
+// writes back the XCD's L2 and cost up to half of an SpMV.)  The host polls the residual history every
 // `check_every` iterations.
 #include <chrono>
 #include <cmath>
@@ -241,6 +243,18 @@ __global__ __launch_bounds__(TB) void k_bicg_x(int n, int G, const double* scal,
 
 // ---- host driver ----------------------------------------------------------------
 // Operator abstraction over the driven system and the generic CSR matrix.
+__global__ __launch_bounds__(TB) void k_finalize(RedSite rs, int G) {
+  double2 a = make_double2(0, 0), b = make_double2(0, 0);
+  for (int i = threadIdx.x; i < G; i += TB) {
+    const double2 p = rs.part[i], q = rs.part[G + i];
+    a.x += p.x; a.y += p.y;
+    b.x += q.x; b.y += q.y;
+  }
+  a = block_sum<TB>(a);
+  b = block_sum<TB>(b);
+  if (threadIdx.x == 0) finalize(rs, a, b);
+}
+
 struct Op {
   Context* c = nullptr;
   int64_t n = 0;
@@ -362,6 +376,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, x);
   if (lo) SUCEM_LAUNCH((k_zero_c<C>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, xlo);
   SUCEM_LAUNCH(k_norm2, G, TB, 0, c->stream, n, G, b, op.mask, site(SITE_NORM, FIN_NORM));
+  finish_reduction(c, site(SITE_NORM, FIN_NORM), G);
   const double bnorm = std::sqrt(read_scalar(c, scal.p + S_BNORM));
   if (bnorm == 0.0) {
     st.status = 0;
@@ -373,6 +388,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, ax.p);
   SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p,
                cocg ? 1 : 0, site(SITE_RESID, FIN_RESID));
+  finish_reduction(c, site(SITE_RESID, FIN_RESID), G);
   if (grad) {
     precondition(site(SITE_UPDATE, FIN_SET_RHO));
     SUCEM_LAUNCH((k_copy<C>), G, TB, 0, c->stream, n, G, z.p, p.p);
@@ -397,6 +413,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
       op.apply_lo(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
       SUCEM_LAUNCH((k_cocg_xr<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, p.p, q.p, dinv.p,
                    site(SITE_UPDATE, FIN_COCG_BETA, it));
+      finish_reduction(c, site(SITE_UPDATE, FIN_COCG_BETA, it), G);
       SUCEM_LAUNCH((k_cocg_p<C>), G, TB, 0, c->stream, n, G, scal.p, dinv.p, r.p, p.p);
       st.spmv_lo += 1;
     } else {
@@ -408,6 +425,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
       op.apply_lo(shat.p, t.p, s.p, 2, site(SITE_SPMV2, FIN_BICG_OMEGA));
       SUCEM_LAUNCH((k_bicg_x<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, s.p, t.p, phat.p,
                    shat.p, rt.p, site(SITE_UPDATE, FIN_BICG_BETA, it));
+      finish_reduction(c, site(SITE_UPDATE, FIN_BICG_BETA, it), G);
       st.spmv_lo += 2;
     }
     const bool check = ((it + 1) % o.check_every) == 0 || it + 1 == o.max_iter;
@@ -426,6 +444,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     st.reliable_updates += 1;
     SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p,
                  cocg ? 1 : 0, site(SITE_RESID, FIN_RESID));
+    finish_reduction(c, site(SITE_RESID, FIN_RESID), G);
     const double true_rel = std::sqrt(read_scalar(c, scal.p + S_TRUE)) / bnorm;
     rel = true_rel;
     last_true = true_rel;
@@ -453,6 +472,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     st.spmv_hi += 1;
     SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p, 1,
                  site(SITE_RESID, FIN_RESID));
+    finish_reduction(c, site(SITE_RESID, FIN_RESID), G);
     rel = std::sqrt(read_scalar(c, scal.p + S_TRUE)) / bnorm;
     if (st.status > 0 && rel <= o.tol) st.status = 0;
     if (it > o.max_iter) it = o.max_iter;
@@ -479,6 +499,11 @@ void check_opts(const SolveOpts& o) {
 }
 
 }  // namespace
+
+void finish_reduction(Context* c, const RedSite& rs, int G) {
+  if (rs.mode == FIN_NONE) return;
+  SUCEM_LAUNCH(k_finalize, 1, TB, 0, c->stream, rs, G);
+}
 
 void solve_system(Context* c, const System* sys, Coef k, const double2* b, double2* x,
                   const SolveOpts& o, SolveStats& st, double* hist_h, int64_t hist_cap) {

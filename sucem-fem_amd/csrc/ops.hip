@@ -21,8 +21,8 @@ template <> struct ST2<double> { using T = double2; };
 
 // One group of L lanes per row, U loads per lane issued before any is used;
 // each XCD gets a contiguous block of rows.  Epilogue (EPI > 0): per-row
-// op(w_r) * y_r (op = conj when EPI == 2) and |y_r|^2, reduced over the grid
-// with the last block finalising the Krylov scalars.
+// op(w_r) * y_r (op = conj when EPI == 2) and |y_r|^2 as per-block partials; a
+// one-block kernel launched behind it finishes the sum and updates the Krylov scalars.
 template <typename R, int L, int U, int EPI>
 __global__ __launch_bounds__(TB) void k_sys_spmv(
     int n, int G, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
@@ -326,6 +326,7 @@ void launch_sys(Context* c, const System* sys, Coef k, const uint8_t* mask, cons
   else if (epi == 1) SUCEM_LAUNCH((k_sys_spmv<R, L, U, 1>), G, TB, 0, c->stream, SYS_ARGS);
   else SUCEM_LAUNCH((k_sys_spmv<R, L, U, 2>), G, TB, 0, c->stream, SYS_ARGS);
 #undef SYS_ARGS
+  if (epi) finish_reduction(c, rs, G);
 }
 
 template <typename R, int L, int U>
@@ -340,6 +341,7 @@ void launch_csr(Context* c, const Csr* A, const uint8_t* mask, const void* x, vo
   else if (epi == 1) SUCEM_LAUNCH((k_csr_spmv<R, L, U, 1>), G, TB, 0, c->stream, CSR_ARGS);
   else SUCEM_LAUNCH((k_csr_spmv<R, L, U, 2>), G, TB, 0, c->stream, CSR_ARGS);
 #undef CSR_ARGS
+  if (epi) finish_reduction(c, rs, G);
 }
 
 template <typename R, int L>
@@ -426,7 +428,7 @@ template void csr_apply<double>(Context*, const Csr*, const uint8_t*, const void
 template <typename R>
 SpmvCfg sys_cfg(Context* c, const System* sys) {
   const SpmvCfg& t = sys->cfg[sizeof(R) == 4 ? 0 : 1];
-  if (t.L > 0) return t;
+  if (t.L > 0 || t.mf) return t;
   return default_cfg(c, sys->space->ndofs, sys->space->nnz);
 }
 template SpmvCfg sys_cfg<float>(Context*, const System*);

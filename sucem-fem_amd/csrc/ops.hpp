@@ -28,6 +28,10 @@ template <typename R>
 void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x, void* y,
               const void* w, int epi, const RedSite& rs, const SpmvCfg& f);
 
+// Second half of a grid reduction: sum the G block partials of rs and finalise its
+// scalars (krylov.hip); a no-op for rs.mode == FIN_NONE.
+void finish_reduction(Context* c, const RedSite& rs, int G);
+
 // gradient-space preconditioner (hiptmair.hip)
 template <typename R>
 void grad_winv(Context* c, System* sys, Coef k, bool masked, void* winv);
