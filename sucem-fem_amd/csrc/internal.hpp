@@ -54,8 +54,8 @@ struct Space {
   DBuf<double> g_val, gt_val;
   DBuf<float> g_val32, gt_val32;
   // matrix-free operator (built on first use): DOF ids per cell as SoA, DOF -> element
-  // slot (cell * nloc + local) map sorted by DOF
-  DBuf<int32_t> mf_cdT, mf_dptr, mf_slot;
+  // slot (cell * nloc + local) map sorted by DOF, and the slot's position in that order
+  DBuf<int32_t> mf_cdT, mf_dptr, mf_slot, mf_posT;
 };
 
 // SpMV launch configuration: lanes per row, loads per lane, blocks (0 = heuristic).

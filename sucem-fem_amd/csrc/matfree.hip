@@ -10,7 +10,8 @@
 //     zero entries generate nothing), and writes
 //     the element product K x_c (nloc complex values) to a per-cell buffer;
 //  2. k_mf_gather: one thread per DOF row sums its element contributions in cell
-//     order (a DOF -> (cell, local) map sorted once per space), adds the compact ABC
+//     order -- the element products are stored in DOF-sorted order (a permutation
+//     computed once per space), so each row reads one contiguous segment -- adds the compact ABC
 //     row i c0 (S0 x)_r, applies the PEC row mask and the optional Krylov epilogue.
 //
 // HBM traffic per application: 12 coefficients + nloc DOF ids per cell, x and the
@@ -69,6 +70,15 @@ __global__ void k_mf_iota(int64_t n, uint32_t* v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     v[i] = (uint32_t)i;
+}
+
+// posT[l * nc + c] = position of element slot (c, l) in the DOF-sorted order.
+__global__ void k_mf_inverse(int64_t n, int64_t nc, int nloc, const uint32_t* slot, int32_t* posT) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = slot[j];
+    posT[(int64_t)(s % nloc) * nc + s / nloc] = (int32_t)j;
+  }
 }
 
 // dptr[d] = first position of DOF d in the sorted key list (every DOF occurs).
@@ -152,7 +162,8 @@ __device__ __forceinline__ void element_product(const R* a, const typename C2<R>
 // Element products, one thread per cell; consecutive blocks of cells go to one XCD.
 template <typename R, int ORDER>
 __global__ __launch_bounds__(TB) void k_mf_cells(int64_t nc, int G, const R* __restrict__ coef,
-                                                 const int32_t* __restrict__ cdT, R cs, R ct,
+                                                 const int32_t* __restrict__ cdT,
+                                                 const int32_t* __restrict__ posT, R cs, R ct,
                                                  const typename C2<R>::T* __restrict__ x,
                                                  typename C2<R>::T* __restrict__ yl) {
   using C = typename C2<R>::T;
@@ -170,15 +181,16 @@ __global__ __launch_bounds__(TB) void k_mf_cells(int64_t nc, int G, const R* __r
       acc[i] = mk(R(0), R(0));
     }
     element_product<R, ORDER>(a, xl, acc, std::make_integer_sequence<int, MfTab<ORDER>::NP>{});
-    C* out = yl + c * NLOC;
+    // each product goes to its place in the DOF-sorted order: the contributions of
+    // neighbouring cells to one DOF land side by side, and the gather streams them
 #pragma unroll
-    for (int i = 0; i < NLOC; ++i) out[i] = acc[i];
+    for (int i = 0; i < NLOC; ++i) yl[posT[(int64_t)i * nc + c]] = acc[i];
   }
 }
 
 template <typename R, int EPI>
 __global__ __launch_bounds__(TB) void k_mf_gather(
-    int n, int G, const int32_t* __restrict__ dptr, const int32_t* __restrict__ slot,
+    int n, int G, const int32_t* __restrict__ dptr,
     const typename C2<R>::T* __restrict__ yl, const int32_t* __restrict__ s0p,
     const int32_t* __restrict__ s0c, const R* __restrict__ s0v, R c0,
     const uint8_t* __restrict__ mask, const typename C2<R>::T* __restrict__ x,
@@ -194,7 +206,7 @@ __global__ __launch_bounds__(TB) void k_mf_gather(
     C acc = mk(R(0), R(0));
     const int j1 = dptr[r + 1];
     for (int j = dptr[r]; j < j1; ++j) {
-      const C v = yl[slot[j]];
+      const C v = yl[j];
       acc.x += v.x;
       acc.y += v.y;
     }
@@ -252,7 +264,11 @@ void mf_prepare(Context* c, System* sys) {
     s->mf_dptr.alloc(s->ndofs + 1);
     SUCEM_LAUNCH(k_mf_heads, grid_for(n, TB), TB, 0, c->stream, n, s->ndofs, keys_out.p,
                  s->mf_dptr.p);
+    s->mf_posT.alloc(n);
+    SUCEM_LAUNCH(k_mf_inverse, grid_for(n, TB), TB, 0, c->stream, n, nc, nloc,
+                 (const uint32_t*)s->mf_slot.p, s->mf_posT.p);
     c->sync();   // temporaries are released on return
+    s->mf_slot.release();
   }
   sys->mf_coef64.alloc((size_t)12 * nc);
   sys->mf_coef32.alloc((size_t)12 * nc);
@@ -281,15 +297,15 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
   C* yl = (C*)sys->mf_yl.p;
   const int Gc = (int)((nc + 8 * TB - 1) / (8 * TB) * 8);   // TB cells per block, multiple of 8
   if (s->order == 2)
-    SUCEM_LAUNCH((k_mf_cells<R, 2>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p, (R)k.s,
-                 (R)k.t, (const C*)x, yl);
+    SUCEM_LAUNCH((k_mf_cells<R, 2>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p,
+                 s->mf_posT.p, (R)k.s, (R)k.t, (const C*)x, yl);
   else
-    SUCEM_LAUNCH((k_mf_cells<R, 1>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p, (R)k.s,
-                 (R)k.t, (const C*)x, yl);
+    SUCEM_LAUNCH((k_mf_cells<R, 1>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p,
+                 s->mf_posT.p, (R)k.s, (R)k.t, (const C*)x, yl);
   const bool has_s0 = (sys->which & SUCEM_MAT_S0) && s->nnz0 > 0;
   const int32_t* s0p = has_s0 ? s->s0_ptr.p : nullptr;
   const int n = (int)s->ndofs;
-#define MF_ARGS n, f.G, s->mf_dptr.p, s->mf_slot.p, (const C*)yl, s0p, s->s0_col.p, s0v, \
+#define MF_ARGS n, f.G, s->mf_dptr.p, (const C*)yl, s0p, s->s0_col.p, s0v, \
                 (R)k.s0, mask, (const C*)x, (C*)y, (const C*)w, rs
   if (epi == 0) SUCEM_LAUNCH((k_mf_gather<R, 0>), f.G, TB, 0, c->stream, MF_ARGS);
   else if (epi == 1) SUCEM_LAUNCH((k_mf_gather<R, 1>), f.G, TB, 0, c->stream, MF_ARGS);
