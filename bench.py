@@ -128,7 +128,8 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     spmv_ms = e0.elapsed_time(e1) / a.spmv_reps
-    spmv_bytes = prob.spmv_bytes(a.precision)
+    mf = bool(tuned[a.precision].get("matrix_free", 0))
+    spmv_bytes = prob.operator_bytes(a.precision, mf)
     if a.copy_probe_mb > 0:
         src = torch.ones(a.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -136,7 +137,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-    log("spmv %.4f ms, %.1f GB/s" % (spmv_ms, achieved))
+    log("operator %.4f ms, %.1f GB/s (%s)" % (spmv_ms, achieved, "matrix-free" if mf else "CSR"))
 
     extra = {}
     fp64_iters = None
@@ -192,7 +193,9 @@ def main():
         "spmv_tuned": {str(k): v for k, v in tuned.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": traffic,
-                     "kernel": "k_sys_spmv (fused S - k0^2 T + i k0 S0, FP32 values, complex64 x)",
+                     "kernel": ("k_mf_cells + k_mf_gather (matrix-free S - k0^2 T + i k0 S0)"
+                                if mf else "k_sys_spmv (assembled S - k0^2 T + i k0 S0)")
+                               + (", FP32, complex64 x" if a.precision == 32 else ", FP64"),
                      "bytes_per_launch": spmv_bytes, "avg_ms": spmv_ms},
         "cpu_baseline": cpu,
     }

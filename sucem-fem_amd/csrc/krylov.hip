@@ -243,15 +243,28 @@ __global__ __launch_bounds__(TB) void k_bicg_x(int n, int G, const double* scal,
 
 // ---- host driver ----------------------------------------------------------------
 // Operator abstraction over the driven system and the generic CSR matrix.
-__global__ __launch_bounds__(TB) void k_finalize(RedSite rs, int G) {
+// One block of FIN_TB threads: the partials were written by other XCDs, so their
+// loads go to memory; all of a thread's loads are issued before any is summed.
+constexpr int FIN_TB = 1024;
+__global__ __launch_bounds__(FIN_TB) void k_finalize(RedSite rs, int G) {
   double2 a = make_double2(0, 0), b = make_double2(0, 0);
-  for (int i = threadIdx.x; i < G; i += TB) {
-    const double2 p = rs.part[i], q = rs.part[G + i];
-    a.x += p.x; a.y += p.y;
-    b.x += q.x; b.y += q.y;
+  constexpr int K = 4;
+  for (int i0 = threadIdx.x; i0 < G; i0 += K * FIN_TB) {
+    double2 p[K], q[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = i0 + k * FIN_TB;
+      p[k] = i < G ? rs.part[i] : make_double2(0, 0);
+      q[k] = i < G ? rs.part[G + i] : make_double2(0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      a.x += p[k].x; a.y += p[k].y;
+      b.x += q[k].x; b.y += q[k].y;
+    }
   }
-  a = block_sum<TB>(a);
-  b = block_sum<TB>(b);
+  a = block_sum<FIN_TB>(a);
+  b = block_sum<FIN_TB>(b);
   if (threadIdx.x == 0) finalize(rs, a, b);
 }
 
@@ -502,7 +515,7 @@ void check_opts(const SolveOpts& o) {
 
 void finish_reduction(Context* c, const RedSite& rs, int G) {
   if (rs.mode == FIN_NONE) return;
-  SUCEM_LAUNCH(k_finalize, 1, TB, 0, c->stream, rs, G);
+  SUCEM_LAUNCH(k_finalize, 1, FIN_TB, 0, c->stream, rs, G);
 }
 
 void solve_system(Context* c, const System* sys, Coef k, const double2* b, double2* x,

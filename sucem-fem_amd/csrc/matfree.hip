@@ -91,19 +91,23 @@ __global__ void k_mf_heads(int64_t n, int64_t ndofs, const uint32_t* keys, int32
 }
 
 // ---- element matrix with the reference tensors folded into the instructions ----------
-// kMfRef<order>[k][p]: k < 6 mass M_k, k >= 6 curl-curl C_{k-6}; p runs over the upper
-// triangle (i <= j) row by row.  Zero entries (~40 % for order 2) generate no code.
+// kMfNum<order>[k][p] / kMfDen<order>: k < 6 mass M_k, k >= 6 curl-curl C_{k-6}; p runs
+// over the upper triangle (i <= j) row by row.  The numerators are small integers
+// (exact instruction literals in FP32 and FP64); zero entries (~40 % for order 2)
+// generate no code.
 template <int ORDER>
 struct MfTab;
 template <>
 struct MfTab<1> {
   static constexpr int NLOC = 6, NP = kMfNp1;
-  static constexpr double v(int k, int p) { return kMfRef1[k][p]; }
+  static constexpr double den = kMfDen1;
+  static constexpr double v(int k, int p) { return kMfNum1[k][p]; }
 };
 template <>
 struct MfTab<2> {
   static constexpr int NLOC = 20, NP = kMfNp2;
-  static constexpr double v(int k, int p) { return kMfRef2[k][p]; }
+  static constexpr double den = kMfDen2;
+  static constexpr double v(int k, int p) { return kMfNum2[k][p]; }
 };
 
 // (i, j) of upper-triangle pair p
@@ -146,9 +150,6 @@ __device__ __forceinline__ void pair_step(const R* a, const typename C2<R>::T* x
       acc[j].x += kij * xl[i].x;
       acc[j].y += kij * xl[i].y;
     }
-    // FP64 constants live in scalar register pairs: stop the scheduler from gathering
-    // the constants of many pairs at once (they would spill)
-    if constexpr (sizeof(R) == 8) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(TB) void k_mf_cells(int64_t nc, int G, const R* __r
   if (c < nc) {
     R a[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) a[k] = coef[k * nc + c] * (k < 6 ? ct : cs);
+    for (int k = 0; k < 12; ++k) a[k] = coef[k * nc + c] * (k < 6 ? ct : cs) * R(1.0 / MfTab<ORDER>::den);
     C xl[NLOC], acc[NLOC];
 #pragma unroll
     for (int i = 0; i < NLOC; ++i) {
@@ -188,7 +189,11 @@ __global__ __launch_bounds__(TB) void k_mf_cells(int64_t nc, int G, const R* __r
   }
 }
 
-template <typename R, int EPI>
+// Row sums of the DOF-sorted element products: L lanes per row, each summing every
+// L-th product of the row's contiguous segment (a wave covers one contiguous stretch
+// of the buffer), then the compact ABC row and the epilogue.  The summation order
+// is fixed per row, so results are deterministic.
+template <typename R, int EPI, int L>
 __global__ __launch_bounds__(TB) void k_mf_gather(
     int n, int G, const int32_t* __restrict__ dptr,
     const typename C2<R>::T* __restrict__ yl, const int32_t* __restrict__ s0p,
@@ -196,41 +201,54 @@ __global__ __launch_bounds__(TB) void k_mf_gather(
     const uint8_t* __restrict__ mask, const typename C2<R>::T* __restrict__ x,
     typename C2<R>::T* __restrict__ y, const typename C2<R>::T* __restrict__ w, RedSite rs) {
   using C = typename C2<R>::T;
+  constexpr int RPB = TB / L;
+  const int lane = threadIdx.x % L;
   const int lb = xcd_block(G);
   int chunk = (n + G - 1) / G;
-  chunk = (chunk + TB - 1) / TB * TB;
+  chunk = (chunk + RPB - 1) / RPB * RPB;
   const int start = lb * chunk;
   const int end = min(n, start + chunk);
   double2 d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0);
-  for (int r = start + threadIdx.x; r < end; r += TB) {
-    C acc = mk(R(0), R(0));
-    const int j1 = dptr[r + 1];
-    for (int j = dptr[r]; j < j1; ++j) {
-      const C v = yl[j];
-      acc.x += v.x;
-      acc.y += v.y;
-    }
-    if (s0p) {
-      C a0 = mk(R(0), R(0));
-      const int k1 = s0p[r + 1];
-      for (int k = s0p[r]; k < k1; ++k) {
-        const R v = s0v[k];
-        const C xc = x[s0c[k]];
-        a0.x += v * xc.x;
-        a0.y += v * xc.y;
+  for (int base = start; base < end; base += RPB) {
+    const int r = base + threadIdx.x / L;
+    C acc = mk(R(0), R(0)), a0 = mk(R(0), R(0));
+    if (r < end) {
+      const int j1 = dptr[r + 1];
+      for (int j = dptr[r] + lane; j < j1; j += L) {
+        const C v = yl[j];
+        acc.x += v.x;
+        acc.y += v.y;
       }
+      if (s0p) {
+        const int k1 = s0p[r + 1];
+        for (int k = s0p[r] + lane; k < k1; k += L) {
+          const R v = s0v[k];
+          const C xc = x[s0c[k]];
+          a0.x += v * xc.x;
+          a0.y += v * xc.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) {
+      acc.x += __shfl_xor(acc.x, o);
+      acc.y += __shfl_xor(acc.y, o);
+      a0.x += __shfl_xor(a0.x, o);
+      a0.y += __shfl_xor(a0.y, o);
+    }
+    if (lane == 0 && r < end) {
       acc.x -= c0 * a0.y;
       acc.y += c0 * a0.x;
-    }
-    if (mask && mask[r]) acc = mk(R(0), R(0));
-    y[r] = acc;
-    if (EPI) {
-      const double2 yd = to_d(acc);
-      double2 wd = to_d(w[r]);
-      if (EPI == 2) wd.y = -wd.y;
-      d1.x += wd.x * yd.x - wd.y * yd.y;
-      d1.y += wd.x * yd.y + wd.y * yd.x;
-      d2.x += yd.x * yd.x + yd.y * yd.y;
+      if (mask && mask[r]) acc = mk(R(0), R(0));
+      y[r] = acc;
+      if (EPI) {
+        const double2 yd = to_d(acc);
+        double2 wd = to_d(w[r]);
+        if (EPI == 2) wd.y = -wd.y;
+        d1.x += wd.x * yd.x - wd.y * yd.y;
+        d1.y += wd.x * yd.y + wd.y * yd.x;
+        d2.x += yd.x * yd.x + yd.y * yd.y;
+      }
     }
   }
   if (EPI) reduce_finalize<TB>(rs, G, d1, d2);
@@ -307,9 +325,9 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
   const int n = (int)s->ndofs;
 #define MF_ARGS n, f.G, s->mf_dptr.p, (const C*)yl, s0p, s->s0_col.p, s0v, \
                 (R)k.s0, mask, (const C*)x, (C*)y, (const C*)w, rs
-  if (epi == 0) SUCEM_LAUNCH((k_mf_gather<R, 0>), f.G, TB, 0, c->stream, MF_ARGS);
-  else if (epi == 1) SUCEM_LAUNCH((k_mf_gather<R, 1>), f.G, TB, 0, c->stream, MF_ARGS);
-  else SUCEM_LAUNCH((k_mf_gather<R, 2>), f.G, TB, 0, c->stream, MF_ARGS);
+  if (epi == 0) SUCEM_LAUNCH((k_mf_gather<R, 0, 4>), f.G, TB, 0, c->stream, MF_ARGS);
+  else if (epi == 1) SUCEM_LAUNCH((k_mf_gather<R, 1, 4>), f.G, TB, 0, c->stream, MF_ARGS);
+  else SUCEM_LAUNCH((k_mf_gather<R, 2, 4>), f.G, TB, 0, c->stream, MF_ARGS);
 #undef MF_ARGS
   if (epi) finish_reduction(c, rs, f.G);
 }

@@ -141,8 +141,22 @@ class DeviceProblem(object):
         _hip.call("sucem_system_apply", self.ctx.handle, self.system.handle.h, float(k0), precision,
                   0, nvec, _hip.ptr(x), _hip.ptr(y))
 
+    def operator_bytes(self, precision, matrix_free):
+        """Algorithmic HBM bytes of one application of the fused operator (one vector)."""
+        if not matrix_free:
+            return self.spmv_bytes(precision)
+        s = 4 if precision == 32 else 8            # real value
+        sx = 2 * s                                  # complex vector element
+        n, nnz0 = self.ndofs, self.nnz0
+        nc = int(np.prod(self.n)) * 6
+        nloc = 6 if self.order == 1 else 20
+        cells = nc * (12 * s + nloc * 4 + nloc * 4)   # coefficients, DOF ids, product positions
+        products = 2 * nc * nloc * sx                 # element products written, then read
+        gather = (n + 1) * 4 + nnz0 * (4 + s) + (n + 1) * 4
+        return cells + products + gather + 2 * n * sx   # x read once, y written once
+
     def spmv_bytes(self, precision):
-        """Algorithmic HBM bytes of one fused SpMV (one vector)."""
+        """Algorithmic HBM bytes of one assembled fused SpMV (one vector)."""
         sv = 8 if precision == 32 else 16          # (S, T) pair
         s0v = 4 if precision == 32 else 8
         sx = 8 if precision == 32 else 16          # complex vector element
