@@ -119,11 +119,12 @@ __global__ void k_grad_winv(int64_t nlag, const double* d3, double cs, double ct
   }
 }
 
-// w = winv .* (G^T r): L lanes per Lagrange row, U loads per lane in flight, rows
-// split into one contiguous share per XCD (vertex rows hold ~100 entries, edge-node
-// rows ~12; the two kinds are contiguous in the node numbering).
+// w = winv .* (G^T r): L lanes per Lagrange row, U loads per lane in flight.  Rows are
+// dealt out round-robin over the groups of the grid: vertex rows hold ~100 entries and
+// edge-node rows ~12, and the two kinds are contiguous in the node numbering, so
+// contiguous shares per block would leave the vertex blocks with most of the work.
 template <typename R, int L, int U>
-__global__ __launch_bounds__(TB) void k_grad_t(int64_t nlag, int G,
+__global__ __launch_bounds__(TB) void k_grad_t(int64_t nlag,
                                                const int32_t* __restrict__ gt_ptr,
                                                const int32_t* __restrict__ gt_col,
                                                const R* __restrict__ gt_val,
@@ -131,17 +132,13 @@ __global__ __launch_bounds__(TB) void k_grad_t(int64_t nlag, int G,
                                                const typename C2<R>::T* __restrict__ winv,
                                                typename C2<R>::T* __restrict__ w) {
   using C = typename C2<R>::T;
-  constexpr int RPB = TB / L;
   const int lane = threadIdx.x % L;
-  const int lb = xcd_block(G);
-  int64_t chunk = (nlag + G - 1) / G;
-  chunk = (chunk + RPB - 1) / RPB * RPB;
-  const int64_t start = (int64_t)lb * chunk;
-  const int64_t end = min(nlag, start + chunk);
-  for (int64_t base = start; base < end; base += RPB) {
-    const int64_t j = base + threadIdx.x / L;
+  const int64_t group = (blockIdx.x * (int64_t)TB + threadIdx.x) / L;
+  const int64_t ngroups = ((int64_t)gridDim.x * TB) / L;
+  for (int64_t jb = 0; jb < nlag; jb += ngroups) {
+    const int64_t j = jb + group;
     C acc = mk(R(0), R(0));
-    if (j < end) {
+    if (j < nlag) {
       const int q1 = gt_ptr[j + 1];
       for (int qb = gt_ptr[j] + lane; qb < q1; qb += L * U) {
         int cc[U];
@@ -167,7 +164,7 @@ __global__ __launch_bounds__(TB) void k_grad_t(int64_t nlag, int G,
       acc.x += __shfl_xor(acc.x, o);
       acc.y += __shfl_xor(acc.y, o);
     }
-    if (lane == 0 && j < end) w[j] = cmul(winv[j], acc);
+    if (lane == 0 && j < nlag) w[j] = cmul(winv[j], acc);
   }
 }
 
@@ -294,9 +291,9 @@ void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv
   const R* gv;
   if constexpr (sizeof(R) == 4) { gtv = s->gt_val32.p; gv = s->g_val32.p; }
   else { gtv = s->gt_val.p; gv = s->g_val.p; }
-  const int Gt = op_grid(c, s->nlag, 16);
-  SUCEM_LAUNCH((k_grad_t<R, 8, 4>), Gt, TB, 0, c->stream, s->nlag, Gt, s->gt_ptr.p, s->gt_col.p,
-               gtv, (const C*)r, (const C*)winv, (C*)w);
+  SUCEM_LAUNCH((k_grad_t<R, 8, 4>), grid_for(s->nlag * 8, TB, (int64_t)c->num_cus * 16), TB, 0,
+               c->stream, s->nlag, s->gt_ptr.p, s->gt_col.p, gtv, (const C*)r, (const C*)winv,
+               (C*)w);
   SUCEM_LAUNCH((k_grad_z<R>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p, s->g_col.p, gv,
                (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
                rs.mode != FIN_NONE ? 1 : 0, rs);

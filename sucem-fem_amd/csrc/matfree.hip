@@ -325,9 +325,15 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
   const int n = (int)s->ndofs;
 #define MF_ARGS n, f.G, s->mf_dptr.p, (const C*)yl, s0p, s->s0_col.p, s0v, \
                 (R)k.s0, mask, (const C*)x, (C*)y, (const C*)w, rs
-  if (epi == 0) SUCEM_LAUNCH((k_mf_gather<R, 0, 4>), f.G, TB, 0, c->stream, MF_ARGS);
-  else if (epi == 1) SUCEM_LAUNCH((k_mf_gather<R, 1, 4>), f.G, TB, 0, c->stream, MF_ARGS);
-  else SUCEM_LAUNCH((k_mf_gather<R, 2, 4>), f.G, TB, 0, c->stream, MF_ARGS);
+  if (f.L == 4) {
+    if (epi == 0) SUCEM_LAUNCH((k_mf_gather<R, 0, 4>), f.G, TB, 0, c->stream, MF_ARGS);
+    else if (epi == 1) SUCEM_LAUNCH((k_mf_gather<R, 1, 4>), f.G, TB, 0, c->stream, MF_ARGS);
+    else SUCEM_LAUNCH((k_mf_gather<R, 2, 4>), f.G, TB, 0, c->stream, MF_ARGS);
+  } else {
+    if (epi == 0) SUCEM_LAUNCH((k_mf_gather<R, 0, 1>), f.G, TB, 0, c->stream, MF_ARGS);
+    else if (epi == 1) SUCEM_LAUNCH((k_mf_gather<R, 1, 1>), f.G, TB, 0, c->stream, MF_ARGS);
+    else SUCEM_LAUNCH((k_mf_gather<R, 2, 1>), f.G, TB, 0, c->stream, MF_ARGS);
+  }
 #undef MF_ARGS
   if (epi) finish_reduction(c, rs, f.G);
 }
