@@ -119,12 +119,13 @@ __global__ void k_grad_winv(int64_t nlag, const double* d3, double cs, double ct
   }
 }
 
-// w = winv .* (G^T r): L lanes per Lagrange row, U loads per lane in flight.  Rows are
-// dealt out round-robin over the groups of the grid: vertex rows hold ~100 entries and
-// edge-node rows ~12, and the two kinds are contiguous in the node numbering, so
-// contiguous shares per block would leave the vertex blocks with most of the work.
+// w = winv .* (G^T r): L lanes per Lagrange row, U loads per lane in flight.  Vertex
+// rows hold ~100 entries, edge-node rows ~12, and the two kinds are contiguous in the
+// node numbering [0, nv) + [nv, nlag).  Each XCD takes one contiguous eighth of each
+// kind (neighbouring rows share the r entries they gather, so they should share an
+// L2), and inside an XCD its blocks deal the rows out round-robin (balanced work).
 template <typename R, int L, int U>
-__global__ __launch_bounds__(TB) void k_grad_t(int64_t nlag,
+__global__ __launch_bounds__(TB) void k_grad_t(int64_t nv, int64_t nlag,
                                                const int32_t* __restrict__ gt_ptr,
                                                const int32_t* __restrict__ gt_col,
                                                const R* __restrict__ gt_val,
@@ -133,12 +134,19 @@ __global__ __launch_bounds__(TB) void k_grad_t(int64_t nlag,
                                                typename C2<R>::T* __restrict__ w) {
   using C = typename C2<R>::T;
   const int lane = threadIdx.x % L;
-  const int64_t group = (blockIdx.x * (int64_t)TB + threadIdx.x) / L;
-  const int64_t ngroups = ((int64_t)gridDim.x * TB) / L;
-  for (int64_t jb = 0; jb < nlag; jb += ngroups) {
-    const int64_t j = jb + group;
+  const int per_xcd = gridDim.x / 8;                 // the grid is a multiple of 8
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+  const int64_t group = ((int64_t)local * TB + threadIdx.x) / L;
+  const int64_t ngroups = ((int64_t)per_xcd * TB) / L;
+  const int64_t ne = nlag - nv;
+  const int64_t v0 = nv * xcd / 8, v1 = nv * (xcd + 1) / 8;
+  const int64_t e0 = nv + ne * xcd / 8, e1 = nv + ne * (xcd + 1) / 8;
+  const int64_t nmine = (v1 - v0) + (e1 - e0);
+  for (int64_t jb = 0; jb < nmine; jb += ngroups) {
+    const int64_t t = jb + group;
+    const int64_t j = t < v1 - v0 ? v0 + t : e0 + (t - (v1 - v0));
     C acc = mk(R(0), R(0));
-    if (j < nlag) {
+    if (t < nmine) {
       const int q1 = gt_ptr[j + 1];
       for (int qb = gt_ptr[j] + lane; qb < q1; qb += L * U) {
         int cc[U];
@@ -164,7 +172,7 @@ __global__ __launch_bounds__(TB) void k_grad_t(int64_t nlag,
       acc.x += __shfl_xor(acc.x, o);
       acc.y += __shfl_xor(acc.y, o);
     }
-    if (lane == 0 && j < nlag) w[j] = cmul(winv[j], acc);
+    if (lane == 0 && t < nmine) w[j] = cmul(winv[j], acc);
   }
 }
 
@@ -291,9 +299,9 @@ void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv
   const R* gv;
   if constexpr (sizeof(R) == 4) { gtv = s->gt_val32.p; gv = s->g_val32.p; }
   else { gtv = s->gt_val.p; gv = s->g_val.p; }
-  SUCEM_LAUNCH((k_grad_t<R, 8, 4>), grid_for(s->nlag * 8, TB, (int64_t)c->num_cus * 16), TB, 0,
-               c->stream, s->nlag, s->gt_ptr.p, s->gt_col.p, gtv, (const C*)r, (const C*)winv,
-               (C*)w);
+  const int Gt = op_grid(c, s->nlag, 16);               // multiple of 8
+  SUCEM_LAUNCH((k_grad_t<R, 8, 4>), Gt, TB, 0, c->stream, (int64_t)s->mesh->nv, s->nlag,
+               s->gt_ptr.p, s->gt_col.p, gtv, (const C*)r, (const C*)winv, (C*)w);
   SUCEM_LAUNCH((k_grad_z<R>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p, s->g_col.p, gv,
                (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
                rs.mode != FIN_NONE ? 1 : 0, rs);
