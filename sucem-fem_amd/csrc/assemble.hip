@@ -234,6 +234,13 @@ void assemble_system(Context* c, Space* s, System* sys, unsigned which) {
   }
 }
 
+void face_coefs(Context* c, const Space* s, double* H_d) {
+  const Mesh* m = s->mesh;
+  if (s->nbf)
+    SUCEM_LAUNCH(k_face_coefs, grid_for(s->nbf, TB), TB, 0, c->stream, s->nbf, m->xyz.p,
+                 m->cells.p, s->bf_cell.p, s->bf_local.p, H_d);
+}
+
 void export_lhs(Context* c, const System* sys, double k0, double2* out_d) {
   const Space* s = sys->space;
   SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S/T were not assembled");

@@ -9,6 +9,7 @@
 // term) almost untouched; the second term acts on exactly that subspace.  M is
 // symmetric, so COCG stays applicable.  A = cs S + ct T + i c0 S0 and
 // G^T S G = 0, so its diagonal is formed per solve from three stored diagonals.
+#include <cmath>
 #include <vector>
 
 #include "internal.hpp"
@@ -40,17 +41,23 @@ __global__ void k_grad_keys(int64_t nc, int nloc, int nlagl, int npairs, const i
   }
 }
 
-// value of each unique entry = the local gradient entry of its first contribution
+// value of each unique entry = the local gradient entry of its first contribution; also
+// packed with its column as (col << 4) | (k & 15), value = k / den
 __global__ void k_grad_vals(int64_t nnz, const int32_t* seg, const uint32_t* perm, int npairs,
-                            const double* pair_vals, double* v64, float* v32) {
+                            const double* pair_vals, const int32_t* pair_num,
+                            const int32_t* colv, double* v64, float* v32, uint32_t* pk) {
   for (int64_t z = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; z < nnz;
        z += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t id = perm[seg[z]];
-    const double v = pair_vals[id % (uint32_t)npairs];
+    const uint32_t id = perm[seg[z]] % (uint32_t)npairs;
+    const double v = pair_vals[id];
     v64[z] = v;
     v32[z] = (float)v;
+    pk[z] = ((uint32_t)colv[z] << 4) | ((uint32_t)pair_num[id] & 15u);
   }
 }
+
+__device__ __forceinline__ int pk_col(uint32_t p) { return (int)(p >> 4); }
+__device__ __forceinline__ int pk_num(uint32_t p) { return ((int)(p << 28)) >> 28; }  // signed
 
 __device__ __forceinline__ int find_col(const int32_t* col, int lo, int hi, int target) {
   hi -= 1;
@@ -127,8 +134,7 @@ __global__ void k_grad_winv(int64_t nlag, const double* d3, double cs, double ct
 template <typename R, int L, int U>
 __global__ __launch_bounds__(TB) void k_grad_t(int64_t nv, int64_t nlag,
                                                const int32_t* __restrict__ gt_ptr,
-                                               const int32_t* __restrict__ gt_col,
-                                               const R* __restrict__ gt_val,
+                                               const uint32_t* __restrict__ gt_pk, R inv_den,
                                                const typename C2<R>::T* __restrict__ r,
                                                const typename C2<R>::T* __restrict__ winv,
                                                typename C2<R>::T* __restrict__ w) {
@@ -154,8 +160,9 @@ __global__ __launch_bounds__(TB) void k_grad_t(int64_t nv, int64_t nlag,
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int q = qb + u * L;
-          cc[u] = q < q1 ? gt_col[q] : -1;
-          gg[u] = q < q1 ? gt_val[q] : R(0);
+          const uint32_t p = q < q1 ? gt_pk[q] : 0u;
+          cc[u] = q < q1 ? pk_col(p) : -1;
+          gg[u] = (R)pk_num(p);
         }
         C v[U];
 #pragma unroll
@@ -172,6 +179,8 @@ __global__ __launch_bounds__(TB) void k_grad_t(int64_t nv, int64_t nlag,
       acc.x += __shfl_xor(acc.x, o);
       acc.y += __shfl_xor(acc.y, o);
     }
+    acc.x *= inv_den;
+    acc.y *= inv_den;
     if (lane == 0 && t < nmine) w[j] = cmul(winv[j], acc);
   }
 }
@@ -179,8 +188,7 @@ __global__ __launch_bounds__(TB) void k_grad_t(int64_t nv, int64_t nlag,
 // z = dinv .* r + G w (masked rows zero); optional reduction of r.z and |r|^2
 template <typename R>
 __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __restrict__ g_ptr,
-                                               const int32_t* __restrict__ g_col,
-                                               const R* __restrict__ g_val,
+                                               const uint32_t* __restrict__ g_pk, R inv_den,
                                                const typename C2<R>::T* __restrict__ w,
                                                const typename C2<R>::T* __restrict__ dinv,
                                                const typename C2<R>::T* __restrict__ r,
@@ -194,13 +202,16 @@ __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __re
   double2 d1 = make_double2(0, 0), d2 = make_double2(0, 0);
   for (int i = start + threadIdx.x; i < end; i += TB) {
     const C ri = r[i];
-    C zi = cmul(dinv[i], ri);
+    C gw = mk(R(0), R(0));
     for (int q = g_ptr[i]; q < g_ptr[i + 1]; ++q) {
-      const R g = g_val[q];
-      const C v = w[g_col[q]];
-      zi.x += g * v.x;
-      zi.y += g * v.y;
+      const uint32_t p = g_pk[q];
+      const R g = (R)pk_num(p);
+      const C v = w[pk_col(p)];
+      gw.x += g * v.x;
+      gw.y += g * v.y;
     }
+    const C di = cmul(dinv[i], ri);
+    C zi = mk(di.x + inv_den * gw.x, di.y + inv_den * gw.y);
     if (mask && mask[i]) zi = mk(R(0), R(0));
     z[i] = zi;
     if (reduce) {
@@ -220,20 +231,30 @@ void ensure_gradient(Context* c, Space* s) {
   const RefData& R = ref_data(s->order);
   const Mesh* m = s->mesh;
   s->nlag = s->order == 1 ? m->nv : m->nv + s->nE;
-  std::vector<int32_t> pairs;
+  std::vector<int32_t> pairs, pnum;
   std::vector<double> pvals;
+  // local entries are integers (order 1) or thirds (order 2): k / den with |k| <= 7
+  s->g_den = s->order == 1 ? 1.0 : 3.0;
   for (int i = 0; i < R.nloc; ++i)
     for (int j = 0; j < R.nlag; ++j)
       if (R.grad[i * R.nlag + j] != 0.0) {
+        const double v = R.grad[i * R.nlag + j];
+        const long k = std::lround(v * s->g_den);
+        SUCEM_CHECK(k != 0 && k >= -8 && k <= 7 && std::fabs(k / s->g_den - v) < 1e-12,
+                    SUCEM_ERR_INVALID, "unexpected discrete-gradient entry");
         pairs.push_back(i);
         pairs.push_back(j);
-        pvals.push_back(R.grad[i * R.nlag + j]);
+        pvals.push_back(v);
+        pnum.push_back((int32_t)k);
       }
   const int npairs = (int)pvals.size();
-  DBuf<int32_t> pairs_d(pairs.size());
+  SUCEM_CHECK(std::max<int64_t>(s->ndofs, m->nv + s->nE) < (int64_t(1) << 28),
+              SUCEM_ERR_UNSUPPORTED, "too many DOFs for the packed gradient");
+  DBuf<int32_t> pairs_d(pairs.size()), pnum_d(pnum.size());
   DBuf<double> pvals_d(pvals.size());
   h2d(c, pairs_d.p, pairs.data(), pairs.size());
   h2d(c, pvals_d.p, pvals.data(), pvals.size());
+  h2d(c, pnum_d.p, pnum.data(), pnum.size());
   const int64_t n = m->nc * npairs;
   SUCEM_CHECK(n < (int64_t(1) << 31), SUCEM_ERR_UNSUPPORTED, "gradient too large");
   const int B = key_bits(std::max<int64_t>(s->ndofs, s->nlag));
@@ -249,12 +270,14 @@ void ensure_gradient(Context* c, Space* s) {
     DBuf<int32_t>& colb = transpose ? s->gt_col : s->g_col;
     DBuf<double>& v64 = transpose ? s->gt_val : s->g_val;
     DBuf<float>& v32 = transpose ? s->gt_val32 : s->g_val32;
+    DBuf<uint32_t>& pk = transpose ? s->gt_pk : s->g_pk;
     const int64_t nnz = build_csr_from_pairs(c, n, keys, vals, B, transpose ? s->nlag : s->ndofs,
                                              ptr, colb, seg, perm);
     v64.alloc(nnz);
     v32.alloc(nnz);
+    pk.alloc(nnz);
     SUCEM_LAUNCH(k_grad_vals, grid_for(nnz, TB), TB, 0, c->stream, nnz, seg.p, perm.p, npairs,
-                 pvals_d.p, v64.p, v32.p);
+                 pvals_d.p, pnum_d.p, colb.p, v64.p, v32.p, pk.p);
     s->nnz_g = nnz;
     c->sync();
   }
@@ -295,15 +318,12 @@ template <typename R>
 void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv, const void* r,
                   void* w, void* z, const uint8_t* mask, int G, const RedSite& rs) {
   using C = typename C2<R>::T;
-  const R* gtv;
-  const R* gv;
-  if constexpr (sizeof(R) == 4) { gtv = s->gt_val32.p; gv = s->g_val32.p; }
-  else { gtv = s->gt_val.p; gv = s->g_val.p; }
+  const R inv_den = (R)(1.0 / s->g_den);
   const int Gt = op_grid(c, s->nlag, 16);               // multiple of 8
   SUCEM_LAUNCH((k_grad_t<R, 8, 4>), Gt, TB, 0, c->stream, (int64_t)s->mesh->nv, s->nlag,
-               s->gt_ptr.p, s->gt_col.p, gtv, (const C*)r, (const C*)winv, (C*)w);
-  SUCEM_LAUNCH((k_grad_z<R>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p, s->g_col.p, gv,
-               (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
+               s->gt_ptr.p, s->gt_pk.p, inv_den, (const C*)r, (const C*)winv, (C*)w);
+  SUCEM_LAUNCH((k_grad_z<R>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p, s->g_pk.p,
+               inv_den, (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
                rs.mode != FIN_NONE ? 1 : 0, rs);
   finish_reduction(c, rs, G);
 }

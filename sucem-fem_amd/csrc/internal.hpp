@@ -53,9 +53,14 @@ struct Space {
   DBuf<int32_t> g_ptr, g_col, gt_ptr, gt_col;
   DBuf<double> g_val, gt_val;
   DBuf<float> g_val32, gt_val32;
-  // matrix-free operator (built on first use): DOF ids per cell as SoA, DOF -> element
-  // slot (cell * nloc + local) map sorted by DOF, and the slot's position in that order
-  DBuf<int32_t> mf_cdT, mf_dptr, mf_slot, mf_posT;
+  // the same entries packed as (column << 4) | (k & 15): value = k / g_den with a small
+  // integer k (the local gradient entries are k/3 for order 2, k for order 1)
+  DBuf<uint32_t> g_pk, gt_pk;
+  double g_den = 1.0;
+  // matrix-free operator (built on first use): DOF ids per cell as SoA; the products of
+  // cells (cell * nloc + local) and boundary faces (face * nfd + face DOF) sorted by DOF:
+  // segment pointers per DOF and each product's position in that order
+  DBuf<int32_t> mf_cdT, mf_dptr, mf_posT, mf_posF;
 };
 
 // SpMV launch configuration: lanes per row, loads per lane, blocks (0 = heuristic).
@@ -80,8 +85,8 @@ struct System {
   DBuf<float> s0_32;
   // matrix-free operator: per-cell coefficients (SoA, 12 per cell) and the element buffer
   bool mf_ready = false;
-  DBuf<double> mf_coef64;
-  DBuf<float> mf_coef32;
+  DBuf<double> mf_coef64, mf_face64;    // 12 per cell / 6 per boundary face, SoA
+  DBuf<float> mf_coef32, mf_face32;
   DBuf<double2> mf_yl;
 };
 
@@ -182,6 +187,9 @@ struct EigStats {
 };
 void eigs_system(Context* c, const System* sys, double sigma, int nev, const EigOpts& o,
                  double* lam_h, double* resid_h, double2* vecs_d, EigStats& st);
+
+// assemble.hip: per boundary face, H[6 f + k] = (area / ref area) (J^-1 (I - n n^T) J^-T)_k
+void face_coefs(Context* c, const Space* s, double* H_d);
 
 // postproc.hip -- field post-processing on the boundary
 // Free-space constants as the reference defines them (Consts.py:24-33):

@@ -9,14 +9,17 @@
 //     reference tensors M_k, C_k compiled into the instructions (generated tables;
 //     zero entries generate nothing), and writes
 //     the element product K x_c (nloc complex values) to a per-cell buffer;
-//  2. k_mf_gather: one thread per DOF row sums its element contributions in cell
-//     order -- the element products are stored in DOF-sorted order (a permutation
-//     computed once per space), so each row reads one contiguous segment -- adds the compact ABC
-//     row i c0 (S0 x)_r, applies the PEC row mask and the optional Krylov epilogue.
+//  2. k_mf_faces: one thread per boundary face forms its ABC matrix i c0 S0_f
+//     (nfd x nfd) and writes S0_f x_f the same way;
+//  3. k_mf_gather: each DOF row sums its products -- they are stored in DOF-sorted
+//     order (cells first, then faces, each in index order: a permutation computed
+//     once per space), so a row is one contiguous segment -- then applies the PEC row
+//     mask and the optional Krylov epilogue.
 //
-// HBM traffic per application: 12 coefficients + nloc DOF ids per cell, x and the
-// element buffer, the DOF map, y -- about a fifth of the assembled operator's
-// bytes for order 2 -- for ~3.3k FMAs per order-2 cell.  The order of every sum is
+// HBM traffic per application: 12 coefficients, nloc DOF ids and nloc product positions
+// per cell, x, the product buffer (written and read once), the segment pointers, y --
+// about a fifth of the assembled operator's bytes for order 2 -- for ~2.1k FMAs per
+// order-2 cell.  The order of every sum is
 // fixed, so results are deterministic.  The element buffer belongs to the System:
 // concurrent applications of one System on different streams are not supported.
 #include <cstring>
@@ -72,12 +75,82 @@ __global__ void k_mf_iota(int64_t n, uint32_t* v) {
     v[i] = (uint32_t)i;
 }
 
-// posT[l * nc + c] = position of element slot (c, l) in the DOF-sorted order.
-__global__ void k_mf_inverse(int64_t n, int64_t nc, int nloc, const uint32_t* slot, int32_t* posT) {
+// Product slots: cells s < nc * nloc (s = c * nloc + l), then faces (nc * nloc + f * nfd + a).
+// posT[l * nc + c] / posF[f * nfd + a] = the slot's position in the DOF-sorted order.
+__global__ void k_mf_inverse(int64_t n, int64_t nc, int nloc, const uint32_t* slot, int32_t* posT,
+                             int32_t* posF) {
+  const int64_t ncl = nc * nloc;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
        j += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t s = slot[j];
-    posT[(int64_t)(s % nloc) * nc + s / nloc] = (int32_t)j;
+    const int64_t s = slot[j];
+    if (s < ncl) posT[(s % nloc) * nc + s / nloc] = (int32_t)j;
+    else posF[s - ncl] = (int32_t)j;
+  }
+}
+
+// Sort keys: the DOF of every cell slot, then of every boundary-face slot.
+__global__ void k_mf_keys(int64_t nc, int nloc, int64_t nbf, int nfd, const int32_t* cd,
+                          const int32_t* bf_cell, const int32_t* bf_local,
+                          const int32_t* face_dofs, uint32_t* keys) {
+  const int64_t ncl = nc * nloc, n = ncl + nbf * nfd;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < ncl) {
+      keys[i] = (uint32_t)cd[i];
+    } else {
+      const int64_t f = (i - ncl) / nfd;
+      const int a = (int)((i - ncl) % nfd);
+      keys[i] = (uint32_t)cd[(int64_t)bf_cell[f] * nloc + face_dofs[bf_local[f] * nfd + a]];
+    }
+  }
+}
+
+__global__ void k_mf_soa6(int64_t nbf, const double* H, double* h64, float* h32) {
+  for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nbf;
+       f += (int64_t)gridDim.x * blockDim.x)
+    for (int k = 0; k < 6; ++k) {
+      h64[k * nbf + f] = H[6 * f + k];
+      h32[k * nbf + f] = (float)H[6 * f + k];
+    }
+}
+
+// ABC face products: v = i c0 S0_f x_f written to the face slots (zero when c0 == 0).
+template <typename R, int NFD>
+__global__ __launch_bounds__(TB) void k_mf_faces(int64_t nbf, int nloc, const R* __restrict__ h,
+                                                 const int32_t* __restrict__ bf_cell,
+                                                 const int32_t* __restrict__ bf_local,
+                                                 const int32_t* __restrict__ cd,
+                                                 const int32_t* __restrict__ face_dofs,
+                                                 const double* __restrict__ ref_face, R c0,
+                                                 const int32_t* __restrict__ posF,
+                                                 const typename C2<R>::T* __restrict__ x,
+                                                 typename C2<R>::T* __restrict__ yl) {
+  using C = typename C2<R>::T;
+  constexpr int KK = NFD * NFD;
+  for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nbf;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    const int lf = bf_local[f];
+    const int64_t c = bf_cell[f];
+    R hk[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) hk[k] = h[k * nbf + f];
+    C xf[NFD];
+#pragma unroll
+    for (int a = 0; a < NFD; ++a) xf[a] = x[cd[c * nloc + face_dofs[lf * NFD + a]]];
+    const double* rf = ref_face + (int64_t)lf * 6 * KK;
+#pragma unroll
+    for (int a = 0; a < NFD; ++a) {
+      C v = mk(R(0), R(0));
+#pragma unroll
+      for (int b = 0; b < NFD; ++b) {
+        R m = R(0);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) m += hk[k] * (R)rf[k * KK + a * NFD + b];
+        v.x += m * xf[b].x;
+        v.y += m * xf[b].y;
+      }
+      yl[posF[f * NFD + a]] = mk(-c0 * v.y, c0 * v.x);
+    }
   }
 }
 
@@ -189,17 +262,15 @@ __global__ __launch_bounds__(TB) void k_mf_cells(int64_t nc, int G, const R* __r
   }
 }
 
-// Row sums of the DOF-sorted element products: L lanes per row, each summing every
-// L-th product of the row's contiguous segment (a wave covers one contiguous stretch
-// of the buffer), then the compact ABC row and the epilogue.  The summation order
-// is fixed per row, so results are deterministic.
+// Row sums of the DOF-sorted products: L lanes per row, each summing every L-th
+// product of the row's contiguous segment (a wave covers one contiguous stretch of the
+// buffer), then the PEC row mask and the epilogue.  The summation order is fixed per
+// row, so results are deterministic.
 template <typename R, int EPI, int L>
 __global__ __launch_bounds__(TB) void k_mf_gather(
-    int n, int G, const int32_t* __restrict__ dptr,
-    const typename C2<R>::T* __restrict__ yl, const int32_t* __restrict__ s0p,
-    const int32_t* __restrict__ s0c, const R* __restrict__ s0v, R c0,
-    const uint8_t* __restrict__ mask, const typename C2<R>::T* __restrict__ x,
-    typename C2<R>::T* __restrict__ y, const typename C2<R>::T* __restrict__ w, RedSite rs) {
+    int n, int G, const int32_t* __restrict__ dptr, const typename C2<R>::T* __restrict__ yl,
+    const uint8_t* __restrict__ mask, typename C2<R>::T* __restrict__ y,
+    const typename C2<R>::T* __restrict__ w, RedSite rs) {
   using C = typename C2<R>::T;
   constexpr int RPB = TB / L;
   const int lane = threadIdx.x % L;
@@ -211,7 +282,7 @@ __global__ __launch_bounds__(TB) void k_mf_gather(
   double2 d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0);
   for (int base = start; base < end; base += RPB) {
     const int r = base + threadIdx.x / L;
-    C acc = mk(R(0), R(0)), a0 = mk(R(0), R(0));
+    C acc = mk(R(0), R(0));
     if (r < end) {
       const int j1 = dptr[r + 1];
       for (int j = dptr[r] + lane; j < j1; j += L) {
@@ -219,26 +290,13 @@ __global__ __launch_bounds__(TB) void k_mf_gather(
         acc.x += v.x;
         acc.y += v.y;
       }
-      if (s0p) {
-        const int k1 = s0p[r + 1];
-        for (int k = s0p[r] + lane; k < k1; k += L) {
-          const R v = s0v[k];
-          const C xc = x[s0c[k]];
-          a0.x += v * xc.x;
-          a0.y += v * xc.y;
-        }
-      }
     }
 #pragma unroll
     for (int o = L / 2; o > 0; o >>= 1) {
       acc.x += __shfl_xor(acc.x, o);
       acc.y += __shfl_xor(acc.y, o);
-      a0.x += __shfl_xor(a0.x, o);
-      a0.y += __shfl_xor(a0.y, o);
     }
     if (lane == 0 && r < end) {
-      acc.x -= c0 * a0.y;
-      acc.y += c0 * a0.x;
       if (mask && mask[r]) acc = mk(R(0), R(0));
       y[r] = acc;
       if (EPI) {
@@ -260,33 +318,34 @@ void mf_prepare(Context* c, System* sys) {
   if (sys->mf_ready) return;
   Space* s = sys->space;
   const Mesh* m = s->mesh;
-  const int64_t nc = m->nc;
-  const int nloc = s->nloc;
+  const int64_t nc = m->nc, nbf = s->nbf;
+  const int nloc = s->nloc, nfd = s->nfd;
   if (!s->mf_cdT.p) {
-    // DOF ids per cell (SoA) and the DOF -> element-slot map, sorted by (DOF, slot)
+    // DOF ids per cell (SoA); products sorted by (DOF, slot) and their positions
     s->mf_cdT.alloc((size_t)nc * nloc);
     SUCEM_LAUNCH(k_mf_transpose_dofs, grid_for(nc * nloc, TB), TB, 0, c->stream, nc, nloc,
                  s->cell_dofs.p, s->mf_cdT.p);
-    const int64_t n = nc * nloc;
-    DBuf<uint32_t> keys_out(n);
-    DBuf<uint32_t> vin(n);
-    s->mf_slot.alloc(n);
+    const int64_t n = nc * nloc + nbf * nfd;
+    SUCEM_CHECK(n < (int64_t(1) << 31), SUCEM_ERR_UNSUPPORTED, "too many element products");
+    DBuf<uint32_t> keys(n), keys_out(n), vin(n), slot(n);
+    SUCEM_LAUNCH(k_mf_keys, grid_for(n, TB), TB, 0, c->stream, nc, nloc, nbf, nfd, s->cell_dofs.p,
+                 s->bf_cell.p, s->bf_local.p, s->ref_face_dofs.p, keys.p);
     SUCEM_LAUNCH(k_mf_iota, grid_for(n, TB), TB, 0, c->stream, n, vin.p);
     const unsigned bits = (unsigned)key_bits(s->ndofs);
     size_t bytes = 0;
-    SUCEM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)s->cell_dofs.p, keys_out.p, vin.p,
-                                        (uint32_t*)s->mf_slot.p, (size_t)n, 0u, bits, c->stream));
+    SUCEM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys.p, keys_out.p, vin.p, slot.p,
+                                        (size_t)n, 0u, bits, c->stream));
     DBuf<char> tmp(bytes);
-    SUCEM_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, (const uint32_t*)s->cell_dofs.p, keys_out.p, vin.p,
-                                        (uint32_t*)s->mf_slot.p, (size_t)n, 0u, bits, c->stream));
+    SUCEM_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, keys.p, keys_out.p, vin.p, slot.p,
+                                        (size_t)n, 0u, bits, c->stream));
     s->mf_dptr.alloc(s->ndofs + 1);
     SUCEM_LAUNCH(k_mf_heads, grid_for(n, TB), TB, 0, c->stream, n, s->ndofs, keys_out.p,
                  s->mf_dptr.p);
-    s->mf_posT.alloc(n);
-    SUCEM_LAUNCH(k_mf_inverse, grid_for(n, TB), TB, 0, c->stream, n, nc, nloc,
-                 (const uint32_t*)s->mf_slot.p, s->mf_posT.p);
+    s->mf_posT.alloc((size_t)nc * nloc);
+    s->mf_posF.alloc((size_t)nbf * nfd);
+    SUCEM_LAUNCH(k_mf_inverse, grid_for(n, TB), TB, 0, c->stream, n, nc, nloc, slot.p,
+                 s->mf_posT.p, s->mf_posF.p);
     c->sync();   // temporaries are released on return
-    s->mf_slot.release();
   }
   sys->mf_coef64.alloc((size_t)12 * nc);
   sys->mf_coef32.alloc((size_t)12 * nc);
@@ -294,7 +353,16 @@ void mf_prepare(Context* c, System* sys) {
                m->eps_r.p, m->inv_mu_r.p, sys->mf_coef64.p);
   SUCEM_LAUNCH((k_mf_coefs<float>), grid_for(nc, TB), TB, 0, c->stream, nc, m->xyz.p, m->cells.p,
                m->eps_r.p, m->inv_mu_r.p, sys->mf_coef32.p);
-  sys->mf_yl.alloc((size_t)nc * nloc * sizeof(double2));
+  sys->mf_face64.alloc((size_t)6 * nbf);
+  sys->mf_face32.alloc((size_t)6 * nbf);
+  if (nbf) {
+    DBuf<double> H((size_t)6 * nbf);
+    face_coefs(c, s, H.p);
+    SUCEM_LAUNCH(k_mf_soa6, grid_for(nbf, TB), TB, 0, c->stream, nbf, H.p, sys->mf_face64.p,
+                 sys->mf_face32.p);
+    c->sync();
+  }
+  sys->mf_yl.alloc((size_t)(nc * nloc + nbf * nfd));
   c->sync();
   sys->mf_ready = true;
 }
@@ -304,13 +372,13 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
               const void* w, int epi, const RedSite& rs, const SpmvCfg& f) {
   using C = typename C2<R>::T;
   const Space* s = sys->space;
-  const int64_t nc = s->mesh->nc;
+  const int64_t nc = s->mesh->nc, nbf = s->nbf;
   const R* coef;
-  const R* s0v;
+  const R* hf;
   if constexpr (sizeof(R) == 4) {
-    coef = sys->mf_coef32.p; s0v = sys->s0_32.p;
+    coef = sys->mf_coef32.p; hf = sys->mf_face32.p;
   } else {
-    coef = sys->mf_coef64.p; s0v = sys->s0_64.p;
+    coef = sys->mf_coef64.p; hf = sys->mf_face64.p;
   }
   C* yl = (C*)sys->mf_yl.p;
   const int Gc = (int)((nc + 8 * TB - 1) / (8 * TB) * 8);   // TB cells per block, multiple of 8
@@ -320,11 +388,20 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
   else
     SUCEM_LAUNCH((k_mf_cells<R, 1>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p,
                  s->mf_posT.p, (R)k.s, (R)k.t, (const C*)x, yl);
-  const bool has_s0 = (sys->which & SUCEM_MAT_S0) && s->nnz0 > 0;
-  const int32_t* s0p = has_s0 ? s->s0_ptr.p : nullptr;
+  if (nbf) {
+    // the ABC term only where S0 was assembled; other systems keep zero face products
+    const R c0 = (sys->which & SUCEM_MAT_S0) ? (R)k.s0 : R(0);
+    if (s->nfd == 8)
+      SUCEM_LAUNCH((k_mf_faces<R, 8>), grid_for(nbf, TB), TB, 0, c->stream, nbf, s->nloc, hf,
+                   s->bf_cell.p, s->bf_local.p, s->cell_dofs.p, s->ref_face_dofs.p,
+                   s->ref_face.p, c0, s->mf_posF.p, (const C*)x, yl);
+    else
+      SUCEM_LAUNCH((k_mf_faces<R, 3>), grid_for(nbf, TB), TB, 0, c->stream, nbf, s->nloc, hf,
+                   s->bf_cell.p, s->bf_local.p, s->cell_dofs.p, s->ref_face_dofs.p,
+                   s->ref_face.p, c0, s->mf_posF.p, (const C*)x, yl);
+  }
   const int n = (int)s->ndofs;
-#define MF_ARGS n, f.G, s->mf_dptr.p, (const C*)yl, s0p, s->s0_col.p, s0v, \
-                (R)k.s0, mask, (const C*)x, (C*)y, (const C*)w, rs
+#define MF_ARGS n, f.G, s->mf_dptr.p, (const C*)yl, mask, (C*)y, (const C*)w, rs
   if (f.L == 4) {
     if (epi == 0) SUCEM_LAUNCH((k_mf_gather<R, 0, 4>), f.G, TB, 0, c->stream, MF_ARGS);
     else if (epi == 1) SUCEM_LAUNCH((k_mf_gather<R, 1, 4>), f.G, TB, 0, c->stream, MF_ARGS);

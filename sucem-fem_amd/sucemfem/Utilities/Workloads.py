@@ -147,13 +147,14 @@ class DeviceProblem(object):
             return self.spmv_bytes(precision)
         s = 4 if precision == 32 else 8            # real value
         sx = 2 * s                                  # complex vector element
-        n, nnz0 = self.ndofs, self.nnz0
+        n = self.ndofs
         nc = int(np.prod(self.n)) * 6
-        nloc = 6 if self.order == 1 else 20
+        nbf = int(self.space.info()[4])
+        nloc, nfd = (6, 3) if self.order == 1 else (20, 8)
         cells = nc * (12 * s + nloc * 4 + nloc * 4)   # coefficients, DOF ids, product positions
-        products = 2 * nc * nloc * sx                 # element products written, then read
-        gather = (n + 1) * 4 + nnz0 * (4 + s) + (n + 1) * 4
-        return cells + products + gather + 2 * n * sx   # x read once, y written once
+        faces = nbf * (6 * s + 8 + 2 * nfd * 4)       # coefficients, cell + local face, DOFs, positions
+        products = 2 * (nc * nloc + nbf * nfd) * sx   # products written, then read
+        return cells + faces + products + (n + 1) * 4 + 2 * n * sx   # segments; x once, y once
 
     def spmv_bytes(self, precision):
         """Algorithmic HBM bytes of one assembled fused SpMV (one vector)."""
