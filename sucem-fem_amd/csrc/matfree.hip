@@ -233,9 +233,51 @@ __device__ __forceinline__ void element_product(const R* a, const typename C2<R>
   (pair_step<R, ORDER, P>(a, xl, acc, std::make_integer_sequence<int, 12>{}), ...);
 }
 
+// Half of the rows at a time (HALF 0: rows i < N/2, HALF 1: rows >= N/2): only that
+// half's accumulators are live, which lowers the register count (more waves per SIMD)
+// at the price of forming the K entries that couple the halves twice.
+template <typename R, int ORDER, int HALF, int P, int... K>
+__device__ __forceinline__ void pair_step_half(const R* a, const typename C2<R>::T* xl,
+                                               typename C2<R>::T* acc,
+                                               std::integer_sequence<int, K...> ks) {
+  constexpr int N = MfTab<ORDER>::NLOC, H = N / 2;
+  constexpr int i = pair_i<N>(P), j = pair_j<N>(P);
+  constexpr bool mine = HALF == 0 ? (i < H) : (j >= H);
+  if constexpr (mine && pair_nonzero<ORDER, P>(ks)) {
+    R kij = R(0);
+    (k_term<R, ORDER, P, K>(kij, a), ...);
+    if constexpr (HALF == 0) {
+      acc[i].x += kij * xl[j].x;
+      acc[i].y += kij * xl[j].y;
+      if constexpr (j < H && j != i) {
+        acc[j].x += kij * xl[i].x;
+        acc[j].y += kij * xl[i].y;
+      }
+    } else {
+      acc[j - H].x += kij * xl[i].x;
+      acc[j - H].y += kij * xl[i].y;
+      if constexpr (i >= H && i != j) {
+        acc[i - H].x += kij * xl[j].x;
+        acc[i - H].y += kij * xl[j].y;
+      }
+    }
+  }
+}
+
+template <typename R, int ORDER, int HALF, int... P>
+__device__ __forceinline__ void element_product_half(const R* a, const typename C2<R>::T* xl,
+                                                     typename C2<R>::T* acc,
+                                                     std::integer_sequence<int, P...>) {
+  (pair_step_half<R, ORDER, HALF, P>(a, xl, acc, std::make_integer_sequence<int, 12>{}), ...);
+}
+
 // Element products, one thread per cell; consecutive blocks of cells go to one XCD.
-template <typename R, int ORDER>
-__global__ __launch_bounds__(TB) void k_mf_cells(int64_t nc, int G, const R* __restrict__ coef,
+// With the half-row split the registers are capped so that more waves share a SIMD
+// (3 FP64, 5 FP32) and the loads of one wave overlap the FMAs of the others; the
+// autotuner compares it with the uncapped whole-element form.
+template <typename R, int ORDER, bool SPLIT>
+__global__ __launch_bounds__(TB, SPLIT ? (sizeof(R) == 8 ? 3 : 5) : 1) void k_mf_cells(
+    int64_t nc, int G, const R* __restrict__ coef,
                                                  const int32_t* __restrict__ cdT,
                                                  const int32_t* __restrict__ posT, R cs, R ct,
                                                  const typename C2<R>::T* __restrict__ x,
@@ -248,17 +290,39 @@ __global__ __launch_bounds__(TB) void k_mf_cells(int64_t nc, int G, const R* __r
     R a[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) a[k] = coef[k * nc + c] * (k < 6 ? ct : cs) * R(1.0 / MfTab<ORDER>::den);
-    C xl[NLOC], acc[NLOC];
+    C xl[NLOC];
 #pragma unroll
-    for (int i = 0; i < NLOC; ++i) {
-      xl[i] = x[cdT[(int64_t)i * nc + c]];
-      acc[i] = mk(R(0), R(0));
-    }
-    element_product<R, ORDER>(a, xl, acc, std::make_integer_sequence<int, MfTab<ORDER>::NP>{});
+    for (int i = 0; i < NLOC; ++i) xl[i] = x[cdT[(int64_t)i * nc + c]];
     // each product goes to its place in the DOF-sorted order: the contributions of
     // neighbouring cells to one DOF land side by side, and the gather streams them
+    if constexpr (!SPLIT) {
+      C acc[NLOC];
 #pragma unroll
-    for (int i = 0; i < NLOC; ++i) yl[posT[(int64_t)i * nc + c]] = acc[i];
+      for (int i = 0; i < NLOC; ++i) acc[i] = mk(R(0), R(0));
+      element_product<R, ORDER>(a, xl, acc, std::make_integer_sequence<int, MfTab<ORDER>::NP>{});
+#pragma unroll
+      for (int i = 0; i < NLOC; ++i) yl[posT[(int64_t)i * nc + c]] = acc[i];
+    } else {
+      constexpr int H = NLOC / 2;
+      C acc[H];
+#pragma unroll
+      for (int i = 0; i < H; ++i) acc[i] = mk(R(0), R(0));
+      element_product_half<R, ORDER, 0>(a, xl, acc,
+                                        std::make_integer_sequence<int, MfTab<ORDER>::NP>{});
+#pragma unroll
+      for (int i = 0; i < H; ++i) {
+        yl[posT[(int64_t)i * nc + c]] = acc[i];
+        acc[i] = mk(R(0), R(0));
+      }
+      // opaque to the optimiser: the coupling entries are formed again rather than kept
+      // live across the halves (which would cost more registers than the split saves)
+#pragma unroll
+      for (int k = 0; k < 12; ++k) asm volatile("" : "+v"(a[k]));
+      element_product_half<R, ORDER, 1>(a, xl, acc,
+                                        std::make_integer_sequence<int, MfTab<ORDER>::NP>{});
+#pragma unroll
+      for (int i = 0; i < H; ++i) yl[posT[(int64_t)(H + i) * nc + c]] = acc[i];
+    }
   }
 }
 
@@ -382,11 +446,14 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
   }
   C* yl = (C*)sys->mf_yl.p;
   const int Gc = (int)((nc + 8 * TB - 1) / (8 * TB) * 8);   // TB cells per block, multiple of 8
-  if (s->order == 2)
-    SUCEM_LAUNCH((k_mf_cells<R, 2>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p,
+  if (s->order == 2 && f.U == 2)
+    SUCEM_LAUNCH((k_mf_cells<R, 2, true>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p,
+                 s->mf_posT.p, (R)k.s, (R)k.t, (const C*)x, yl);
+  else if (s->order == 2)
+    SUCEM_LAUNCH((k_mf_cells<R, 2, false>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p,
                  s->mf_posT.p, (R)k.s, (R)k.t, (const C*)x, yl);
   else
-    SUCEM_LAUNCH((k_mf_cells<R, 1>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p,
+    SUCEM_LAUNCH((k_mf_cells<R, 1, false>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p,
                  s->mf_posT.p, (R)k.s, (R)k.t, (const C*)x, yl);
   if (nbf) {
     // the ABC term only where S0 was assembled; other systems keep zero face products

@@ -478,10 +478,11 @@ SpmvCfg tune_sys(Context* c, System* sys, int reps) {
   // matrix-free candidates: element products + row gather
   if (s->nloc == 6 || s->nloc == 20) {
     mf_prepare(c, sys);
+    for (int split : {1, 2})
     for (int lanes : {1, 4})
     for (int pc : per_cu) {
       SpmvCfg f;
-      f.mf = 1; f.L = lanes; f.G = op_grid(c, s->ndofs, pc);
+      f.mf = 1; f.L = lanes; f.U = split; f.G = op_grid(c, s->ndofs, pc);
       sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);  // warm
       SUCEM_HIP(hipEventRecord(e0, c->stream));
       for (int i = 0; i < reps; ++i) sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);

@@ -128,6 +128,7 @@ class DeviceProblem(object):
                     blocks=int(cfg[3])), ms.value
 
     def set_spmv(self, precision, matrix_free=0, lanes=16, loads=4, blocks=2048):
+        # matrix-free: lanes per gather row (1, 4); loads 1 = whole element, 2 = half rows
         """Select the operator explicitly (see ``tune``)."""
         cfg = np.array([matrix_free, lanes, loads, blocks], np.int32)
         _hip.call("sucem_system_set_spmv", self.ctx.handle, self.system.handle.h, int(precision),

@@ -384,7 +384,7 @@ def test_matrix_free_operator_matches_assembled(dev, order):
         for pec in (0, 1):
             ys = []
             for mf in (0, 1):
-                cfg = np.array([mf, 8 if mf == 0 else 4, 4, 64], np.int32)
+                cfg = np.array([mf, 8 if mf == 0 else 4, 4 if mf == 0 else 1 + pec, 64], np.int32)
                 _hip.call("sucem_system_set_spmv", V.ctx.handle, sysm.handle.h, prec, _hip.ptr(cfg))
                 yd = torch.zeros_like(xd)
                 _hip.call("sucem_system_apply", V.ctx.handle, sysm.handle.h, k0, prec, pec, 2,
@@ -409,7 +409,7 @@ def test_matrix_free_solve_matches_direct(dev, precision):
     its = []
     for mf in (0, 1):
         for p in (precision, 64):
-            prob.set_spmv(p, matrix_free=mf, lanes=4, loads=4, blocks=64)
+            prob.set_spmv(p, matrix_free=mf, lanes=4, loads=2 if mf else 4, blocks=64)
         x = torch.zeros_like(b)
         st = prob.solve(k0, b, x, precision=precision, tol=1e-9, max_iter=20000)
         assert st["status"] == 0 and st["rel_residual"] <= 1e-9, st
