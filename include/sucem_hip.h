@@ -170,7 +170,7 @@ int sucem_system_apply_coef(sucem_ctx* ctx, sucem_system* sys, double cs, double
 int sucem_system_tune(sucem_ctx* ctx, sucem_system* sys, int precision, int reps, int* cfg4_h,
                       double* ms_h);
 /* Set the operator explicitly: cfg4_h = {matrix_free (0/1), lanes (4, 8, 16, 32; matrix-free:
- * lanes per row of its gather, 1 or 4), loads (1, 2, 4; matrix-free: 1 = element rows in
+ * lanes per row of its gather, 1 or 4, or 0 for one row per thread staged through LDS), loads (1, 2, 4; matrix-free: 1 = element rows in
  * one pass, 2 = half the rows at a time), blocks (multiple of 8)}.
  * Calls on one system must not overlap (the matrix-free element buffer is per system). */
 int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, const int* cfg4_h);

@@ -416,7 +416,8 @@ int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, cons
   if (f.mf == 1) {
     SUCEM_CHECK(sys->s.space->nloc == 6 || sys->s.space->nloc == 20, SUCEM_ERR_UNSUPPORTED,
                 "matrix-free operator: orders 1 and 2 only");
-    SUCEM_CHECK(f.L == 1 || f.L == 4, SUCEM_ERR_INVALID, "matrix-free lanes per row must be 1 or 4");
+    SUCEM_CHECK(f.L == 0 || f.L == 1 || f.L == 4, SUCEM_ERR_INVALID,
+                "matrix-free lanes per row must be 0 (LDS-staged rows), 1 or 4");
     SUCEM_CHECK(f.U == 1 || f.U == 2, SUCEM_ERR_INVALID,
                 "matrix-free loads must be 1 (whole element) or 2 (half the rows at a time)");
     mf_prepare(c, &sys->s);

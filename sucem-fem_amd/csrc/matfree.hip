@@ -376,6 +376,57 @@ __global__ __launch_bounds__(TB) void k_mf_gather(
   if (EPI) reduce_finalize<TB>(rs, G, d1, d2);
 }
 
+// The same row sums with the block's products staged through LDS: a block's rows own
+// one contiguous stretch of the product buffer, which is read fully coalesced in
+// pieces of CAP products; every thread then sums its own segment from LDS in index
+// order (the same order as the one-lane form, hence the same result).
+template <typename R, int EPI>
+__global__ __launch_bounds__(TB) void k_mf_gather_lds(
+    int n, int G, const int32_t* __restrict__ dptr, const typename C2<R>::T* __restrict__ yl,
+    const uint8_t* __restrict__ mask, typename C2<R>::T* __restrict__ y,
+    const typename C2<R>::T* __restrict__ w, RedSite rs) {
+  using C = typename C2<R>::T;
+  constexpr int CAP = 4 * TB;
+  __shared__ C buf[CAP];
+  const int lb = xcd_block(G);
+  int chunk = (n + G - 1) / G;
+  chunk = (chunk + TB - 1) / TB * TB;
+  const int start = lb * chunk;
+  const int end = min(n, start + chunk);
+  double2 d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0);
+  for (int base = start; base < end; base += TB) {
+    const int r = base + threadIdx.x;
+    const int s0 = r < end ? dptr[r] : 0, s1 = r < end ? dptr[r + 1] : 0;
+    const int p0 = dptr[base], p1 = dptr[min(base + TB, end)];
+    C acc = mk(R(0), R(0));
+    for (int q0 = p0; q0 < p1; q0 += CAP) {
+      const int q1 = min(p1, q0 + CAP);
+      __syncthreads();
+      for (int q = q0 + threadIdx.x; q < q1; q += TB) buf[q - q0] = yl[q];
+      __syncthreads();
+      const int a = max(s0, q0), b = min(s1, q1);
+      for (int q = a; q < b; ++q) {
+        const C v = buf[q - q0];
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+    }
+    if (r < end) {
+      if (mask && mask[r]) acc = mk(R(0), R(0));
+      y[r] = acc;
+      if (EPI) {
+        const double2 yd = to_d(acc);
+        double2 wd = to_d(w[r]);
+        if (EPI == 2) wd.y = -wd.y;
+        d1.x += wd.x * yd.x - wd.y * yd.y;
+        d1.y += wd.x * yd.y + wd.y * yd.x;
+        d2.x += yd.x * yd.x + yd.y * yd.y;
+      }
+    }
+  }
+  if (EPI) reduce_finalize<TB>(rs, G, d1, d2);
+}
+
 }  // namespace
 
 void mf_prepare(Context* c, System* sys) {
@@ -469,7 +520,11 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
   }
   const int n = (int)s->ndofs;
 #define MF_ARGS n, f.G, s->mf_dptr.p, (const C*)yl, mask, (C*)y, (const C*)w, rs
-  if (f.L == 4) {
+  if (f.L == 0) {
+    if (epi == 0) SUCEM_LAUNCH((k_mf_gather_lds<R, 0>), f.G, TB, 0, c->stream, MF_ARGS);
+    else if (epi == 1) SUCEM_LAUNCH((k_mf_gather_lds<R, 1>), f.G, TB, 0, c->stream, MF_ARGS);
+    else SUCEM_LAUNCH((k_mf_gather_lds<R, 2>), f.G, TB, 0, c->stream, MF_ARGS);
+  } else if (f.L == 4) {
     if (epi == 0) SUCEM_LAUNCH((k_mf_gather<R, 0, 4>), f.G, TB, 0, c->stream, MF_ARGS);
     else if (epi == 1) SUCEM_LAUNCH((k_mf_gather<R, 1, 4>), f.G, TB, 0, c->stream, MF_ARGS);
     else SUCEM_LAUNCH((k_mf_gather<R, 2, 4>), f.G, TB, 0, c->stream, MF_ARGS);

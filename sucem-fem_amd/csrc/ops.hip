@@ -479,7 +479,7 @@ SpmvCfg tune_sys(Context* c, System* sys, int reps) {
   if (s->nloc == 6 || s->nloc == 20) {
     mf_prepare(c, sys);
     for (int split : {1, 2})
-    for (int lanes : {1, 4})
+    for (int lanes : {0, 1, 4})
     for (int pc : per_cu) {
       SpmvCfg f;
       f.mf = 1; f.L = lanes; f.U = split; f.G = op_grid(c, s->ndofs, pc);

@@ -379,19 +379,24 @@ def test_matrix_free_operator_matches_assembled(dev, order):
     k0 = 2 * np.pi * 1e9 / O.c0
     n = V.dim()
     x = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+    # matrix-free forms: gather lanes 0 (LDS-staged), 1, 4; element rows whole or halved
+    forms = [(8, 4)] + [(lanes, split) for lanes in (0, 1, 4) for split in (1, 2)]
     for prec, dt, tol in ((64, torch.complex128, 1e-12), (32, torch.complex64, 2e-5)):
         xd = torch.as_tensor(x, dtype=dt, device="cuda")
         for pec in (0, 1):
             ys = []
-            for mf in (0, 1):
-                cfg = np.array([mf, 8 if mf == 0 else 4, 4 if mf == 0 else 1 + pec, 64], np.int32)
+            for i, (lanes, loads) in enumerate(forms):
+                cfg = np.array([0 if i == 0 else 1, lanes, loads, 64], np.int32)
                 _hip.call("sucem_system_set_spmv", V.ctx.handle, sysm.handle.h, prec, _hip.ptr(cfg))
                 yd = torch.zeros_like(xd)
                 _hip.call("sucem_system_apply", V.ctx.handle, sysm.handle.h, k0, prec, pec, 2,
                           _hip.ptr(xd), _hip.ptr(yd))
                 ys.append(yd.cpu().numpy().astype(np.complex128))
             scale = np.abs(ys[0]).max()
-            assert np.abs(ys[1] - ys[0]).max() <= tol * scale, (prec, pec)
+            for i in range(1, len(forms)):
+                assert np.abs(ys[i] - ys[0]).max() <= tol * scale, (prec, pec, forms[i])
+            # one-lane and LDS-staged gathers sum in the same order: identical results
+            assert np.array_equal(ys[1], ys[3]) and np.array_equal(ys[2], ys[4])
 
 
 @pytest.mark.parametrize("precision", [64, 32])
