@@ -6,9 +6,10 @@ first-order ABC on all 19,200 boundary triangles, z-directed dipole (moment
 lambda/1000) at the cube centre, f = 1 GHz.  One step = one driven solve of
 A x = b to relative residual 1e-8 (FP64 ||b - A x|| / ||b||): COCG with the
 gradient-space preconditioner, operator autotuned (matrix-free or assembled CSR),
-mesh data, matrices and b resident in HBM.  The headline path is plain FP64
-(--precision 64, the faster one on this part); the other path (FP32 operator and
-vectors with FP64 reliable updates) is timed once beside it for the comparison.
+mesh data, matrices and b resident in HBM.  The headline path is FP32+IR
+(--precision 32: FP32 operator, preconditioner and vectors, FP64 dot products, FP64
+accumulation of x and FP64 reliable updates of the residual); plain FP64 is timed once
+beside it for the comparison.
 
 Prints one JSON line.  ``--gpus`` above 1 is refused: this build runs on one GPU.
 """
@@ -81,7 +82,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=40, help="boxes per axis")
     ap.add_argument("--order", type=int, default=2)
-    ap.add_argument("--precision", type=int, default=64, help="headline path: 64 or 32 (+IR)")
+    ap.add_argument("--precision", type=int, default=32, help="headline path: 32 (+IR) or 64")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=60000)
     ap.add_argument("--no-compare", action="store_true",

@@ -115,6 +115,8 @@ __global__ void k_mf_soa6(int64_t nbf, const double* H, double* h64, float* h32)
 }
 
 // ABC face products: v = i c0 S0_f x_f written to the face slots (zero when c0 == 0).
+// One thread per (face, face-DOF row): 8 threads share a face's coefficients and x
+// values through the cache; the reference face tensors are read from LDS.
 template <typename R, int NFD>
 __global__ __launch_bounds__(TB) void k_mf_faces(int64_t nbf, int nloc, const R* __restrict__ h,
                                                  const int32_t* __restrict__ bf_cell,
@@ -127,30 +129,31 @@ __global__ __launch_bounds__(TB) void k_mf_faces(int64_t nbf, int nloc, const R*
                                                  typename C2<R>::T* __restrict__ yl) {
   using C = typename C2<R>::T;
   constexpr int KK = NFD * NFD;
-  for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nbf;
-       f += (int64_t)gridDim.x * blockDim.x) {
+  __shared__ R rf_s[4 * 6 * KK];
+  for (int t = threadIdx.x; t < 4 * 6 * KK; t += TB) rf_s[t] = (R)ref_face[t];
+  __syncthreads();
+  const int64_t total = nbf * NFD;
+  for (int64_t id = blockIdx.x * (int64_t)TB + threadIdx.x; id < total;
+       id += (int64_t)gridDim.x * TB) {
+    const int64_t f = id / NFD;
+    const int a = (int)(id % NFD);
     const int lf = bf_local[f];
     const int64_t c = bf_cell[f];
     R hk[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) hk[k] = h[k * nbf + f];
-    C xf[NFD];
+    const R* rf = rf_s + lf * 6 * KK + a * NFD;
+    C v = mk(R(0), R(0));
 #pragma unroll
-    for (int a = 0; a < NFD; ++a) xf[a] = x[cd[c * nloc + face_dofs[lf * NFD + a]]];
-    const double* rf = ref_face + (int64_t)lf * 6 * KK;
+    for (int b = 0; b < NFD; ++b) {
+      R m = R(0);
 #pragma unroll
-    for (int a = 0; a < NFD; ++a) {
-      C v = mk(R(0), R(0));
-#pragma unroll
-      for (int b = 0; b < NFD; ++b) {
-        R m = R(0);
-#pragma unroll
-        for (int k = 0; k < 6; ++k) m += hk[k] * (R)rf[k * KK + a * NFD + b];
-        v.x += m * xf[b].x;
-        v.y += m * xf[b].y;
-      }
-      yl[posF[f * NFD + a]] = mk(-c0 * v.y, c0 * v.x);
+      for (int k = 0; k < 6; ++k) m += hk[k] * rf[k * KK + b];
+      const C xb = x[cd[c * nloc + face_dofs[lf * NFD + b]]];
+      v.x += m * xb.x;
+      v.y += m * xb.y;
     }
+    yl[posF[id]] = mk(-c0 * v.y, c0 * v.x);
   }
 }
 
@@ -510,11 +513,11 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
     // the ABC term only where S0 was assembled; other systems keep zero face products
     const R c0 = (sys->which & SUCEM_MAT_S0) ? (R)k.s0 : R(0);
     if (s->nfd == 8)
-      SUCEM_LAUNCH((k_mf_faces<R, 8>), grid_for(nbf, TB), TB, 0, c->stream, nbf, s->nloc, hf,
+      SUCEM_LAUNCH((k_mf_faces<R, 8>), grid_for(nbf * 8, TB), TB, 0, c->stream, nbf, s->nloc, hf,
                    s->bf_cell.p, s->bf_local.p, s->cell_dofs.p, s->ref_face_dofs.p,
                    s->ref_face.p, c0, s->mf_posF.p, (const C*)x, yl);
     else
-      SUCEM_LAUNCH((k_mf_faces<R, 3>), grid_for(nbf, TB), TB, 0, c->stream, nbf, s->nloc, hf,
+      SUCEM_LAUNCH((k_mf_faces<R, 3>), grid_for(nbf * 3, TB), TB, 0, c->stream, nbf, s->nloc, hf,
                    s->bf_cell.p, s->bf_local.p, s->cell_dofs.p, s->ref_face_dofs.p,
                    s->ref_face.p, c0, s->mf_posF.p, (const C*)x, yl);
   }
