@@ -227,6 +227,7 @@ __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __re
 }  // namespace
 
 void ensure_gradient(Context* c, Space* s) {
+  std::lock_guard<std::mutex> hold(s->lazy);
   if (s->g_ptr.p) return;
   const RefData& R = ref_data(s->order);
   const Mesh* m = s->mesh;

@@ -13,6 +13,8 @@
 //   system : st64 double2[nnz] = (S, T) interleaved, st32 float2[nnz] (same
 //            values rounded), s0_64 double[nnz0], s0_32 float[nnz0]
 #pragma once
+#include <mutex>
+
 #include "common.hpp"
 
 namespace sucem {
@@ -28,6 +30,7 @@ struct Mesh {
 
 struct Space {
   Context* ctx = nullptr;
+  std::mutex lazy;                // guards the data built on first use (gradient, matrix-free)
   Mesh* mesh = nullptr;
   int order = 1, nloc = 6, nfd = 3;
   int64_t nE = 0, nF = 0, nbf = 0, ndofs = 0, nnz = 0, nnz0 = 0, npec = 0;

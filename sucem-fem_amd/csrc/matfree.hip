@@ -438,6 +438,7 @@ void mf_prepare(Context* c, System* sys) {
   const Mesh* m = s->mesh;
   const int64_t nc = m->nc, nbf = s->nbf;
   const int nloc = s->nloc, nfd = s->nfd;
+  std::lock_guard<std::mutex> hold(s->lazy);
   if (!s->mf_cdT.p) {
     // DOF ids per cell (SoA); products sorted by (DOF, slot) and their positions
     s->mf_cdT.alloc((size_t)nc * nloc);
