@@ -94,6 +94,10 @@ typedef struct sucem_solve_stats {
   double elapsed_ms;
   int64_t spmv_lo;      /* SpMVs in the working precision */
   int64_t spmv_hi;      /* FP64 residual SpMVs */
+  double op_ms;         /* mean device time (ms) of one working-precision operator application
+                           (its kernels, with the fused dot-product epilogue), measured with
+                           HIP events on the solve's stream at one iteration per check interval */
+  int64_t op_samples;   /* number of such samples (0: none taken) */
 } sucem_solve_stats;
 
 const char* sucem_last_error(void);

@@ -589,6 +589,8 @@ static void convert_stats(const SolveStats& s, sucem_solve_stats* out) {
   out->elapsed_ms = s.elapsed_ms;
   out->spmv_lo = s.spmv_lo;
   out->spmv_hi = s.spmv_hi;
+  out->op_ms = s.op_ms;
+  out->op_samples = s.op_samples;
 }
 
 int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const double* b_d,

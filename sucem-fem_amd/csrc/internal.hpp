@@ -158,6 +158,8 @@ struct SolveStats {
   int status = 0, iterations = 0, reliable_updates = 0;
   double rel_residual = 0, elapsed_ms = 0;
   int64_t spmv_lo = 0, spmv_hi = 0;
+  double op_ms = 0;         // mean device time of a sampled working-precision operator
+  int64_t op_samples = 0;
 };
 void solve_system(Context* c, const System* sys, Coef k, const double2* b, double2* x,
                   const SolveOpts& o, SolveStats& st, double* hist_h, int64_t hist_cap);

@@ -334,6 +334,42 @@ double read_scalar(Context* c, const double* d) {
   return h[0];
 }
 
+// One pair of timing events, reused: start/stop bracket work on the stream, harvest reads
+// the pair once the stream has been synchronised past it.
+struct OpTimer {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool running = false, pending = false;
+  double total_ms = 0;
+  int64_t samples = 0;
+  void start(Context* c) {
+    if (pending || running) return;
+    if (!e0) {
+      SUCEM_HIP(hipEventCreate(&e0));
+      SUCEM_HIP(hipEventCreate(&e1));
+    }
+    SUCEM_HIP(hipEventRecord(e0, c->stream));
+    running = true;
+  }
+  void stop(Context* c) {
+    if (!running) return;
+    SUCEM_HIP(hipEventRecord(e1, c->stream));
+    running = false;
+    pending = true;
+  }
+  void harvest() {
+    if (!pending) return;
+    float ms = 0.f;
+    SUCEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    pending = false;
+    total_ms += ms;
+    ++samples;
+  }
+  ~OpTimer() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
 enum Site { SITE_NORM = 0, SITE_RESID, SITE_SPMV1, SITE_SPMV2, SITE_UPDATE, SITE_COUNT };
 
 template <typename R>
@@ -415,15 +451,27 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   int restart = 1;
   st.status = o.max_iter;
   double rel = 1.0;
+  // Operator timing: at the first iteration of every check interval (from the second
+  // interval on) the working-precision operator application (its kernels and its fused
+  // reduction epilogue, not the one-block finalisation) is bracketed by two events on the
+  // solve's stream; the pair is read back after the interval's host check has synchronised.
+  OpTimer timer;
+  auto apply_epi = [&](const void* xv, void* yv, const void* wv, int epi, const RedSite& rs) {
+    const bool sample = it >= o.check_every && it % o.check_every == 0;
+    if (sample) timer.start(c);
+    op.apply_lo(xv, yv, wv, epi, rs);
+    if (sample) timer.stop(c);
+    finish_reduction(c, rs, op.flo.G);
+  };
   for (it = 0; it < o.max_iter; ++it) {
     if (cocg && grad) {
-      op.apply_lo(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
+      apply_epi(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
       SUCEM_LAUNCH((k_cocg_xr_plain<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, p.p, q.p);
       precondition(site(SITE_UPDATE, FIN_COCG_BETA, it));
       SUCEM_LAUNCH((k_cocg_pz<C>), G, TB, 0, c->stream, n, G, scal.p, z.p, p.p);
       st.spmv_lo += 1;
     } else if (cocg) {
-      op.apply_lo(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
+      apply_epi(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
       SUCEM_LAUNCH((k_cocg_xr<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, p.p, q.p, dinv.p,
                    site(SITE_UPDATE, FIN_COCG_BETA, it));
       finish_reduction(c, site(SITE_UPDATE, FIN_COCG_BETA, it), G);
@@ -433,9 +481,9 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
       SUCEM_LAUNCH((k_bicg_p<C>), G, TB, 0, c->stream, n, G, restart, scal.p, r.p, p.p, v.p, dinv.p,
                    phat.p);
       restart = 0;
-      op.apply_lo(phat.p, v.p, rt.p, 2, site(SITE_SPMV1, FIN_BICG_ALPHA));
+      apply_epi(phat.p, v.p, rt.p, 2, site(SITE_SPMV1, FIN_BICG_ALPHA));
       SUCEM_LAUNCH((k_bicg_s<C>), G, TB, 0, c->stream, n, G, scal.p, r.p, v.p, dinv.p, s.p, shat.p);
-      op.apply_lo(shat.p, t.p, s.p, 2, site(SITE_SPMV2, FIN_BICG_OMEGA));
+      apply_epi(shat.p, t.p, s.p, 2, site(SITE_SPMV2, FIN_BICG_OMEGA));
       SUCEM_LAUNCH((k_bicg_x<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, s.p, t.p, phat.p,
                    shat.p, rt.p, site(SITE_UPDATE, FIN_BICG_BETA, it));
       finish_reduction(c, site(SITE_UPDATE, FIN_BICG_BETA, it), G);
@@ -444,6 +492,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     const bool check = ((it + 1) % o.check_every) == 0 || it + 1 == o.max_iter;
     if (!check) continue;
     rel = read_scalar(c, hist.p + it) / bnorm;
+    timer.harvest();
     if (!std::isfinite(rel)) {
       st.status = -1;   // breakdown
       break;
@@ -499,6 +548,9 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     for (int64_t i = 0; i < m; ++i) hist_h[i] /= bnorm;
   }
   c->sync();
+  timer.harvest();
+  st.op_samples = timer.samples;
+  st.op_ms = timer.samples ? timer.total_ms / (double)timer.samples : 0.0;
   st.elapsed_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -538,7 +538,6 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
     else SUCEM_LAUNCH((k_mf_gather<R, 2, 1>), f.G, TB, 0, c->stream, MF_ARGS);
   }
 #undef MF_ARGS
-  if (epi) finish_reduction(c, rs, f.G);
 }
 template void mf_apply<float>(Context*, const System*, Coef, const uint8_t*, const void*, void*,
                               const void*, int, const RedSite&, const SpmvCfg&);

@@ -326,7 +326,6 @@ void launch_sys(Context* c, const System* sys, Coef k, const uint8_t* mask, cons
   else if (epi == 1) SUCEM_LAUNCH((k_sys_spmv<R, L, U, 1>), G, TB, 0, c->stream, SYS_ARGS);
   else SUCEM_LAUNCH((k_sys_spmv<R, L, U, 2>), G, TB, 0, c->stream, SYS_ARGS);
 #undef SYS_ARGS
-  if (epi) finish_reduction(c, rs, G);
 }
 
 template <typename R, int L, int U>
@@ -341,7 +340,6 @@ void launch_csr(Context* c, const Csr* A, const uint8_t* mask, const void* x, vo
   else if (epi == 1) SUCEM_LAUNCH((k_csr_spmv<R, L, U, 1>), G, TB, 0, c->stream, CSR_ARGS);
   else SUCEM_LAUNCH((k_csr_spmv<R, L, U, 2>), G, TB, 0, c->stream, CSR_ARGS);
 #undef CSR_ARGS
-  if (epi) finish_reduction(c, rs, G);
 }
 
 template <typename R, int L>

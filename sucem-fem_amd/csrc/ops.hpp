@@ -10,7 +10,8 @@ int op_grid(Context* c, int64_t n, int per_cu);
 SpmvCfg default_cfg(Context* c, int64_t n, int64_t nnz);
 
 // y = op(x) in precision R (float: float2 vectors / float values).  epi = 1 (or 2):
-// the grid also reduces sum_r w_r y_r (conj(w_r) y_r) and sum |y_r|^2 into rs.
+// the grid also reduces sum_r w_r y_r (conj(w_r) y_r) and sum |y_r|^2 into the block
+// partials of rs; the caller finishes them with finish_reduction(c, rs, f.G).
 template <typename R>
 void sys_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
                void* y, const void* w, int epi, const RedSite& rs, const SpmvCfg& f);

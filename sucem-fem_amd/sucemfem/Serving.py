@@ -205,8 +205,13 @@ def summarize(results, makespan):
 # ---------------------------------------------------------------------------------------
 # device executor
 # ---------------------------------------------------------------------------------------
-def device_executor(tol=1e-8, max_iter=50000):
-    """Executor running each request on the device on the calling thread's own stream."""
+def device_executor(tol=1e-8, max_iter=50000, precision=64):
+    """Executor running each request on the device on the calling thread's own stream.
+    ``precision``: 64 (FP64 throughout) or 32 (FP32 operator and Krylov vectors with FP64
+    residual replacement) for the driven solves; the eigen solver's inner solves stay FP64
+    (the Lanczos recursion needs inner solves accurate to 1e-11)."""
+    if precision not in (32, 64):
+        raise ValueError("precision must be 32 or 64")
     import torch
     from .Utilities.Workloads import DeviceProblem
     from . import _hip
@@ -234,7 +239,7 @@ def device_executor(tol=1e-8, max_iter=50000):
             centre = np.asarray(req.dims) / 2 + 1e-9
             b = prob.rhs_dipoles(centre[None, :], np.array([[0, 0, c0 / req.freq / 1000]]), k0)
             x = torch.zeros_like(b)
-            st = prob.solve(k0, b, x, precision=64, tol=tol, max_iter=max_iter)
+            st = prob.solve(k0, b, x, precision=precision, tol=tol, max_iter=max_iter)
             stream.synchronize()
             return x.cpu().numpy(), dict(st, ndofs=prob.ndofs)
 

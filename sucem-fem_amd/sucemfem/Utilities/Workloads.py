@@ -102,11 +102,13 @@ class DeviceProblem(object):
                   _hip.ptr(k0), _hip.ptr(B), _hip.ptr(X), ctypes.byref(o), st)
         return [s.as_dict() for s in st]
 
-    def eigs(self, sigma, nev, ncv=0, tol=1e-10, inner_tol=1e-11, vectors=False):
+    def eigs(self, sigma, nev, ncv=0, tol=1e-10, inner_tol=1e-11, vectors=False,
+             inner_precision=64):
         """Lowest nev k^2 > sigma of the PEC cavity: (k2, relative residuals, stats[, modes])."""
         o = _hip.EigOpts()
         _hip.load().sucem_eig_opts_default(ctypes.byref(o))
         o.ncv, o.tol, o.inner_tol = int(ncv), tol, inner_tol
+        o.inner_precision = int(inner_precision)
         k2 = np.zeros(nev)
         resid = np.zeros(nev)
         vecs = self.ctx.tensor(nev * self.ndofs) if vectors else None
@@ -142,8 +144,12 @@ class DeviceProblem(object):
         _hip.call("sucem_system_apply", self.ctx.handle, self.system.handle.h, float(k0), precision,
                   0, nvec, _hip.ptr(x), _hip.ptr(y))
 
-    def operator_bytes(self, precision, matrix_free):
-        """Algorithmic HBM bytes of one application of the fused operator (one vector)."""
+    def operator_bytes(self, precision, matrix_free, epilogue=False):
+        """Algorithmic HBM bytes of one application of the fused operator (one vector);
+        ``epilogue``: plus the read of w for the fused dot product sum_r w_r y_r."""
+        if epilogue:
+            return (self.operator_bytes(precision, matrix_free)
+                    + self.ndofs * (8 if precision == 32 else 16))
         if not matrix_free:
             return self.spmv_bytes(precision)
         s = 4 if precision == 32 else 8            # real value

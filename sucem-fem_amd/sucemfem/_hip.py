@@ -29,7 +29,8 @@ class SolveStats(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int),
                 ("reliable_updates", ctypes.c_int), ("pad_", ctypes.c_int),
                 ("rel_residual", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
-                ("spmv_lo", ctypes.c_int64), ("spmv_hi", ctypes.c_int64)]
+                ("spmv_lo", ctypes.c_int64), ("spmv_hi", ctypes.c_int64),
+                ("op_ms", ctypes.c_double), ("op_samples", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
