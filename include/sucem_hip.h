@@ -29,7 +29,8 @@ enum {
   SUCEM_ERR_OOM = 3,           /* device allocation failed */
   SUCEM_ERR_OUTSIDE = 4,       /* point outside the mesh (point_source.py:79 AssertionError) */
   SUCEM_ERR_MATERIAL = 5,      /* unknown material region (Materials.py:147-148 ValueError) */
-  SUCEM_ERR_UNSUPPORTED = 6    /* order / size outside what the library supports */
+  SUCEM_ERR_UNSUPPORTED = 6,   /* order / size outside what the library supports */
+  SUCEM_ERR_INTERNAL = 7       /* a consistency check of the library's own data failed */
 };
 
 /* matrix selectors */
@@ -166,16 +167,19 @@ int sucem_system_apply_coef(sucem_ctx* ctx, sucem_system* sys, double cs, double
                             int precision, int eliminate_pec, int64_t nvec, const void* x_d,
                             void* y_d);
 /* Autotune the fused operator for precision 32 or 64 by timing every candidate with HIP
- * events: the assembled CSR operator over (lanes per row, loads per lane, blocks) and the
- * matrix-free operator (element products + row gather) over its gather grid.  The fastest
- * is kept on the system and used by apply and solve.  cfg4_h (nullable) receives
- * {matrix_free, lanes, loads, blocks}, ms_h the time per application.
- * (No reference counterpart: launch tuning of the device operator.) */
+ * events: the assembled CSR operator over (lanes per row, loads per lane, blocks), the
+ * matrix-free operator (element products + row gather) over its gather grid and the patch
+ * operator (element products summed per patch of cells in LDS + shared-DOF sums) over the
+ * grid of its shared-DOF kernel.  The fastest is kept on the system and used by apply and
+ * solve.  cfg4_h (nullable) receives {matrix_free, lanes, loads, blocks}, ms_h the time
+ * per application.  (No reference counterpart: launch tuning of the device operator.) */
 int sucem_system_tune(sucem_ctx* ctx, sucem_system* sys, int precision, int reps, int* cfg4_h,
                       double* ms_h);
-/* Set the operator explicitly: cfg4_h = {matrix_free (0/1), lanes (4, 8, 16, 32; matrix-free:
- * lanes per row of its gather, 1 or 4, or 0 for one row per thread staged through LDS), loads (1, 2, 4; matrix-free: 1 = element rows in
- * one pass, 2 = half the rows at a time), blocks (multiple of 8)}.
+/* Set the operator explicitly: cfg4_h = {matrix_free (0 assembled, 1 matrix-free per cell,
+ * 2 matrix-free per patch), lanes (4, 8, 16, 32; matrix-free: lanes per row of its gather,
+ * 1 or 4, or 0 for one row per thread staged through LDS; patch: ignored), loads (1, 2, 4;
+ * matrix-free: 1 = element rows in one pass, 2 = half the rows at a time; patch: ignored),
+ * blocks (multiple of 8; patch: grid of the shared-DOF kernel)}.
  * Calls on one system must not overlap (the matrix-free element buffer is per system). */
 int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, const int* cfg4_h);
 /* Materialised A values (complex128 interleaved) on the full pattern. */

@@ -423,7 +423,14 @@ int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, cons
               "blocks must be a positive multiple of 8");
   Context* c = &ctx->c;
   c->use();
-  if (f.mf == 1) {
+  if (f.mf == 2) {
+    SUCEM_CHECK(pt_prepare(c, &sys->s), SUCEM_ERR_UNSUPPORTED,
+                "patch operator: orders 1 and 2, patches of at most 65535 products");
+    SUCEM_CHECK(precision == 32 ? pt_fits<float>(sys->s.space) : pt_fits<double>(sys->s.space),
+                SUCEM_ERR_UNSUPPORTED,
+                "the patch operator needs more LDS per workgroup than the device has");
+    f = pt_cfg(sys->s.space, f.G);
+  } else if (f.mf == 1) {
     SUCEM_CHECK(sys->s.space->nloc == 6 || sys->s.space->nloc == 20, SUCEM_ERR_UNSUPPORTED,
                 "matrix-free operator: orders 1 and 2 only");
     SUCEM_CHECK(f.L == 0 || f.L == 1 || f.L == 4, SUCEM_ERR_INVALID,
@@ -432,7 +439,7 @@ int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, cons
                 "matrix-free loads must be 1 (whole element) or 2 (half the rows at a time)");
     mf_prepare(c, &sys->s);
   } else {
-    SUCEM_CHECK(f.mf == 0, SUCEM_ERR_INVALID, "matrix_free must be 0 or 1");
+    SUCEM_CHECK(f.mf == 0, SUCEM_ERR_INVALID, "matrix_free must be 0, 1 or 2");
     SUCEM_CHECK(f.L == 4 || f.L == 8 || f.L == 16 || f.L == 32, SUCEM_ERR_INVALID,
                 "lanes must be 4, 8, 16 or 32");
     SUCEM_CHECK(f.U == 1 || f.U == 2 || f.U == 4, SUCEM_ERR_INVALID, "loads must be 1, 2 or 4");

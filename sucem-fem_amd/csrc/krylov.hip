@@ -379,7 +379,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   Context* c = op.c;
   const int n = (int)op.n;
   const int G = op_grid(c, n, 8);            // vector kernels
-  const int Gm = std::max(G, op.flo.G);      // partial buffer size
+  const int Gm = std::max(G, op.flo.parts());  // partial buffer size
   const bool lo = sizeof(R) == 4;
   const bool cocg = o.method == SUCEM_METHOD_COCG;
   const bool grad = o.precond == SUCEM_PRECOND_GRADIENT;
@@ -461,7 +461,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     if (sample) timer.start(c);
     op.apply_lo(xv, yv, wv, epi, rs);
     if (sample) timer.stop(c);
-    finish_reduction(c, rs, op.flo.G);
+    finish_reduction(c, rs, op.flo.parts());
   };
   for (it = 0; it < o.max_iter; ++it) {
     if (cocg && grad) {

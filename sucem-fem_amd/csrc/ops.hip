@@ -391,6 +391,10 @@ SpmvCfg default_cfg(Context* c, int64_t n, int64_t nnz) {
 template <typename R>
 void sys_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
                void* y, const void* w, int epi, const RedSite& rs, const SpmvCfg& f) {
+  if (f.mf == 2) {
+    pt_apply<R>(c, sys, k, mask, x, y, w, epi, rs, f);
+    return;
+  }
   if (f.mf) {
     SUCEM_CHECK(sys->mf_ready, SUCEM_ERR_INVALID, "matrix-free operator not prepared");
     mf_apply<R>(c, sys, k, mask, x, y, w, epi, rs, f);
@@ -491,6 +495,24 @@ SpmvCfg tune_sys(Context* c, System* sys, int reps) {
       f.ms = ms / reps;
       if (f.ms < best.ms) best = f;
     }
+    // patch operator: element products assembled per patch in LDS + shared-DOF sums
+    if (pt_prepare(c, sys) && pt_fits<R>(s))
+      for (int pc : {2, 4, 8}) {
+        SpmvCfg f = pt_cfg(s, op_grid(c, std::max<int64_t>(s->pt_npd, 1), pc));
+        try {
+          sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);  // warm
+        } catch (const Error&) {
+          break;   // a launch the device refuses (LDS per workgroup): not a candidate
+        }
+        SUCEM_HIP(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < reps; ++i) sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);
+        SUCEM_HIP(hipEventRecord(e1, c->stream));
+        SUCEM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SUCEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        f.ms = ms / reps;
+        if (f.ms < best.ms) best = f;
+      }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

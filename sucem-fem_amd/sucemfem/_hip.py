@@ -12,7 +12,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SUCEM_HIP_LIB", os.path.join(_HERE, "_native", "libsucem_hip.so"))
 
 SUCEM_OK = 0
-ERR_INVALID, ERR_HIP, ERR_OOM, ERR_OUTSIDE, ERR_MATERIAL, ERR_UNSUPPORTED = 1, 2, 3, 4, 5, 6
+ERR_INVALID, ERR_HIP, ERR_OOM, ERR_OUTSIDE, ERR_MATERIAL, ERR_UNSUPPORTED, ERR_INTERNAL = (
+    1, 2, 3, 4, 5, 6, 7)
 MAT_S, MAT_T, MAT_S0 = 1, 2, 4
 METHOD_COCG, METHOD_BICGSTAB = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_GRADIENT = 0, 1, 2
