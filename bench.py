@@ -61,14 +61,15 @@ def cpu_baseline(prob, fp64_iterations, budget_s=20.0):
                       % (setup_s, its, n, per_it, fp64_iterations)}
 
 
-def read_traffic(matrix_free, precision):
+def read_traffic(matrix_free, precision, cb):
     """HBM bytes per operator application measured with rocprofv3 counters (FETCH_SIZE and
     WRITE_SIZE, scaled by a copy of known size) for the 40^3 order-2 workload, from the
     summary kept in profiles/; None when no measurement matches the operator."""
     path = os.path.join(ROOT, "profiles", "operator_driven40_o2_traffic.md")
     if not os.path.exists(path):
         return None
-    key = "| %s | FP%d |" % ("matrix-free" if matrix_free else "assembled", precision)
+    name = ("assembled", "matrix-free", "patch %d" % cb)[matrix_free]
+    key = "| %s | FP%d |" % (name, precision)
     for line in open(path):
         if line.startswith(key):
             return float(line.split("|")[-2])
@@ -153,9 +154,10 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     spmv_ms = e0.elapsed_time(e1) / a.spmv_reps
-    mf = bool(tuned[a.precision].get("matrix_free", 0))
-    spmv_bytes = prob.operator_bytes(a.precision, mf)
-    op_bytes = prob.operator_bytes(a.precision, mf, epilogue=True)
+    mf = int(tuned[a.precision].get("matrix_free", 0))
+    cb = int(tuned[a.precision].get("loads", 256))
+    spmv_bytes = prob.operator_bytes(a.precision, mf, cb=cb)
+    op_bytes = prob.operator_bytes(a.precision, mf, epilogue=True, cb=cb)
     if a.copy_probe_mb > 0:
         src = torch.ones(a.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -164,9 +166,9 @@ def main():
         del src, dst
     isolated = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     achieved = op_bytes / (op_ms * 1e-3) / 1e9 if op_ms else 0.0
+    form = ("assembled CSR", "matrix-free per cell", "matrix-free per patch of %d cells" % cb)[mf]
     log("operator in the solves %s ms (%d samples), %.1f GB/s; alone %.4f ms, %.1f GB/s (%s)"
-        % ("%.4f" % op_ms if op_ms else "-", nsamp, achieved, spmv_ms, isolated,
-           "matrix-free" if mf else "CSR"))
+        % ("%.4f" % op_ms if op_ms else "-", nsamp, achieved, spmv_ms, isolated, form))
 
     extra = {}
     other = 32 if a.precision == 64 else 64
@@ -190,7 +192,7 @@ def main():
         log("%s solve %.2fs iters %d rel %.3e" % (name, other_s, sc["iterations"],
                                                   sc["rel_residual"]))
 
-    traffic = read_traffic(mf, a.precision) if (a.n, a.order) == (40, 2) else None
+    traffic = read_traffic(mf, a.precision, cb) if (a.n, a.order) == (40, 2) else None
 
     cpu = None
     if not a.no_cpu_baseline and fp64_iters:
@@ -227,10 +229,12 @@ def main():
         "spmv_tuned": {str(k): v for k, v in tuned.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": traffic,
-                     "kernel": ("k_mf_cells + k_mf_faces + k_mf_gather_lds (matrix-free "
-                                "S - k0^2 T + i k0 S0 with the fused p.Ap epilogue)"
-                                if mf else "k_sys_spmv (assembled S - k0^2 T + i k0 S0 with the "
-                                           "fused p.Ap epilogue)")
+                     "kernel": ("k_sys_spmv (assembled S - k0^2 T + i k0 S0",
+                                "k_mf_cells + k_mf_faces + k_mf_gather_lds (matrix-free "
+                                "S - k0^2 T + i k0 S0",
+                                "k_mf_patch + k_mf_psum (matrix-free S - k0^2 T + i k0 S0, "
+                                "patches of %d cells" % cb)[mf]
+                               + " with the fused p.Ap epilogue)"
                                + (", FP32, complex64 x" if a.precision == 32 else ", FP64"),
                      "bytes_per_launch": op_bytes, "avg_ms": op_ms, "samples": nsamp,
                      "timed_in": "the timed solves (HIP events on the solve's stream)",

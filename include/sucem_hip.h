@@ -178,10 +178,16 @@ int sucem_system_tune(sucem_ctx* ctx, sucem_system* sys, int precision, int reps
 /* Set the operator explicitly: cfg4_h = {matrix_free (0 assembled, 1 matrix-free per cell,
  * 2 matrix-free per patch), lanes (4, 8, 16, 32; matrix-free: lanes per row of its gather,
  * 1 or 4, or 0 for one row per thread staged through LDS; patch: ignored), loads (1, 2, 4;
- * matrix-free: 1 = element rows in one pass, 2 = half the rows at a time; patch: ignored),
+ * matrix-free: 1 = element rows in one pass, 2 = half the rows at a time; patch: cells
+ * per patch, 128, 192 or 256 (order 1: 256)),
  * blocks (multiple of 8; patch: grid of the shared-DOF kernel)}.
  * Calls on one system must not overlap (the matrix-free element buffer is per system). */
 int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, const int* cfg4_h);
+/* Sizes of the patch operator's maps for patches of cb cells (built if needed):
+ * info6_h = {patches, patch DOFs, DOFs shared by patches, partial sums, largest LDS
+ * entry count of a patch, largest (boundary face, row) count of a patch}; for the
+ * operator's byte count.  (No reference counterpart.) */
+int sucem_system_patch_info(sucem_ctx* ctx, sucem_system* sys, int cb, int64_t* info6_h);
 /* Materialised A values (complex128 interleaved) on the full pattern. */
 int sucem_system_export_lhs(sucem_system* sys, double k0, double* values_h);
 

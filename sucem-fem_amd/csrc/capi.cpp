@@ -424,12 +424,17 @@ int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, cons
   Context* c = &ctx->c;
   c->use();
   if (f.mf == 2) {
-    SUCEM_CHECK(pt_prepare(c, &sys->s), SUCEM_ERR_UNSUPPORTED,
+    SUCEM_CHECK(f.U == 128 || f.U == 192 || f.U == 256, SUCEM_ERR_INVALID,
+                "patch operator: loads (cells per patch) must be 128, 192 or 256");
+    SUCEM_CHECK(sys->s.space->order == 2 || f.U == 256, SUCEM_ERR_UNSUPPORTED,
+                "patch operator: order-1 patches hold 256 cells");
+    SUCEM_CHECK(pt_prepare(c, &sys->s, f.U), SUCEM_ERR_UNSUPPORTED,
                 "patch operator: orders 1 and 2, patches of at most 65535 products");
-    SUCEM_CHECK(precision == 32 ? pt_fits<float>(sys->s.space) : pt_fits<double>(sys->s.space),
+    SUCEM_CHECK(precision == 32 ? pt_fits<float>(sys->s.space, f.U)
+                                : pt_fits<double>(sys->s.space, f.U),
                 SUCEM_ERR_UNSUPPORTED,
                 "the patch operator needs more LDS per workgroup than the device has");
-    f = pt_cfg(sys->s.space, f.G);
+    f = pt_cfg(sys->s.space, f.U, f.G);
   } else if (f.mf == 1) {
     SUCEM_CHECK(sys->s.space->nloc == 6 || sys->s.space->nloc == 20, SUCEM_ERR_UNSUPPORTED,
                 "matrix-free operator: orders 1 and 2 only");
@@ -445,6 +450,21 @@ int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, cons
     SUCEM_CHECK(f.U == 1 || f.U == 2 || f.U == 4, SUCEM_ERR_INVALID, "loads must be 1, 2 or 4");
   }
   sys->s.cfg[precision == 32 ? 0 : 1] = f;
+  API_END
+}
+
+int sucem_system_patch_info(sucem_ctx* ctx, sucem_system* sys, int cb, int64_t* info6_h) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys); NOT_NULL(info6_h);
+  SUCEM_CHECK(cb == 128 || cb == 192 || cb == 256, SUCEM_ERR_INVALID,
+              "cells per patch must be 128, 192 or 256");
+  Context* c = &ctx->c;
+  c->use();
+  SUCEM_CHECK(pt_prepare(c, &sys->s, cb), SUCEM_ERR_UNSUPPORTED,
+              "patch operator: orders 1 and 2, patches of at most 65535 products");
+  const PatchMaps& m = sys->s.space->pt[cb == 128 ? 0 : (cb == 192 ? 1 : 2)];
+  info6_h[0] = m.np; info6_h[1] = m.nu; info6_h[2] = m.npd; info6_h[3] = m.npe;
+  info6_h[4] = m.lds; info6_h[5] = m.maxfr;
   API_END
 }
 

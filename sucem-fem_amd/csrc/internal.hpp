@@ -28,6 +28,24 @@ struct Mesh {
   DBuf<double> inv_mu_r;  // nc
 };
 
+// Patch operator maps for patches of cb consecutive cells (in Morton order).  Per patch
+// its DOFs ("patch DOFs", global ids ascending); per cell slot (cell rank r, local DOF l),
+// at lp[l * nc + r], the slot's patch-DOF index (low 16 bits) and its place among the
+// patch's products sorted by patch DOF (high 16 bits); per patch DOF the end of its
+// product segment and where its sum goes (-1: the DOF lies in this patch only, its sum is
+// the result; else its slot in the partial-sum buffer, which is sorted by DOF and then
+// patch); per boundary face its products' places; the DOFs shared by patches with the
+// segments of their partial sums.
+struct PatchMaps {
+  int cb = 0;                           // cells per patch; 0: not built
+  int64_t np = 0, nu = 0, npd = 0, npe = 0;
+  int lds = 0;                          // largest max(patch DOFs, patch products)
+  int maxfr = 0;                        // largest number of (face, row) pairs of a patch
+  DBuf<uint32_t> lp;                    // nloc * nc
+  DBuf<int32_t> uptr, dof, out, fptr, qptr, qdof;
+  DBuf<uint16_t> seg, fpos;
+};
+
 struct Space {
   Context* ctx = nullptr;
   std::mutex lazy;                // guards the data built on first use (gradient, matrix-free)
@@ -64,33 +82,22 @@ struct Space {
   // cells (cell * nloc + local) and boundary faces (face * nfd + face DOF) sorted by DOF:
   // segment pointers per DOF and each product's position in that order
   DBuf<int32_t> mf_cdT, mf_dptr, mf_posT, mf_posF;
-  // patch operator (built on first use): cells in Morton order of their centroids, cut
-  // into patches of pt_cb consecutive cells.  Per patch its DOFs ("patch DOFs", global
-  // ids ascending); per cell slot (cell rank r, local DOF l), at pt_lp[l * nc + r], the
-  // slot's patch-DOF index (low 16 bits) and its place among the patch's products sorted
-  // by patch DOF (high 16 bits); per patch DOF the end of its product segment and where
-  // its sum goes (-1: the DOF lies in this patch only, its sum is the result; else its
-  // slot in the partial-sum buffer, which is sorted by DOF and then patch).
-  int pt_cb = 0;
-  int64_t pt_np = 0, pt_nu = 0, pt_npd = 0, pt_npe = 0;
-  int pt_lds = 0;                       // largest max(patch DOFs, patch products)
-                                        // (the face stage of pt_maxfr entries follows)
-  int pt_maxfr = 0;                     // largest number of (face, row) pairs in a patch
+  // patch operator (built on first use): cells in Morton order of their centroids
+  // (pt_cell: rank -> cell) and the boundary faces in the order of their cells' ranks
+  // (pt_face: original id, pt_fmeta: (cell rank << 2) | local face); per patch size, the
+  // maps of PatchMaps
+  int pt_state = 0;                     // 0 not built, 1 built, -1 space unsuited
   int pt_max_lds = 0;                   // LDS bytes per workgroup of the device
-  DBuf<int32_t> pt_cell;                // rank -> cell
-  DBuf<uint32_t> pt_lp;                 // nloc * nc
-  DBuf<int32_t> pt_uptr, pt_dof, pt_out;
-  DBuf<uint16_t> pt_seg;
-  DBuf<int32_t> pt_fptr, pt_face, pt_fmeta;   // faces in patch order: original id, (cell rank, local face)
-  DBuf<uint16_t> pt_fpos;               // nfd per patch-ordered face
-  DBuf<int32_t> pt_qptr, pt_qdof;       // DOFs shared by patches: partial-sum segments
+  DBuf<int32_t> pt_cell, pt_face, pt_fmeta;
+  PatchMaps pt[3];                      // patches of 128, 192, 256 cells
 };
 
 // SpMV launch configuration: lanes per row, loads per lane, blocks (0 = heuristic).
 struct SpmvCfg {
   int L = 0, U = 0, G = 0;
   int mf = 0;      // 1: matrix-free operator (G = grid of its gather kernel),
-                   // 2: patch operator (G = grid of its partial-sum kernel)
+                   // 2: patch operator (U = cells per patch, G = grid of its
+                   //    partial-sum kernel)
   int P = 0;       // block partials an epilogue launch writes (0: G)
   double ms = 0;
   int parts() const { return P > 0 ? P : G; }

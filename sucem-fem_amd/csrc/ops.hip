@@ -440,13 +440,18 @@ SpmvCfg csr_cfg(Context* c, const Csr* A) {
   return A->cfg.L > 0 ? A->cfg : default_cfg(c, A->n, A->nnz);
 }
 
-// Autotuner: time every (lanes per row, loads per lane, blocks) candidate of the
-// fused operator with HIP events on the context's stream and keep the fastest.
+// Autotuner: time every candidate of the fused operator with HIP events on the
+// context's stream and keep the fastest.  Inside a solve the operator's inputs have been
+// pushed out of the caches by the other kernels of the iteration, so every timed
+// application here follows a write of a buffer twice the size of the last-level cache
+// (256 MB on MI355X); back-to-back applications would be served from that cache and
+// favour a different form.
 template <typename R>
 SpmvCfg tune_sys(Context* c, System* sys, int reps) {
   const Space* s = sys->space;
   using C = typename C2<R>::T;
   DBuf<C> x(s->ndofs), y(s->ndofs);
+  DBuf<char> flush((size_t)512 << 20);
   SUCEM_HIP(hipMemsetAsync(x.p, 0, x.bytes(), c->stream));
   hipEvent_t e0, e1;
   SUCEM_HIP(hipEventCreate(&e0));
@@ -456,6 +461,22 @@ SpmvCfg tune_sys(Context* c, System* sys, int reps) {
   RedSite none{};
   SpmvCfg best = default_cfg(c, s->ndofs, s->nnz);
   best.ms = 1e30;
+  auto time_cfg = [&](SpmvCfg& f) {
+    sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);  // warm (first-use set-up)
+    double total = 0;
+    for (int i = 0; i < reps; ++i) {
+      SUCEM_HIP(hipMemsetAsync(flush.p, i & 0xff, flush.bytes(), c->stream));
+      SUCEM_HIP(hipEventRecord(e0, c->stream));
+      sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);
+      SUCEM_HIP(hipEventRecord(e1, c->stream));
+      SUCEM_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      SUCEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      total += ms;
+    }
+    f.ms = total / reps;
+    if (f.ms < best.ms) best = f;
+  };
   const int Ls[] = {4, 8, 16, 32};
   const int Us[] = {1, 2, 4};
   const int per_cu[] = {4, 8, 16, 32};
@@ -466,53 +487,32 @@ SpmvCfg tune_sys(Context* c, System* sys, int reps) {
       for (int pc : per_cu) {
         SpmvCfg f;
         f.L = L; f.U = U; f.G = op_grid(c, s->ndofs, pc);
-        sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);  // warm
-        SUCEM_HIP(hipEventRecord(e0, c->stream));
-        for (int i = 0; i < reps; ++i) sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);
-        SUCEM_HIP(hipEventRecord(e1, c->stream));
-        SUCEM_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        SUCEM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        f.ms = ms / reps;
-        if (f.ms < best.ms) best = f;
+        time_cfg(f);
       }
     }
   // matrix-free candidates: element products + row gather
   if (s->nloc == 6 || s->nloc == 20) {
     mf_prepare(c, sys);
     for (int split : {1, 2})
-    for (int lanes : {0, 1, 4})
-    for (int pc : per_cu) {
-      SpmvCfg f;
-      f.mf = 1; f.L = lanes; f.U = split; f.G = op_grid(c, s->ndofs, pc);
-      sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);  // warm
-      SUCEM_HIP(hipEventRecord(e0, c->stream));
-      for (int i = 0; i < reps; ++i) sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);
-      SUCEM_HIP(hipEventRecord(e1, c->stream));
-      SUCEM_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      SUCEM_HIP(hipEventElapsedTime(&ms, e0, e1));
-      f.ms = ms / reps;
-      if (f.ms < best.ms) best = f;
-    }
-    // patch operator: element products assembled per patch in LDS + shared-DOF sums
-    if (pt_prepare(c, sys) && pt_fits<R>(s))
-      for (int pc : {2, 4, 8}) {
-        SpmvCfg f = pt_cfg(s, op_grid(c, std::max<int64_t>(s->pt_npd, 1), pc));
+      for (int lanes : {0, 1, 4})
+        for (int pc : per_cu) {
+          SpmvCfg f;
+          f.mf = 1; f.L = lanes; f.U = split; f.G = op_grid(c, s->ndofs, pc);
+          time_cfg(f);
+        }
+    // patch operator: element products summed per patch of cells in LDS + shared-DOF sums
+    for (int cb : {128, 192, 256}) {
+      if ((s->order == 1 && cb != 256) || !pt_prepare(c, sys, cb) || !pt_fits<R>(s, cb)) continue;
+      const int64_t npd = s->pt[cb == 128 ? 0 : (cb == 192 ? 1 : 2)].npd;
+      for (int pc : {4, 16}) {
+        SpmvCfg f = pt_cfg(s, cb, op_grid(c, std::max<int64_t>(npd, 1), pc));
         try {
-          sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);  // warm
+          time_cfg(f);
         } catch (const Error&) {
           break;   // a launch the device refuses (LDS per workgroup): not a candidate
         }
-        SUCEM_HIP(hipEventRecord(e0, c->stream));
-        for (int i = 0; i < reps; ++i) sys_apply<R>(c, sys, k, nullptr, x.p, y.p, nullptr, 0, none, f);
-        SUCEM_HIP(hipEventRecord(e1, c->stream));
-        SUCEM_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        SUCEM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        f.ms = ms / reps;
-        if (f.ms < best.ms) best = f;
       }
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -29,17 +29,17 @@ template <typename R>
 void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x, void* y,
               const void* w, int epi, const RedSite& rs, const SpmvCfg& f);
 
-// patch operator (patch.hip): pt_prepare before the first pt_apply on a system; false
-// when the space does not suit it (then nothing is built)
-bool pt_prepare(Context* c, System* sys);
+// patch operator (patch.hip) with patches of cb cells (128, 192, 256): pt_prepare before
+// the first pt_apply with that cb on a system; false when the space does not suit it
+bool pt_prepare(Context* c, System* sys, int cb);
 template <typename R>
 void pt_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x, void* y,
               const void* w, int epi, const RedSite& rs, const SpmvCfg& f);
 // whether the patch kernel's LDS need in precision R fits the device
 template <typename R>
-bool pt_fits(const Space* s);
-// the cfg of the patch operator with a partial-sum grid of G blocks
-SpmvCfg pt_cfg(const Space* s, int G);
+bool pt_fits(const Space* s, int cb);
+// the cfg of the (prepared) patch operator with a partial-sum grid of G blocks
+SpmvCfg pt_cfg(const Space* s, int cb, int G);
 
 // Second half of a grid reduction: sum the G block partials of rs and finalise its
 // scalars (krylov.hip); a no-op for rs.mode == FIN_NONE.

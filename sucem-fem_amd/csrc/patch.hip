@@ -3,7 +3,8 @@
 // The per-cell form (matfree.hip) writes every element product to HBM and reads it back
 // in a second kernel that sums each DOF's products.  Here the cells are ordered along a
 // Morton curve of their centroids and cut into patches of CB consecutive cells (compact
-// clusters of the mesh), one workgroup per patch:
+// clusters of the mesh; CB = 128, 192 or 256, chosen by the autotuner), one workgroup
+// per patch:
 //
 //  1. the patch's DOFs ("patch DOFs": the union of its cells' DOFs, ~1.4 per DOF of the
 //     mesh) are gathered from x into LDS once;
@@ -37,7 +38,10 @@ namespace sucem {
 namespace {
 
 constexpr int TB = 256;    // threads of the set-up kernels and of k_mf_psum
-constexpr int CB = 256;    // cells per patch = threads of k_mf_patch
+constexpr int UB = 8;      // patch DOFs per thread and batch in k_mf_patch
+constexpr int kCbs[3] = {128, 192, 256};   // cells per patch = threads of k_mf_patch
+
+int cb_slot(int cb) { return cb == 128 ? 0 : (cb == 192 ? 1 : (cb == 256 ? 2 : -1)); }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   v &= 0x3ffu;
@@ -95,6 +99,12 @@ __global__ void k_pt_face_meta(int64_t nbf, const uint32_t* rank_sorted, const i
     meta[i] = (int32_t)((rank_sorted[i] << 2) | (uint32_t)bf_local[face[i]]);
 }
 
+__global__ void k_pt_meta_rank(int64_t nbf, const int32_t* meta, uint32_t* rank) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nbf;
+       i += (int64_t)gridDim.x * blockDim.x)
+    rank[i] = (uint32_t)(meta[i] >> 2);
+}
+
 // ptr[q] = first i with grp(i) >= q (q = 0..ng), for group ids ascending in i; the
 // group of entry i is key[i] >> shift, divided by div.
 template <typename K>
@@ -112,7 +122,7 @@ __global__ void k_pt_ptr(int64_t n, const K* key, int shift, int64_t div, int64_
 
 // Every product slot: cells e = r * nloc + l (r = rank), then the faces in patch order
 // (nc * nloc + fp * nfd + a); key = (patch << 32) | DOF.
-__global__ void k_pt_entries(int64_t nc, int nloc, int64_t nbf, int nfd, const int32_t* order,
+__global__ void k_pt_entries(int64_t nc, int nloc, int64_t nbf, int nfd, int cb, const int32_t* order,
                              const int32_t* cell_dofs, const int32_t* meta,
                              const int32_t* face_dofs, uint64_t* key, uint32_t* val) {
   const int64_t ncl = nc * nloc, n = ncl + nbf * nfd;
@@ -131,7 +141,7 @@ __global__ void k_pt_entries(int64_t nc, int nloc, int64_t nbf, int nfd, const i
       l = face_dofs[(mt & 3) * nfd + a];
     }
     const uint64_t dof = (uint64_t)cell_dofs[(int64_t)order[r] * nloc + l];
-    key[e] = ((uint64_t)(r / CB) << 32) | dof;
+    key[e] = ((uint64_t)(r / cb) << 32) | dof;
     val[e] = (uint32_t)e;
   }
 }
@@ -224,11 +234,11 @@ __global__ void k_pt_permute(int64_t n, int k, const int32_t* from, const T* src
   }
 }
 
-template <typename R>
+template <int NT>
 __device__ __forceinline__ void block_partials(const RedSite& rs, int Gt, int at, double2 d1,
                                                double2 d2) {
-  d1 = block_sum<CB>(d1);
-  d2 = block_sum<CB>(d2);
+  d1 = block_sum<NT>(d1);
+  d2 = block_sum<NT>(d2);
   if (threadIdx.x == 0) {
     rs.part[at] = d1;
     rs.part[Gt + at] = d2;
@@ -248,7 +258,7 @@ __device__ __forceinline__ void epilogue(double2& d1, double2& d2, C v, C wv) {
 // One workgroup per patch (XCD-aware order: consecutive patches share x entries and
 // should share an L2); Gp = grid (a multiple of 8), Gt = all block partials of the
 // operator (Gp + the grid of k_mf_psum).
-template <typename R, int ORDER, int NFD, int EPI>
+template <typename R, int ORDER, int NFD, int EPI, int CB>
 __global__ __launch_bounds__(CB) void k_mf_patch(
     int64_t np, int Gp, int Gt, int64_t nc, int64_t nbf, int lds_main, const R* __restrict__ coef,
     const uint32_t* __restrict__ lp, const int32_t* __restrict__ uptr,
@@ -270,12 +280,27 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
   double2 d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0);
   if (p < np) {   // uniform over the workgroup
     const int u0 = uptr[p], nu = uptr[p + 1] - u0;
-    for (int j = threadIdx.x; j < nu; j += CB) sm[j] = x[pdof[u0 + j]];
+    // batches of UB entries per thread: all id loads, then all x loads, in flight together
+    for (int j0 = 0; j0 < nu; j0 += UB * CB) {
+      int dv[UB];
+#pragma unroll
+      for (int q = 0; q < UB; ++q) {
+        const int j = j0 + q * CB + threadIdx.x;
+        dv[q] = j < nu ? pdof[u0 + j] : -1;
+      }
+      C xv[UB];
+#pragma unroll
+      for (int q = 0; q < UB; ++q) xv[q] = dv[q] >= 0 ? x[dv[q]] : mk(R(0), R(0));
+#pragma unroll
+      for (int q = 0; q < UB; ++q) {
+        const int j = j0 + q * CB + threadIdx.x;
+        if (j < nu) sm[j] = xv[q];
+      }
+    }
     __syncthreads();
     const int64_t r = p * CB + threadIdx.x;
     const bool has = r < nc;
     C acc[NLOC];
-    uint32_t lpv[NLOC];
     if (has) {
       R a[12];
 #pragma unroll
@@ -284,8 +309,7 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
       C xl[NLOC];
 #pragma unroll
       for (int i = 0; i < NLOC; ++i) {
-        lpv[i] = lp[(int64_t)i * nc + r];
-        xl[i] = sm[lpv[i] & 0xffffu];
+        xl[i] = sm[lp[(int64_t)i * nc + r] & 0xffffu];
       }
 #pragma unroll
       for (int i = 0; i < NLOC; ++i) acc[i] = mk(R(0), R(0));
@@ -323,30 +347,42 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
     __syncthreads();   // every read of the x copy is done: the products overwrite it
     if (has) {
 #pragma unroll
-      for (int i = 0; i < NLOC; ++i) sm[lpv[i] >> 16] = acc[i];
+      for (int i = 0; i < NLOC; ++i) sm[lp[(int64_t)i * nc + r] >> 16] = acc[i];
     }
     for (int id = threadIdx.x; id < nfr; id += CB) sm[fpos[(int64_t)f0 * NFD + id]] = stage[id];
     __syncthreads();
-    for (int j = threadIdx.x; j < nu; j += CB) {
-      const int lo = j ? (int)pseg[u0 + j - 1] : 0, hi = pseg[u0 + j];
-      C v = mk(R(0), R(0));
-      for (int q = lo; q < hi; ++q) {
-        const C t = sm[q];
-        v.x += t.x;
-        v.y += t.y;
+    for (int j0 = 0; j0 < nu; j0 += UB * CB) {
+      int lo[UB], hi[UB], out[UB], dv[UB];
+#pragma unroll
+      for (int q = 0; q < UB; ++q) {
+        const int j = j0 + q * CB + threadIdx.x;
+        const bool in = j < nu;
+        lo[q] = (in && j) ? (int)pseg[u0 + j - 1] : 0;
+        hi[q] = in ? (int)pseg[u0 + j] : 0;
+        out[q] = in ? pout[u0 + j] : 0;
+        dv[q] = in ? pdof[u0 + j] : 0;
       }
-      const int out = pout[u0 + j];
-      if (out >= 0) {
-        part[out] = v;
-      } else {
-        const int d = pdof[u0 + j];
-        if (mask && mask[d]) v = mk(R(0), R(0));
-        y[d] = v;
-        if (EPI) epilogue<EPI>(d1, d2, v, w[d]);
+#pragma unroll
+      for (int q = 0; q < UB; ++q) {
+        C v = mk(R(0), R(0));
+        for (int t = lo[q]; t < hi[q]; ++t) {
+          const C a = sm[t];
+          v.x += a.x;
+          v.y += a.y;
+        }
+        if (hi[q] > 0) {   // an entry of this patch (segments are never empty)
+          if (out[q] >= 0) {
+            part[out[q]] = v;
+          } else {
+            if (mask && mask[dv[q]]) v = mk(R(0), R(0));
+            y[dv[q]] = v;
+            if (EPI) epilogue<EPI>(d1, d2, v, w[dv[q]]);
+          }
+        }
       }
     }
   }
-  if (EPI) block_partials<R>(rs, Gt, lb, d1, d2);
+  if (EPI) block_partials<CB>(rs, Gt, lb, d1, d2);
 }
 
 // Shared DOFs: y_d = sum of the partial sums of its patches, in patch order.
@@ -379,7 +415,7 @@ __global__ __launch_bounds__(TB) void k_mf_psum(int npd, int Gg, int Gt, int at0
     y[d] = v;
     if (EPI) epilogue<EPI>(d1, d2, v, w[d]);
   }
-  if (EPI) block_partials<R>(rs, Gt, at0 + lb, d1, d2);
+  if (EPI) block_partials<TB>(rs, Gt, at0 + lb, d1, d2);
 }
 
 template <typename T>
@@ -409,23 +445,21 @@ int32_t read_i32(Context* c, const int32_t* d) {
   return h;
 }
 
-void pt_validate(Context* c, const Space* s, const std::vector<int32_t>& ps,
+void pt_validate(Context* c, const Space* s, const PatchMaps& m, const std::vector<int32_t>& ps,
                  const std::vector<int32_t>& up, const std::vector<int32_t>& fp);
 
-// Space-level maps; sets s->pt_cb = -1 when the space does not suit the patch kernel.
-void pt_build_space(Context* c, Space* s) {
+// Morton order of the cells and patch order of the boundary faces (shared by every patch
+// size); s->pt_state = -1 when the space does not suit the patch kernel.
+void pt_build_order(Context* c, Space* s) {
   const Mesh* m = s->mesh;
   const int64_t nc = m->nc, nbf = s->nbf;
-  const int nloc = s->nloc, nfd = s->nfd;
-  if (!(nloc == 20 && nfd == 8) && !(nloc == 6 && nfd == 3)) {
-    s->pt_cb = -1;
+  if (!(s->nloc == 20 && s->nfd == 8) && !(s->nloc == 6 && s->nfd == 3)) {
+    s->pt_state = -1;
     return;
   }
-  const int64_t np = (nc + CB - 1) / CB;
-  const int64_t ne = nc * nloc + nbf * nfd;
-  SUCEM_CHECK(ne < (int64_t(1) << 31) && s->ndofs < (int64_t(1) << 31), SUCEM_ERR_UNSUPPORTED,
-              "mesh too large for the patch operator");
-  // Morton order of the cell centroids
+  const int64_t ne = nc * s->nloc + nbf * s->nfd;
+  SUCEM_CHECK(ne < (int64_t(1) << 31) && s->ndofs < (int64_t(1) << 31) && nc < (int64_t(1) << 29),
+              SUCEM_ERR_UNSUPPORTED, "mesh too large for the patch operator");
   std::vector<double> xyz((size_t)m->nv * 3);
   d2h(c, xyz.data(), m->xyz.p, xyz.size());
   c->sync();
@@ -452,11 +486,8 @@ void pt_build_space(Context* c, Space* s) {
     SUCEM_LAUNCH(k_pt_rank, grid_for(nc, TB), TB, 0, c->stream, nc, s->pt_cell.p, rank.p);
     c->sync();
   }
-  // boundary faces in patch order
-  s->pt_fptr.alloc(np + 1);
   s->pt_face.alloc(std::max<int64_t>(nbf, 1));
   s->pt_fmeta.alloc(std::max<int64_t>(nbf, 1));
-  s->pt_fpos.alloc(std::max<int64_t>(nbf * nfd, 1));
   if (nbf) {
     DBuf<uint32_t> key(nbf), key_s(nbf), val(nbf);
     SUCEM_LAUNCH(k_pt_face_keys, grid_for(nbf, TB), TB, 0, c->stream, nbf, s->bf_cell.p, rank.p,
@@ -470,16 +501,36 @@ void pt_build_space(Context* c, Space* s) {
                                         (uint32_t*)s->pt_face.p, (size_t)nbf, 0u, bits, c->stream));
     SUCEM_LAUNCH(k_pt_face_meta, grid_for(nbf, TB), TB, 0, c->stream, nbf, key_s.p, s->pt_face.p,
                  s->bf_local.p, s->pt_fmeta.p);
-    SUCEM_LAUNCH((k_pt_ptr<uint32_t>), grid_for(nbf, TB), TB, 0, c->stream, nbf, key_s.p, 0,
-                 (int64_t)CB, np, s->pt_fptr.p);
+    c->sync();
+  }
+  SUCEM_HIP(hipDeviceGetAttribute(&s->pt_max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                  c->device));
+  s->pt_state = 1;
+}
+
+// The maps of patches of cb cells (pm.cb = cb on return, or -cb when a patch would hold
+// 65536 or more products).
+void pt_build_maps(Context* c, Space* s, PatchMaps& pm, int cb) {
+  const int64_t nc = s->mesh->nc, nbf = s->nbf;
+  const int nloc = s->nloc, nfd = s->nfd;
+  const int64_t np = (nc + cb - 1) / cb;
+  const int64_t ne = nc * nloc + nbf * nfd;
+  pm.fptr.alloc(np + 1);
+  pm.fpos.alloc(std::max<int64_t>(nbf * nfd, 1));
+  if (nbf) {
+    // face ranks ascend in patch order (pt_fmeta >> 2)
+    DBuf<uint32_t> fr(nbf);
+    SUCEM_LAUNCH(k_pt_meta_rank, grid_for(nbf, TB), TB, 0, c->stream, nbf, s->pt_fmeta.p, fr.p);
+    SUCEM_LAUNCH((k_pt_ptr<uint32_t>), grid_for(nbf, TB), TB, 0, c->stream, nbf, fr.p, 0,
+                 (int64_t)cb, np, pm.fptr.p);
     c->sync();
   } else {
-    SUCEM_HIP(hipMemsetAsync(s->pt_fptr.p, 0, (np + 1) * sizeof(int32_t), c->stream));
+    SUCEM_HIP(hipMemsetAsync(pm.fptr.p, 0, (np + 1) * sizeof(int32_t), c->stream));
   }
   // product slots sorted by (patch, DOF, slot)
   DBuf<uint64_t> key(ne), key_s(ne);
   DBuf<uint32_t> val(ne), slot(ne);
-  SUCEM_LAUNCH(k_pt_entries, grid_for(ne, TB), TB, 0, c->stream, nc, nloc, nbf, nfd,
+  SUCEM_LAUNCH(k_pt_entries, grid_for(ne, TB), TB, 0, c->stream, nc, nloc, nbf, nfd, cb,
                s->pt_cell.p, s->cell_dofs.p, s->pt_fmeta.p, s->ref_face_dofs.p, key.p, val.p);
   {
     const unsigned bits = 32u + (unsigned)key_bits(np);
@@ -499,24 +550,31 @@ void pt_build_space(Context* c, Space* s) {
   SUCEM_LAUNCH(k_pt_heads, grid_for(ne, TB), TB, 0, c->stream, ne, key_s.p, flag.p);
   scan<int32_t>(c, flag.p, uidx.p, ne, true);
   const int64_t nu = read_i32(c, uidx.p + ne - 1);
-  s->pt_np = np;
-  s->pt_nu = nu;
-  s->pt_dof.alloc(nu);
-  s->pt_seg.alloc(nu);
-  s->pt_out.alloc(nu);
-  s->pt_uptr.alloc(np + 1);
-  s->pt_lp.alloc((size_t)nc * nloc);
-  SUCEM_LAUNCH(k_pt_units, grid_for(ne, TB), TB, 0, c->stream, ne, key_s.p, uidx.p, pstart.p,
-               s->pt_dof.p, s->pt_seg.p);
-  SUCEM_LAUNCH(k_pt_uptr, grid_for(np + 1, TB), TB, 0, c->stream, np, ne, pstart.p, uidx.p,
-               s->pt_uptr.p);
-  SUCEM_LAUNCH(k_pt_places, grid_for(ne, TB), TB, 0, c->stream, ne, nc, nloc, key_s.p, slot.p,
-               uidx.p, pstart.p, s->pt_uptr.p, s->pt_lp.p, s->pt_fpos.p);
-  // LDS need and face load per patch
-  std::vector<int32_t> ps(np + 1), up(np + 1), fp(np + 1);
+  std::vector<int32_t> ps(np + 1);
   d2h(c, ps.data(), pstart.p, np + 1);
-  d2h(c, up.data(), s->pt_uptr.p, np + 1);
-  d2h(c, fp.data(), s->pt_fptr.p, np + 1);
+  c->sync();
+  int64_t maxs = 0;
+  for (int64_t p = 0; p < np; ++p) maxs = std::max<int64_t>(maxs, ps[p + 1] - ps[p]);
+  if (maxs >= 65536) {   // the 16-bit places cannot address such a patch
+    pm.cb = -cb;
+    return;
+  }
+  pm.np = np;
+  pm.nu = nu;
+  pm.dof.alloc(nu);
+  pm.seg.alloc(nu);
+  pm.out.alloc(nu);
+  pm.uptr.alloc(np + 1);
+  pm.lp.alloc((size_t)nc * nloc);
+  SUCEM_LAUNCH(k_pt_units, grid_for(ne, TB), TB, 0, c->stream, ne, key_s.p, uidx.p, pstart.p,
+               pm.dof.p, pm.seg.p);
+  SUCEM_LAUNCH(k_pt_uptr, grid_for(np + 1, TB), TB, 0, c->stream, np, ne, pstart.p, uidx.p,
+               pm.uptr.p);
+  SUCEM_LAUNCH(k_pt_places, grid_for(ne, TB), TB, 0, c->stream, ne, nc, nloc, key_s.p, slot.p,
+               uidx.p, pstart.p, pm.uptr.p, pm.lp.p, pm.fpos.p);
+  std::vector<int32_t> up(np + 1), fp(np + 1);
+  d2h(c, up.data(), pm.uptr.p, np + 1);
+  d2h(c, fp.data(), pm.fptr.p, np + 1);
   c->sync();
   int64_t lds = 0, maxfr = 0;
   for (int64_t p = 0; p < np; ++p) {
@@ -526,16 +584,16 @@ void pt_build_space(Context* c, Space* s) {
   // shared DOFs: partial sums sorted by (DOF, patch)
   DBuf<int32_t> cnt(s->ndofs);
   SUCEM_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), c->stream));
-  SUCEM_LAUNCH(k_pt_count, grid_for(nu, TB), TB, 0, c->stream, nu, s->pt_dof.p, cnt.p);
+  SUCEM_LAUNCH(k_pt_count, grid_for(nu, TB), TB, 0, c->stream, nu, pm.dof.p, cnt.p);
   DBuf<uint32_t> du_in(nu), dk(nu), du(nu);
   SUCEM_LAUNCH(k_pt_iota, grid_for(nu, TB), TB, 0, c->stream, nu, du_in.p);
   {
     const unsigned bits = (unsigned)key_bits(s->ndofs);
     size_t bytes = 0;
-    SUCEM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)s->pt_dof.p, dk.p,
+    SUCEM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)pm.dof.p, dk.p,
                                         du_in.p, du.p, (size_t)nu, 0u, bits, c->stream));
     DBuf<char> tmp(bytes);
-    SUCEM_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, (const uint32_t*)s->pt_dof.p, dk.p,
+    SUCEM_HIP(rocprim::radix_sort_pairs(tmp.p, bytes, (const uint32_t*)pm.dof.p, dk.p,
                                         du_in.p, du.p, (size_t)nu, 0u, bits, c->stream));
     c->sync();
   }
@@ -546,24 +604,20 @@ void pt_build_space(Context* c, Space* s) {
   scan<int32_t>(c, head.p, qi.p, nu, false);
   const int64_t npe = read_i32(c, ppos.p + nu - 1) + read_i32(c, shared.p + nu - 1);
   const int64_t npd = read_i32(c, qi.p + nu - 1) + read_i32(c, head.p + nu - 1);
-  s->pt_qdof.alloc(std::max<int64_t>(npd, 1));
-  s->pt_qptr.alloc(npd + 1);
+  pm.qdof.alloc(std::max<int64_t>(npd, 1));
+  pm.qptr.alloc(npd + 1);
   SUCEM_LAUNCH(k_pt_outs, grid_for(nu, TB), TB, 0, c->stream, nu, dk.p, du.p, shared.p, ppos.p,
-               head.p, qi.p, s->pt_out.p, s->pt_qdof.p, s->pt_qptr.p);
+               head.p, qi.p, pm.out.p, pm.qdof.p, pm.qptr.p);
   const int32_t npe32 = (int32_t)npe;
-  SUCEM_HIP(hipMemcpyAsync(s->pt_qptr.p + npd, &npe32, sizeof(int32_t), hipMemcpyHostToDevice,
+  SUCEM_HIP(hipMemcpyAsync(pm.qptr.p + npd, &npe32, sizeof(int32_t), hipMemcpyHostToDevice,
                            c->stream));
   c->sync();
-  s->pt_npd = npd;
-  s->pt_npe = npe;
-  s->pt_lds = (int)lds;
-  s->pt_maxfr = (int)maxfr;
-  int max_lds = 0;
-  SUCEM_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-  s->pt_max_lds = max_lds;
-  const bool fits = lds < 65536 && (lds + maxfr) * (int64_t)sizeof(float2) <= (int64_t)max_lds;
-  s->pt_cb = fits ? CB : -1;
-  if (fits) pt_validate(c, s, ps, up, fp);
+  pm.npd = npd;
+  pm.npe = npe;
+  pm.lds = (int)lds;
+  pm.maxfr = (int)maxfr;
+  pm.cb = cb;
+  pt_validate(c, s, pm, ps, up, fp);
 }
 
 // Host check of every index the patch kernels follow (one pass over the maps when they
@@ -571,23 +625,23 @@ void pt_build_space(Context* c, Space* s) {
 // segments ascending and closing at the patch's product count, destinations inside the
 // partial-sum buffer, faces inside their patch.  A map that fails is an internal error,
 // reported before any kernel reads through it.
-void pt_validate(Context* c, const Space* s, const std::vector<int32_t>& ps,
+void pt_validate(Context* c, const Space* s, const PatchMaps& pm, const std::vector<int32_t>& ps,
                  const std::vector<int32_t>& up, const std::vector<int32_t>& fp) {
-  const int64_t nc = s->mesh->nc, nbf = s->nbf, np = s->pt_np, nu = s->pt_nu;
-  const int nloc = s->nloc, nfd = s->nfd;
+  const int64_t nc = s->mesh->nc, nbf = s->nbf, np = pm.np, nu = pm.nu;
+  const int nloc = s->nloc, nfd = s->nfd, cb = pm.cb;
   std::vector<uint32_t> lp((size_t)nc * nloc);
-  std::vector<int32_t> dof(nu), out(nu), meta(nbf), qptr(s->pt_npd + 1), qdof(s->pt_npd);
+  std::vector<int32_t> dof(nu), out(nu), meta(nbf), qptr(pm.npd + 1), qdof(pm.npd);
   std::vector<uint16_t> seg(nu), fpos((size_t)nbf * nfd);
   std::vector<int32_t> cell(nc);
-  d2h(c, lp.data(), s->pt_lp.p, lp.size());
-  d2h(c, dof.data(), s->pt_dof.p, nu);
-  d2h(c, out.data(), s->pt_out.p, nu);
-  d2h(c, seg.data(), s->pt_seg.p, nu);
+  d2h(c, lp.data(), pm.lp.p, lp.size());
+  d2h(c, dof.data(), pm.dof.p, nu);
+  d2h(c, out.data(), pm.out.p, nu);
+  d2h(c, seg.data(), pm.seg.p, nu);
   d2h(c, cell.data(), s->pt_cell.p, nc);
   d2h(c, meta.data(), s->pt_fmeta.p, nbf);
-  d2h(c, fpos.data(), s->pt_fpos.p, fpos.size());
-  d2h(c, qptr.data(), s->pt_qptr.p, qptr.size());
-  d2h(c, qdof.data(), s->pt_qdof.p, qdof.size());
+  d2h(c, fpos.data(), pm.fpos.p, fpos.size());
+  d2h(c, qptr.data(), pm.qptr.p, qptr.size());
+  d2h(c, qdof.data(), pm.qdof.p, qdof.size());
   c->sync();
   bool ok = ps[0] == 0 && up[0] == 0 && fp[0] == 0 && up[np] == nu && fp[np] == nbf &&
             ps[np] == nc * nloc + nbf * nfd;
@@ -598,84 +652,35 @@ void pt_validate(Context* c, const Space* s, const std::vector<int32_t>& ps,
   }
   for (int64_t p = 0; ok && p < np; ++p) {
     const int64_t S = ps[p + 1] - ps[p], U = up[p + 1] - up[p];
-    ok = S > 0 && U > 0 && S <= s->pt_lds && U <= s->pt_lds && fp[p + 1] >= fp[p];
-    for (int64_t r = p * CB; ok && r < std::min<int64_t>(nc, (p + 1) * CB); ++r)
+    ok = S > 0 && U > 0 && S <= pm.lds && U <= pm.lds && fp[p + 1] >= fp[p];
+    for (int64_t r = p * cb; ok && r < std::min<int64_t>(nc, (p + 1) * cb); ++r)
       for (int l = 0; ok && l < nloc; ++l) {
         const uint32_t v = lp[(size_t)l * nc + r];
         ok = (int64_t)(v & 0xffffu) < U && (int64_t)(v >> 16) < S;
       }
     int64_t prev = 0;
     for (int64_t u = up[p]; ok && u < up[p + 1]; ++u) {
-      ok = seg[u] > prev && dof[u] >= 0 && dof[u] < s->ndofs && out[u] >= -1 && out[u] < s->pt_npe;
+      ok = seg[u] > prev && dof[u] >= 0 && dof[u] < s->ndofs && out[u] >= -1 && out[u] < pm.npe;
       prev = seg[u];
     }
     ok = ok && prev == S;
     for (int64_t f = fp[p]; ok && f < fp[p + 1]; ++f) {
-      ok = (meta[f] >> 2) / CB == p;
+      ok = (meta[f] >> 2) / cb == p;
       for (int a = 0; ok && a < nfd; ++a) ok = fpos[(size_t)f * nfd + a] < S;
     }
   }
-  for (int64_t i = 0; ok && i < s->pt_npd; ++i)
+  for (int64_t i = 0; ok && i < pm.npd; ++i)
     ok = qptr[i] < qptr[i + 1] && qdof[i] >= 0 && qdof[i] < s->ndofs;
-  ok = ok && (s->pt_npd == 0 || (qptr[0] == 0 && qptr[s->pt_npd] == s->pt_npe));
+  ok = ok && (pm.npd == 0 || (qptr[0] == 0 && qptr[pm.npd] == pm.npe));
   SUCEM_CHECK(ok, SUCEM_ERR_INTERNAL, "patch operator maps failed their consistency check");
 }
 
-}  // namespace
-
-bool pt_prepare(Context* c, System* sys) {
-  if (sys->pt_ready) return true;
-  Space* s = sys->space;
-  {
-    std::lock_guard<std::mutex> hold(s->lazy);
-    if (s->pt_cb == 0) pt_build_space(c, s);
-  }
-  if (s->pt_cb < 0) return false;
-  mf_prepare(c, sys);   // the per-cell coefficients, permuted below
-  const int64_t nc = s->mesh->nc, nbf = s->nbf;
-  sys->pt_coef64.alloc((size_t)12 * nc);
-  sys->pt_coef32.alloc((size_t)12 * nc);
-  SUCEM_LAUNCH((k_pt_permute<double>), grid_for(12 * nc, TB), TB, 0, c->stream, nc, 12,
-               s->pt_cell.p, sys->mf_coef64.p, sys->pt_coef64.p);
-  SUCEM_LAUNCH((k_pt_permute<float>), grid_for(12 * nc, TB), TB, 0, c->stream, nc, 12,
-               s->pt_cell.p, sys->mf_coef32.p, sys->pt_coef32.p);
-  sys->pt_face64.alloc(std::max<int64_t>(6 * nbf, 1));
-  sys->pt_face32.alloc(std::max<int64_t>(6 * nbf, 1));
-  if (nbf) {
-    SUCEM_LAUNCH((k_pt_permute<double>), grid_for(6 * nbf, TB), TB, 0, c->stream, nbf, 6,
-                 s->pt_face.p, sys->mf_face64.p, sys->pt_face64.p);
-    SUCEM_LAUNCH((k_pt_permute<float>), grid_for(6 * nbf, TB), TB, 0, c->stream, nbf, 6,
-                 s->pt_face.p, sys->mf_face32.p, sys->pt_face32.p);
-  }
-  sys->pt_part.alloc(std::max<int64_t>(s->pt_npe, 1));
-  c->sync();
-  sys->pt_ready = true;
-  return true;
-}
-
-template <typename R>
-bool pt_fits(const Space* s) {
-  return s->pt_cb > 0 && (int64_t)(s->pt_lds + s->pt_maxfr) * (int64_t)sizeof(typename C2<R>::T) <=
-                            (int64_t)s->pt_max_lds;
-}
-template bool pt_fits<float>(const Space*);
-template bool pt_fits<double>(const Space*);
-
-SpmvCfg pt_cfg(const Space* s, int G) {
-  SpmvCfg f;
-  f.mf = 2;
-  f.G = G;
-  f.P = (int)((s->pt_np + 7) / 8 * 8) + G;
-  return f;
-}
-
-template <typename R>
-void pt_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x, void* y,
-              const void* w, int epi, const RedSite& rs, const SpmvCfg& f) {
+template <typename R, int ORDER, int NFD, int CB>
+void launch_patch(Context* c, const System* sys, const PatchMaps& pm, Coef k,
+                  const uint8_t* mask, const void* x, void* y, const void* w, int epi,
+                  const RedSite& rs, int Gp, int Gt) {
   using C = typename C2<R>::T;
-  SUCEM_CHECK(sys->pt_ready, SUCEM_ERR_INVALID, "patch operator not prepared");
   const Space* s = sys->space;
-  const int64_t nc = s->mesh->nc, nbf = s->nbf, np = s->pt_np;
   const R* coef;
   const R* hf;
   if constexpr (sizeof(R) == 4) {
@@ -683,31 +688,106 @@ void pt_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
   } else {
     coef = sys->pt_coef64.p; hf = sys->pt_face64.p;
   }
-  const int Gp = (int)((np + 7) / 8 * 8);
+  const R c0 = (sys->which & SUCEM_MAT_S0) ? (R)k.s0 : R(0);
+  const size_t lds = (size_t)(pm.lds + pm.maxfr) * sizeof(C);
+#define PT_ARGS pm.np, Gp, Gt, s->mesh->nc, s->nbf, pm.lds, coef, pm.lp.p, pm.uptr.p, pm.dof.p, \
+                pm.seg.p, pm.out.p, pm.fptr.p, s->pt_fmeta.p, pm.fpos.p, hf, s->ref_face_dofs.p, \
+                s->ref_face.p, (R)k.s, (R)k.t, c0, mask, (const C*)x, (C*)y, (C*)sys->pt_part.p, \
+                (const C*)w, rs
+  if (epi == 0) SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 0, CB>), Gp, CB, lds, c->stream, PT_ARGS);
+  else if (epi == 1) SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 1, CB>), Gp, CB, lds, c->stream, PT_ARGS);
+  else SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 2, CB>), Gp, CB, lds, c->stream, PT_ARGS);
+#undef PT_ARGS
+}
+
+}  // namespace
+
+bool pt_prepare(Context* c, System* sys, int cb) {
+  Space* s = sys->space;
+  const int slot = cb_slot(cb);
+  SUCEM_CHECK(slot >= 0, SUCEM_ERR_INVALID, "cells per patch must be 128, 192 or 256");
+  {
+    std::lock_guard<std::mutex> hold(s->lazy);
+    if (s->pt_state == 0) pt_build_order(c, s);
+    if (s->pt_state > 0 && s->pt[slot].cb == 0)
+      pt_build_maps(c, s, s->pt[slot], cb);
+  }
+  if (s->pt_state < 0 || s->pt[slot].cb != cb) return false;
+  if (!sys->pt_ready) {
+    mf_prepare(c, sys);   // the per-cell coefficients, permuted below
+    const int64_t nc = s->mesh->nc, nbf = s->nbf;
+    sys->pt_coef64.alloc((size_t)12 * nc);
+    sys->pt_coef32.alloc((size_t)12 * nc);
+    SUCEM_LAUNCH((k_pt_permute<double>), grid_for(12 * nc, TB), TB, 0, c->stream, nc, 12,
+                 s->pt_cell.p, sys->mf_coef64.p, sys->pt_coef64.p);
+    SUCEM_LAUNCH((k_pt_permute<float>), grid_for(12 * nc, TB), TB, 0, c->stream, nc, 12,
+                 s->pt_cell.p, sys->mf_coef32.p, sys->pt_coef32.p);
+    sys->pt_face64.alloc(std::max<int64_t>(6 * nbf, 1));
+    sys->pt_face32.alloc(std::max<int64_t>(6 * nbf, 1));
+    if (nbf) {
+      SUCEM_LAUNCH((k_pt_permute<double>), grid_for(6 * nbf, TB), TB, 0, c->stream, nbf, 6,
+                   s->pt_face.p, sys->mf_face64.p, sys->pt_face64.p);
+      SUCEM_LAUNCH((k_pt_permute<float>), grid_for(6 * nbf, TB), TB, 0, c->stream, nbf, 6,
+                   s->pt_face.p, sys->mf_face32.p, sys->pt_face32.p);
+    }
+    c->sync();
+    sys->pt_ready = true;
+  }
+  if ((int64_t)sys->pt_part.n < std::max<int64_t>(s->pt[slot].npe, 1)) {
+    c->sync();
+    sys->pt_part.alloc(std::max<int64_t>(s->pt[slot].npe, 1));
+  }
+  return true;
+}
+
+template <typename R>
+bool pt_fits(const Space* s, int cb) {
+  const int slot = cb_slot(cb);
+  if (slot < 0 || s->pt_state <= 0 || s->pt[slot].cb != cb) return false;
+  const PatchMaps& pm = s->pt[slot];
+  return (int64_t)(pm.lds + pm.maxfr) * (int64_t)sizeof(typename C2<R>::T) <=
+         (int64_t)s->pt_max_lds;
+}
+template bool pt_fits<float>(const Space*, int);
+template bool pt_fits<double>(const Space*, int);
+
+SpmvCfg pt_cfg(const Space* s, int cb, int G) {
+  SpmvCfg f;
+  f.mf = 2;
+  f.U = cb;
+  f.G = G;
+  const int slot = cb_slot(cb);
+  f.P = (int)((s->pt[slot < 0 ? 0 : slot].np + 7) / 8 * 8) + G;
+  return f;
+}
+
+template <typename R>
+void pt_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x, void* y,
+              const void* w, int epi, const RedSite& rs, const SpmvCfg& f) {
+  using C = typename C2<R>::T;
+  const Space* s = sys->space;
+  const int slot = cb_slot(f.U);
+  SUCEM_CHECK(sys->pt_ready && slot >= 0 && s->pt[slot].cb == f.U, SUCEM_ERR_INVALID,
+              "patch operator not prepared");
+  const PatchMaps& pm = s->pt[slot];
+  const int Gp = (int)((pm.np + 7) / 8 * 8);
   const int Gg = f.G;
   const int Gt = Gp + Gg;
   SUCEM_CHECK(Gg > 0 && Gg % 8 == 0 && f.parts() == Gt, SUCEM_ERR_INVALID,
               "bad patch operator launch configuration");
-  const R c0 = (sys->which & SUCEM_MAT_S0) ? (R)k.s0 : R(0);
-  const size_t lds = (size_t)(s->pt_lds + s->pt_maxfr) * sizeof(C);
-  SUCEM_CHECK(pt_fits<R>(s), SUCEM_ERR_UNSUPPORTED,
+  SUCEM_CHECK(pt_fits<R>(s, f.U), SUCEM_ERR_UNSUPPORTED,
               "the patch operator needs more LDS per workgroup than the device has");
-  C* part = (C*)sys->pt_part.p;
-#define PT_ARGS np, Gp, Gt, nc, nbf, s->pt_lds, coef, s->pt_lp.p, s->pt_uptr.p, s->pt_dof.p, s->pt_seg.p, \
-                s->pt_out.p, s->pt_fptr.p, s->pt_fmeta.p, s->pt_fpos.p, hf, s->ref_face_dofs.p, \
-                s->ref_face.p, (R)k.s, (R)k.t, c0, mask, (const C*)x, (C*)y, part, (const C*)w, rs
+  SUCEM_CHECK((int64_t)sys->pt_part.n >= pm.npe, SUCEM_ERR_INVALID, "patch operator not prepared");
   if (s->order == 2) {
-    if (epi == 0) SUCEM_LAUNCH((k_mf_patch<R, 2, 8, 0>), Gp, CB, lds, c->stream, PT_ARGS);
-    else if (epi == 1) SUCEM_LAUNCH((k_mf_patch<R, 2, 8, 1>), Gp, CB, lds, c->stream, PT_ARGS);
-    else SUCEM_LAUNCH((k_mf_patch<R, 2, 8, 2>), Gp, CB, lds, c->stream, PT_ARGS);
+    if (f.U == 128) launch_patch<R, 2, 8, 128>(c, sys, pm, k, mask, x, y, w, epi, rs, Gp, Gt);
+    else if (f.U == 192) launch_patch<R, 2, 8, 192>(c, sys, pm, k, mask, x, y, w, epi, rs, Gp, Gt);
+    else launch_patch<R, 2, 8, 256>(c, sys, pm, k, mask, x, y, w, epi, rs, Gp, Gt);
   } else {
-    if (epi == 0) SUCEM_LAUNCH((k_mf_patch<R, 1, 3, 0>), Gp, CB, lds, c->stream, PT_ARGS);
-    else if (epi == 1) SUCEM_LAUNCH((k_mf_patch<R, 1, 3, 1>), Gp, CB, lds, c->stream, PT_ARGS);
-    else SUCEM_LAUNCH((k_mf_patch<R, 1, 3, 2>), Gp, CB, lds, c->stream, PT_ARGS);
+    SUCEM_CHECK(f.U == 256, SUCEM_ERR_UNSUPPORTED, "order-1 patches hold 256 cells");
+    launch_patch<R, 1, 3, 256>(c, sys, pm, k, mask, x, y, w, epi, rs, Gp, Gt);
   }
-#undef PT_ARGS
-  const int npd = (int)s->pt_npd;
-#define PS_ARGS npd, Gg, Gt, Gp, s->pt_qptr.p, s->pt_qdof.p, (const C*)part, mask, (C*)y, \
+  const int npd = (int)pm.npd;
+#define PS_ARGS npd, Gg, Gt, Gp, pm.qptr.p, pm.qdof.p, (const C*)sys->pt_part.p, mask, (C*)y, \
                 (const C*)w, rs
   if (epi == 0) SUCEM_LAUNCH((k_mf_psum<R, 0>), Gg, TB, 0, c->stream, PS_ARGS);
   else if (epi == 1) SUCEM_LAUNCH((k_mf_psum<R, 1>), Gg, TB, 0, c->stream, PS_ARGS);

@@ -130,8 +130,10 @@ class DeviceProblem(object):
                     blocks=int(cfg[3])), ms.value
 
     def set_spmv(self, precision, matrix_free=0, lanes=16, loads=4, blocks=2048):
-        # matrix-free: lanes per gather row (1, 4); loads 1 = whole element, 2 = half rows
-        """Select the operator explicitly (see ``tune``)."""
+        """Select the operator explicitly (see ``tune``): matrix_free 0 (assembled CSR: lanes
+        per row, loads per lane), 1 (per-cell element products: lanes per gather row 0, 1, 4;
+        loads 1 = whole element, 2 = half the rows at a time) or 2 (patches: loads = cells
+        per patch, 128, 192 or 256); blocks = grid of the row / shared-DOF kernel."""
         cfg = np.array([matrix_free, lanes, loads, blocks], np.int32)
         _hip.call("sucem_system_set_spmv", self.ctx.handle, self.system.handle.h, int(precision),
                   _hip.ptr(cfg))
@@ -144,14 +146,39 @@ class DeviceProblem(object):
         _hip.call("sucem_system_apply", self.ctx.handle, self.system.handle.h, float(k0), precision,
                   0, nvec, _hip.ptr(x), _hip.ptr(y))
 
-    def operator_bytes(self, precision, matrix_free, epilogue=False):
+    def patch_info(self, cb):
+        """{patches, patch_dofs, shared_dofs, partial_sums, lds, face_rows} of the patch
+        operator with patches of ``cb`` cells."""
+        info = np.zeros(6, np.int64)
+        _hip.call("sucem_system_patch_info", self.ctx.handle, self.system.handle.h, int(cb),
+                  _hip.ptr(info))
+        return dict(zip(("patches", "patch_dofs", "shared_dofs", "partial_sums", "lds",
+                         "face_rows"), (int(v) for v in info)))
+
+    def operator_bytes(self, precision, matrix_free, epilogue=False, cb=256):
         """Algorithmic HBM bytes of one application of the fused operator (one vector);
-        ``epilogue``: plus the read of w for the fused dot product sum_r w_r y_r."""
+        ``epilogue``: plus the read of w for the fused dot product sum_r w_r y_r;
+        ``matrix_free`` 2 (patches of ``cb`` cells): per cell 12 coefficients and nloc
+        (patch-DOF index, product place) pairs, per patch DOF its id, segment end,
+        destination and x value, per shared DOF its id and segment pointer, the partial
+        sums written and read, per boundary face 6 coefficients, its cell and local face
+        and its products' places, y once."""
         if epilogue:
-            return (self.operator_bytes(precision, matrix_free)
+            return (self.operator_bytes(precision, matrix_free, cb=cb)
                     + self.ndofs * (8 if precision == 32 else 16))
         if not matrix_free:
             return self.spmv_bytes(precision)
+        if matrix_free == 2:
+            s = 4 if precision == 32 else 8
+            sx = 2 * s
+            nc = int(np.prod(self.n)) * 6
+            nbf = int(self.space.info()[4])
+            nloc, nfd = (6, 3) if self.order == 1 else (20, 8)
+            pi = self.patch_info(cb)
+            return (nc * (12 * s + nloc * 4) + pi["patch_dofs"] * (4 + 2 + 4 + sx)
+                    + pi["shared_dofs"] * 8 + 2 * pi["partial_sums"] * sx
+                    + nbf * (6 * s + 4 + nfd * 2) + 2 * (pi["patches"] + 1) * 4
+                    + self.ndofs * sx)
         s = 4 if precision == 32 else 8            # real value
         sx = 2 * s                                  # complex vector element
         n = self.ndofs

@@ -79,6 +79,7 @@ SIGNATURES = {
     "sucem_mesh_destroy": (ctypes.c_int, [_P]),
     "sucem_space_create": (ctypes.c_int, [_P, _P, ctypes.c_int, _PP]),
     "sucem_space_info": (ctypes.c_int, [_P, _P]),
+    "sucem_system_patch_info": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "sucem_space_export_pattern": (ctypes.c_int, [_P, _P, _P]),
     "sucem_space_export_dofmap": (ctypes.c_int, [_P, _P]),
     "sucem_space_export_entities": (ctypes.c_int, [_P, _P, _P]),

@@ -407,8 +407,8 @@ def test_patch_operator_matches_assembled(dev, order, dims, perturb):
     1e-12, FP32 to 2e-5 (relative to max |y|), with and without the PEC row mask, with and
     without the ABC term, on heterogeneous (eps_r, mu_r) meshes of one patch (4x3x3 boxes,
     216 cells) and of several (1260 and 1296 cells; one with interior vertices moved off
-    the grid).  Repeated applications and both grids of the shared-DOF kernel give
-    identical results."""
+    the grid), with patches of 256 cells and (order 2) of 128 and 192.  Repeated
+    applications and both grids of the shared-DOF kernel give identical results."""
     import torch
     from sucemfem import _hip
     _say("patch operator order %d dims %s perturb %g" % (order, dims, perturb))
@@ -430,7 +430,9 @@ def test_patch_operator_matches_assembled(dev, order, dims, perturb):
                 xd = torch.as_tensor(x, dtype=dt, device="cuda")
                 for pec in (0, 1):
                     ys = []
-                    for cfg in ([0, 8, 4, 64], [2, 0, 0, 8], [2, 0, 0, 64], [2, 0, 0, 64]):
+                    cbs = (256, 256) if order == 1 else (128, 192)
+                    for cfg in ([0, 8, 4, 64], [2, 0, 256, 8], [2, 0, 256, 64], [2, 0, 256, 64],
+                                [2, 0, cbs[0], 16], [2, 0, cbs[1], 64]):
                         cfg = np.array(cfg, np.int32)
                         _hip.call("sucem_system_set_spmv", V.ctx.handle, sysm.handle.h, prec,
                                   _hip.ptr(cfg))
@@ -439,7 +441,8 @@ def test_patch_operator_matches_assembled(dev, order, dims, perturb):
                                   pec, 2, _hip.ptr(xd), _hip.ptr(yd))
                         ys.append(yd.cpu().numpy().astype(np.complex128))
                     scale = np.abs(ys[0]).max()
-                    assert np.abs(ys[1] - ys[0]).max() <= tol * scale, (which, prec, pec)
+                    for i in (1, 4, 5):
+                        assert np.abs(ys[i] - ys[0]).max() <= tol * scale, (which, prec, pec, i)
                     assert np.array_equal(ys[1], ys[2]) and np.array_equal(ys[2], ys[3])
 
 
@@ -459,7 +462,7 @@ def test_matrix_free_solve_matches_direct(dev, precision):
     xs = []
     for mf in (0, 1, 2):
         for p in (precision, 64):
-            prob.set_spmv(p, matrix_free=mf, lanes=4, loads=2 if mf else 4, blocks=64)
+            prob.set_spmv(p, matrix_free=mf, lanes=4, loads=(4, 2, 192)[mf], blocks=64)
         x = torch.zeros_like(b)
         st = prob.solve(k0, b, x, precision=precision, tol=1e-9, max_iter=20000)
         assert st["status"] == 0 and st["rel_residual"] <= 1e-9, st
