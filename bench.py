@@ -90,6 +90,10 @@ def main():
                     help="skip the comparison solve in the other precision")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=200)
+    ap.add_argument("--operator", default="auto",
+                    help="auto (autotuned), assembled, cells (matrix-free per cell) or "
+                         "patch128 / patch192 / patch256 (matrix-free per patch of cells), "
+                         "for both precisions")
     ap.add_argument("--copy-probe-mb", type=int, default=256,
                     help="one plain device copy of this size after the SpMV timing: the "
                          "known-size kernel that scales HBM byte counters")
@@ -110,10 +114,18 @@ def main():
     setup_s = time.perf_counter() - t0
     log("setup %.2fs: N=%d nnz=%d nnz_abc=%d" % (setup_s, prob.ndofs, prob.nnz, prob.nnz0))
     tuned = {}
+    fixed = {"assembled": (0, 16, 4, 2048), "cells": (1, 0, 1, 2048),
+             "patch128": (2, 0, 128, 4096), "patch192": (2, 0, 192, 4096),
+             "patch256": (2, 0, 256, 4096)}
     for prec in (32, 64):
-        cfg, ms = prob.tune(prec)
+        if a.operator == "auto":
+            cfg, ms = prob.tune(prec)
+        else:
+            mf, lanes, loads, blocks = fixed[a.operator]
+            prob.set_spmv(prec, mf, lanes, loads, blocks)
+            cfg, ms = dict(matrix_free=mf, lanes=lanes, loads=loads, blocks=blocks), None
         tuned[prec] = dict(cfg, ms=ms)
-        log("tuned FP%d SpMV: %s %.4f ms" % (prec, cfg, ms))
+        log("FP%d operator: %s %s ms" % (prec, cfg, "%.4f" % ms if ms else "-"))
     b = prob.rhs_dipoles(prob.source_pos, prob.source_moment, prob.k0)
     x = torch.zeros_like(b)
     stream = torch.cuda.current_stream()
