@@ -46,6 +46,10 @@ enum {
 /* Jacobi plus the gradient-space correction G diag(G^T A G)^-1 G^T (driven / shifted
  * Nedelec systems, COCG only) */
 #define SUCEM_PRECOND_GRADIENT 2
+/* Auxiliary-space (Hiptmair-Xu) form: the gradient term plus the vector nodal term
+ * Pi diag(Pi^T A Pi)^-1 Pi^T, Pi the nodal interpolation of piecewise-linear vector fields
+ * (COCG only) */
+#define SUCEM_PRECOND_AUXILIARY 3
 
 typedef struct sucem_ctx sucem_ctx;
 typedef struct sucem_mesh sucem_mesh;

@@ -24,6 +24,11 @@ function                               pinned by
 ``solve_direct``                       order-2 solution fixtures (UMFPACK solves in the reference)
 ``cavity_eigs``                        analytic cavity modes (discretisation check), dense ``eigh``
                                        on the eliminated pencil (small), null-space dimension
+``discrete_gradient``                  S.G = 0; G applied to nodal values of a linear function
+                                       reproduces its gradient's DOFs
+``nodal_interpolation``                reproduction of linear (order 2) / a + b x x (order 1)
+                                       fields through the basis expansion
+``cocg_gradient`` / ``cocg_auxiliary`` convergence to ``solve_direct``'s solution
 =====================================  ===========================================================
 """
 from __future__ import division
@@ -783,6 +788,82 @@ def cocg_gradient(A, b, G, tol=1e-8, maxiter=10000, gdiag=None):
 
     def prec(r):
         return D * r + G @ (W * (Gt @ r))
+
+    nb = np.linalg.norm(b)
+    x = np.zeros_like(b, dtype=np.complex128)
+    r = np.asarray(b, np.complex128).copy()
+    z = prec(r)
+    p = z.copy()
+    rho = r @ z
+    hist = []
+    for it in range(maxiter):
+        q = A @ p
+        alpha = rho / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        z = prec(r)
+        rho_new = r @ z
+        p = z + (rho_new / rho) * p
+        rho = rho_new
+        hist.append(np.linalg.norm(r) / nb)
+        if hist[-1] <= tol:
+            break
+    return x, it + 1, np.array(hist)
+
+
+def nodal_interpolation(space):
+    """Pi: nodal values of a continuous piecewise-linear vector field (3 per vertex,
+    column 3 v + d) -> Nedelec DOFs, Pi[i, 3v + d] = lambda_v(x_i) (t_i)_d for the DOF i
+    with dual point x_i and physical tangent t_i = J t_ref (the FIAT nodal dual of
+    ``_dual_functionals``).  Not a reference function: the auxiliary space of the
+    device's Hiptmair-Xu preconditioner.  Pinned by reproduction: for a linear field
+    (order 2) or a field a + b x x (order 1), Pi applied to its vertex values is its
+    Nedelec interpolant, whose basis expansion equals the field everywhere.
+    """
+    duals = _dual_functionals(space.order)
+    cells = np.asarray(space.cells)
+    X = space.coords[cells]
+    J = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]], axis=2)
+    rows, cols, vals = [], [], []
+    for i, (pt, t) in enumerate(duals):
+        lam = np.array([1 - pt.sum(), pt[0], pt[1], pt[2]])
+        tp = J @ t
+        for v in range(4):
+            if abs(lam[v]) < 1e-14:
+                continue
+            for d in range(3):
+                rows.append(space.cell_dofs[:, i])
+                cols.append(3 * cells[:, v] + d)
+                vals.append(lam[v] * tp[:, d])
+    rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    shape = (space.ndofs, 3 * len(space.coords))
+    Pi = sp.coo_matrix((vals, (rows, cols)), shape=shape).tocsr()
+    cnt = sp.coo_matrix((np.ones(len(vals)), (rows, cols)), shape=shape).tocsr()
+    Pi.data = Pi.data / cnt.data          # every cell writes the same entry
+    return Pi
+
+
+def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000):
+    """COCG with the auxiliary-space preconditioner, FP64 (scipy CSR):
+    M^-1 r = D^-1 r + G diag(G^T A G)^-1 G^T r + Pi diag(Pi^T A Pi)^-1 Pi^T r
+    (zero diagonals dropped).  Returns ``(x, iterations, history)``."""
+    A = A.tocsr()
+    D = 1.0 / A.diagonal()
+
+    def inv(d):
+        w = np.zeros(len(d), np.complex128)
+        nz = d != 0
+        w[nz] = 1.0 / d[nz]
+        return w
+    W = inv(gradient_diag(A, G))
+    V = inv(gradient_diag(A, Pi))
+    G = G.tocsr().astype(np.complex128)
+    Gt = G.T.tocsr()
+    Pi = Pi.tocsr().astype(np.complex128)
+    Pit = Pi.T.tocsr()
+
+    def prec(r):
+        return D * r + G @ (W * (Gt @ r)) + Pi @ (V * (Pit @ r))
 
     nb = np.linalg.norm(b)
     x = np.zeros_like(b, dtype=np.complex128)

@@ -608,7 +608,7 @@ static SolveOpts convert(const sucem_solve_opts* o) {
   SUCEM_CHECK(s.tol > 0 && s.ru_delta > 0 && s.ru_delta < 1, SUCEM_ERR_INVALID,
               "tol must be > 0 and ru_delta in (0, 1)");
   SUCEM_CHECK(s.precond == SUCEM_PRECOND_NONE || s.precond == SUCEM_PRECOND_JACOBI ||
-                  s.precond == SUCEM_PRECOND_GRADIENT,
+                  s.precond == SUCEM_PRECOND_GRADIENT || s.precond == SUCEM_PRECOND_AUXILIARY,
               SUCEM_ERR_UNSUPPORTED, "unknown preconditioner");
   return s;
 }

@@ -9,6 +9,16 @@
 // term) almost untouched; the second term acts on exactly that subspace.  M is
 // symmetric, so COCG stays applicable.  A = cs S + ct T + i c0 S0 and
 // G^T S G = 0, so its diagonal is formed per solve from three stored diagonals.
+//
+// The auxiliary-space form adds the vector nodal term of the Hiptmair-Xu splitting:
+//
+//   M^-1 r = D^-1 r + G W G^T r + Pi V Pi^T r,   V = diag(Pi^T A Pi)^-1,
+//
+// Pi the nodal interpolation of continuous piecewise-linear vector fields into the
+// Nedelec space: Pi[i][(v, d)] = lambda_v(x_i) (t_i)_d for the DOF i with dual point x_i
+// and tangent t_i.  Jacobi and the gradient term leave the smooth non-gradient fields
+// (the large-scale part of curl-curl) to the Krylov method; Pi^T A Pi acts on them as a
+// vector Laplacian-like operator whose diagonal is a usable smoother.
 #include <cmath>
 #include <vector>
 
@@ -186,7 +196,7 @@ __global__ __launch_bounds__(TB) void k_grad_t(int64_t nv, int64_t nlag,
 }
 
 // z = dinv .* r + G w (masked rows zero); optional reduction of r.z and |r|^2
-template <typename R>
+template <typename R, bool NODAL>
 __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __restrict__ g_ptr,
                                                const uint32_t* __restrict__ g_pk, R inv_den,
                                                const typename C2<R>::T* __restrict__ w,
@@ -194,7 +204,11 @@ __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __re
                                                const typename C2<R>::T* __restrict__ r,
                                                typename C2<R>::T* __restrict__ z,
                                                const uint8_t* __restrict__ mask, int reduce,
-                                               RedSite rs) {
+                                               RedSite rs, int64_t nnz_pi,
+                                               const int32_t* __restrict__ pi_ptr,
+                                               const int32_t* __restrict__ pi_col,
+                                               const R* __restrict__ pi_val,
+                                               const typename C2<R>::T* __restrict__ u) {
   using C = typename C2<R>::T;
   const int lb = xcd_block(G);
   const int chunk = (n + G - 1) / G;
@@ -212,6 +226,17 @@ __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __re
     }
     const C di = cmul(dinv[i], ri);
     C zi = mk(di.x + inv_den * gw.x, di.y + inv_den * gw.y);
+    if constexpr (NODAL) {
+      for (int q = pi_ptr[i]; q < pi_ptr[i + 1]; ++q) {
+        const C* uv = u + 3 * (int64_t)pi_col[q];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const R g = pi_val[d * nnz_pi + q];
+          zi.x += g * uv[d].x;
+          zi.y += g * uv[d].y;
+        }
+      }
+    }
     if (mask && mask[i]) zi = mk(R(0), R(0));
     z[i] = zi;
     if (reduce) {
@@ -224,7 +249,214 @@ __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __re
   if (reduce) reduce_finalize<TB>(rs, G, d1, d2);
 }
 
+// ---- vector nodal term --------------------------------------------------------------
+// Contribution t = cell * npairs + q, q over the (local DOF i, cell vertex v) pairs with
+// lambda_v(x_i) != 0; key (DOF, vertex) or, transposed, (vertex, DOF).
+__global__ void k_nodal_keys(int64_t nc, int nloc, int npairs, const int32_t* pairs,
+                             const int32_t* cell_dofs, const int32_t* cells, int B, int transpose,
+                             uint64_t* keys, uint32_t* vals) {
+  const int64_t n = nc * npairs;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t / npairs;
+    const int q = (int)(t % npairs);
+    const uint64_t dof = (uint64_t)cell_dofs[c * nloc + pairs[2 * q]];
+    const uint64_t vert = (uint64_t)cells[4 * c + pairs[2 * q + 1]];
+    keys[t] = transpose ? ((vert << B) | dof) : ((dof << B) | vert);
+    vals[t] = (uint32_t)t;
+  }
+}
+
+// Value of each unique entry from its first contribution (the smallest contribution id,
+// in both orders, so Pi and Pi^T hold the same numbers): lambda * (J t_ref)_d, SoA over d.
+__global__ void k_nodal_vals(int64_t nnz, const int32_t* seg, const uint32_t* perm, int npairs,
+                             const int32_t* pairs, const double* plam, const double* dual_t,
+                             const double* xyz, const int32_t* cells, double* v64, float* v32) {
+  for (int64_t z = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; z < nnz;
+       z += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t t = perm[seg[z]];
+    const int64_t c = t / (uint32_t)npairs;
+    const int q = (int)(t % (uint32_t)npairs);
+    const int i = pairs[2 * q];
+    const double* x0 = xyz + 3 * (int64_t)cells[4 * c];
+    const double* x1 = xyz + 3 * (int64_t)cells[4 * c + 1];
+    const double* x2 = xyz + 3 * (int64_t)cells[4 * c + 2];
+    const double* x3 = xyz + 3 * (int64_t)cells[4 * c + 3];
+    const double* tr = dual_t + 3 * i;
+    for (int d = 0; d < 3; ++d) {
+      const double td = tr[0] * (x1[d] - x0[d]) + tr[1] * (x2[d] - x0[d]) + tr[2] * (x3[d] - x0[d]);
+      const double v = plam[q] * td;
+      v64[d * nnz + z] = v;
+      v32[d * nnz + z] = (float)v;
+    }
+  }
+}
+
+// One wavefront per vertex v: out[3 (3v + d) + {0,1,2}] = p^T {S, T, S0} p for the
+// column p = (v, d) of Pi over its (unmasked) support.
+__global__ __launch_bounds__(TB) void k_nodal_diag(int64_t nv, int64_t nnz, const int32_t* pt_ptr,
+                                                   const int32_t* pt_col, const double* pt_val,
+                                                   const int32_t* rp, const int32_t* col,
+                                                   const double2* st, const int32_t* s0p,
+                                                   const int32_t* s0c, const double* s0v,
+                                                   const uint8_t* mask, double* out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wave; v < nv; v += nwaves) {
+    const int a = pt_ptr[v], m = pt_ptr[v + 1] - a;
+    double sS[3] = {0, 0, 0}, sT[3] = {0, 0, 0}, s0[3] = {0, 0, 0};
+    for (int p = lane; p < m * m; p += 64) {
+      const int ii = p / m, kk = p - ii * m;
+      const int i = pt_col[a + ii], k = pt_col[a + kk];
+      if (mask && (mask[i] || mask[k])) continue;
+      double w[3];
+      for (int d = 0; d < 3; ++d) w[d] = pt_val[d * nnz + a + ii] * pt_val[d * nnz + a + kk];
+      const int pos = find_col(col, rp[i], rp[i + 1], k);
+      if (pos >= 0)
+        for (int d = 0; d < 3; ++d) {
+          sS[d] += w[d] * st[pos].x;
+          sT[d] += w[d] * st[pos].y;
+        }
+      if (s0p) {
+        const int q = find_col(s0c, s0p[i], s0p[i + 1], k);
+        if (q >= 0)
+          for (int d = 0; d < 3; ++d) s0[d] += w[d] * s0v[q];
+      }
+    }
+    for (int d = 0; d < 3; ++d)
+      for (int o = 32; o > 0; o >>= 1) {
+        sS[d] += __shfl_xor(sS[d], o);
+        sT[d] += __shfl_xor(sT[d], o);
+        s0[d] += __shfl_xor(s0[d], o);
+      }
+    if (lane == 0)
+      for (int d = 0; d < 3; ++d) {
+        out[3 * (3 * v + d)] = sS[d];
+        out[3 * (3 * v + d) + 1] = sT[d];
+        out[3 * (3 * v + d) + 2] = s0[d];
+      }
+  }
+}
+
+// u[3v + d] = vinv[3v + d] (Pi^T r)_(v, d): L lanes per vertex row.
+template <typename R, int L>
+__global__ __launch_bounds__(TB) void k_nodal_t(int64_t nv, int64_t nnz,
+                                                const int32_t* __restrict__ pt_ptr,
+                                                const int32_t* __restrict__ pt_col,
+                                                const R* __restrict__ pt_val,
+                                                const typename C2<R>::T* __restrict__ r,
+                                                const typename C2<R>::T* __restrict__ vinv,
+                                                typename C2<R>::T* __restrict__ u) {
+  using C = typename C2<R>::T;
+  const int lane = threadIdx.x % L;
+  const int64_t group = (blockIdx.x * (int64_t)TB + threadIdx.x) / L;
+  const int64_t ngroups = ((int64_t)gridDim.x * TB) / L;
+  for (int64_t vb = 0; vb < nv; vb += ngroups) {
+    const int64_t v = vb + group;
+    C acc[3];
+    for (int d = 0; d < 3; ++d) acc[d] = mk(R(0), R(0));
+    if (v < nv) {
+      const int q1 = pt_ptr[v + 1];
+      for (int q = pt_ptr[v] + lane; q < q1; q += L) {
+        const C rv = r[pt_col[q]];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const R g = pt_val[d * nnz + q];
+          acc[d].x += g * rv.x;
+          acc[d].y += g * rv.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int o = L / 2; o > 0; o >>= 1) {
+        acc[d].x += __shfl_xor(acc[d].x, o);
+        acc[d].y += __shfl_xor(acc[d].y, o);
+      }
+    if (lane == 0 && v < nv)
+      for (int d = 0; d < 3; ++d) u[3 * v + d] = cmul(vinv[3 * v + d], acc[d]);
+  }
+}
+
 }  // namespace
+
+void ensure_nodal(Context* c, Space* s) {
+  std::lock_guard<std::mutex> hold(s->lazy);
+  if (s->pi_ptr.p) return;
+  const RefData& R = ref_data(s->order);
+  const Mesh* m = s->mesh;
+  std::vector<int32_t> pairs;
+  std::vector<double> plam;
+  for (int i = 0; i < R.nloc; ++i)
+    for (int v = 0; v < 4; ++v)
+      if (R.dual_lam[4 * i + v] != 0.0) {
+        pairs.push_back(i);
+        pairs.push_back(v);
+        plam.push_back(R.dual_lam[4 * i + v]);
+      }
+  const int npairs = (int)plam.size();
+  const int64_t n = m->nc * npairs;
+  SUCEM_CHECK(n < (int64_t(1) << 31), SUCEM_ERR_UNSUPPORTED, "nodal interpolation too large");
+  DBuf<int32_t> pairs_d(pairs.size());
+  DBuf<double> plam_d(plam.size()), dual_t_d(R.dual_t.size());
+  h2d(c, pairs_d.p, pairs.data(), pairs.size());
+  h2d(c, plam_d.p, plam.data(), plam.size());
+  h2d(c, dual_t_d.p, R.dual_t.data(), R.dual_t.size());
+  const int B = key_bits(std::max<int64_t>(s->ndofs, m->nv));
+  for (int transpose = 0; transpose < 2; ++transpose) {
+    DBuf<uint64_t> keys(n);
+    DBuf<uint32_t> vals(n);
+    SUCEM_LAUNCH(k_nodal_keys, grid_for(n, TB), TB, 0, c->stream, m->nc, s->nloc, npairs,
+                 pairs_d.p, s->cell_dofs.p, m->cells.p, B, transpose, keys.p, vals.p);
+    DBuf<int32_t> seg;
+    DBuf<uint32_t> perm;
+    DBuf<int32_t>& ptr = transpose ? s->pit_ptr : s->pi_ptr;
+    DBuf<int32_t>& colb = transpose ? s->pit_col : s->pi_col;
+    DBuf<double>& v64 = transpose ? s->pit_val : s->pi_val;
+    DBuf<float>& v32 = transpose ? s->pit_val32 : s->pi_val32;
+    const int64_t nnz = build_csr_from_pairs(c, n, keys, vals, B, transpose ? m->nv : s->ndofs,
+                                             ptr, colb, seg, perm);
+    v64.alloc(3 * nnz);
+    v32.alloc(3 * nnz);
+    SUCEM_LAUNCH(k_nodal_vals, grid_for(nnz, TB), TB, 0, c->stream, nnz, seg.p, perm.p, npairs,
+                 pairs_d.p, plam_d.p, dual_t_d.p, m->xyz.p, m->cells.p, v64.p, v32.p);
+    s->nnz_pi = nnz;
+    c->sync();
+  }
+}
+
+const double* nodal_diag(Context* c, System* sys, bool masked) {
+  Space* s = sys->space;
+  ensure_nodal(c, s);
+  DBuf<double>& d = sys->ndiag[masked ? 1 : 0];
+  if (masked && sys->ndiag_pec_version != s->pec_version) d.release();
+  if (!d.p) {
+    if (masked) sys->ndiag_pec_version = s->pec_version;
+    SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S and T must be assembled");
+    const int64_t nv = s->mesh->nv;
+    d.alloc(9 * nv);
+    const bool has_s0 = (sys->which & SUCEM_MAT_S0) && s->nnz0 > 0;
+    SUCEM_LAUNCH(k_nodal_diag, grid_for(nv * 64, TB), TB, 0, c->stream, nv, s->nnz_pi,
+                 s->pit_ptr.p, s->pit_col.p, s->pit_val.p, s->row_ptr.p, s->col.p, sys->st64.p,
+                 has_s0 ? s->s0_ptr.p : nullptr, s->s0_col.p, sys->s0_64.p,
+                 masked ? s->pec.p : nullptr, d.p);
+    c->sync();
+  }
+  return d.p;
+}
+
+template <typename R>
+void nodal_winv(Context* c, System* sys, Coef k, bool masked, void* vinv) {
+  const double* d9 = nodal_diag(c, sys, masked);
+  using C = typename C2<R>::T;
+  const int64_t nn = 3 * sys->space->mesh->nv;
+  SUCEM_LAUNCH((k_grad_winv<C>), grid_for(nn, TB), TB, 0, c->stream, nn, d9, k.s, k.t, k.s0,
+               (C*)vinv);
+}
+template void nodal_winv<float>(Context*, System*, Coef, bool, void*);
+template void nodal_winv<double>(Context*, System*, Coef, bool, void*);
 
 void ensure_gradient(Context* c, Space* s) {
   std::lock_guard<std::mutex> hold(s->lazy);
@@ -317,20 +549,38 @@ template void grad_winv<double>(Context*, System*, Coef, bool, void*);
 // z = M^-1 r; with rs.mode != FIN_NONE the grid also reduces r.z and |r|^2.
 template <typename R>
 void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv, const void* r,
-                  void* w, void* z, const uint8_t* mask, int G, const RedSite& rs) {
+                  void* w, void* z, const uint8_t* mask, int G, const RedSite& rs,
+                  const void* vinv, void* u) {
   using C = typename C2<R>::T;
   const R inv_den = (R)(1.0 / s->g_den);
   const int Gt = op_grid(c, s->nlag, 16);               // multiple of 8
   SUCEM_LAUNCH((k_grad_t<R, 8, 4>), Gt, TB, 0, c->stream, (int64_t)s->mesh->nv, s->nlag,
                s->gt_ptr.p, s->gt_pk.p, inv_den, (const C*)r, (const C*)winv, (C*)w);
-  SUCEM_LAUNCH((k_grad_z<R>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p, s->g_pk.p,
-               inv_den, (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
-               rs.mode != FIN_NONE ? 1 : 0, rs);
+  const int reduce = rs.mode != FIN_NONE ? 1 : 0;
+  if (vinv) {
+    const R* pt_val;
+    const R* pi_val;
+    if constexpr (sizeof(R) == 4) { pt_val = s->pit_val32.p; pi_val = s->pi_val32.p; }
+    else { pt_val = s->pit_val.p; pi_val = s->pi_val.p; }
+    const int64_t nv = s->mesh->nv;
+    SUCEM_LAUNCH((k_nodal_t<R, 16>), op_grid(c, nv * 16 / 64, 16), TB, 0, c->stream, nv,
+                 s->nnz_pi, s->pit_ptr.p, s->pit_col.p, pt_val, (const C*)r, (const C*)vinv,
+                 (C*)u);
+    SUCEM_LAUNCH((k_grad_z<R, true>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p,
+                 s->g_pk.p, inv_den, (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
+                 reduce, rs, s->nnz_pi, s->pi_ptr.p, s->pi_col.p, pi_val, (const C*)u);
+  } else {
+    SUCEM_LAUNCH((k_grad_z<R, false>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p,
+                 s->g_pk.p, inv_den, (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
+                 reduce, rs, (int64_t)0, nullptr, nullptr, (const R*)nullptr, (const C*)nullptr);
+  }
   finish_reduction(c, rs, G);
 }
 template void grad_precond<float>(Context*, const Space*, const void*, const void*, const void*,
-                                  void*, void*, const uint8_t*, int, const RedSite&);
+                                  void*, void*, const uint8_t*, int, const RedSite&, const void*,
+                                  void*);
 template void grad_precond<double>(Context*, const Space*, const void*, const void*, const void*,
-                                   void*, void*, const uint8_t*, int, const RedSite&);
+                                   void*, void*, const uint8_t*, int, const RedSite&, const void*,
+                                   void*);
 
 }  // namespace sucem

@@ -78,6 +78,13 @@ struct Space {
   // integer k (the local gradient entries are k/3 for order 2, k for order 1)
   DBuf<uint32_t> g_pk, gt_pk;
   double g_den = 1.0;
+  // nodal interpolation Pi of P1 vector fields (N x 3 nv, built on first use), stored per
+  // (DOF, vertex) pair with its 3 components (SoA: val[d * nnz_pi + entry]), and Pi^T per
+  // (vertex, DOF) pair the same way
+  int64_t nnz_pi = 0;
+  DBuf<int32_t> pi_ptr, pi_col, pit_ptr, pit_col;
+  DBuf<double> pi_val, pit_val;
+  DBuf<float> pi_val32, pit_val32;
   // matrix-free operator (built on first use): DOF ids per cell as SoA; the products of
   // cells (cell * nloc + local) and boundary faces (face * nfd + face DOF) sorted by DOF:
   // segment pointers per DOF and each product's position in that order
@@ -110,6 +117,9 @@ struct System {
   // PEC rows removed (built on first use by the gradient preconditioner)
   DBuf<double> gdiag[2];
   int gdiag_pec_version = -1;     // Space::pec_version gdiag[1] was built with
+  // the same for the vector nodal term: diag(Pi^T {S, T, S0} Pi) per (vertex, component)
+  DBuf<double> ndiag[2];
+  int ndiag_pec_version = -1;
   Space* space = nullptr;
   unsigned which = 0;
   DBuf<double2> st64;
@@ -204,6 +214,7 @@ void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const Sol
 // hiptmair.hip: additive gradient-space preconditioner
 //   M^-1 r = D^-1 r + G (diag(G^T A G))^-1 G^T r
 void ensure_gradient(Context* c, Space* s);
+void ensure_nodal(Context* c, Space* s);
 const double* gradient_diag(Context* c, System* sys, bool masked);
 
 // batch.hip: nb driven systems (one k0 each) on one assembled system

@@ -280,7 +280,10 @@ struct Op {
   virtual int64_t nlag() { return 0; }
   virtual void winv_lo(void* winv) {}
   virtual void precond_lo(const void* winv, const void* dinv, const void* r, void* w, void* z,
-                          int G, const RedSite& rs) {}
+                          int G, const RedSite& rs, const void* vinv, void* u) {}
+  // vector nodal term of the auxiliary-space preconditioner (3 nv entries)
+  virtual int64_t nnod() { return 0; }
+  virtual void vinv_lo(void* vinv) {}
   virtual ~Op() {}
 };
 
@@ -308,8 +311,15 @@ struct SysOp : Op {
     grad_winv<R>(c, const_cast<System*>(sys), k, mask != nullptr, winv);
   }
   void precond_lo(const void* winv, const void* dinv, const void* r, void* w, void* z, int G,
-                  const RedSite& rs) override {
-    grad_precond<R>(c, sys->space, winv, dinv, r, w, z, mask, G, rs);
+                  const RedSite& rs, const void* vinv, void* u) override {
+    grad_precond<R>(c, sys->space, winv, dinv, r, w, z, mask, G, rs, vinv, u);
+  }
+  int64_t nnod() override {
+    ensure_nodal(c, sys->space);
+    return 3 * sys->space->mesh->nv;
+  }
+  void vinv_lo(void* vinv) override {
+    nodal_winv<R>(c, const_cast<System*>(sys), k, mask != nullptr, vinv);
   }
 };
 
@@ -382,12 +392,15 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   const int Gm = std::max(G, op.flo.parts());  // partial buffer size
   const bool lo = sizeof(R) == 4;
   const bool cocg = o.method == SUCEM_METHOD_COCG;
-  const bool grad = o.precond == SUCEM_PRECOND_GRADIENT;
+  const bool aux = o.precond == SUCEM_PRECOND_AUXILIARY;
+  const bool grad = o.precond == SUCEM_PRECOND_GRADIENT || aux;
   SUCEM_CHECK(!grad || cocg, SUCEM_ERR_UNSUPPORTED,
-              "the gradient preconditioner is available with COCG");
+              "the gradient and auxiliary-space preconditioners are available with COCG");
   const int64_t nl = grad ? op.nlag() : 0;
   SUCEM_CHECK(!grad || nl > 0, SUCEM_ERR_UNSUPPORTED,
-              "the gradient preconditioner needs an assembled Nedelec system");
+              "the gradient and auxiliary-space preconditioners need an assembled Nedelec "
+              "system");
+  const int64_t nn = aux ? op.nnod() : 0;
   auto t0 = std::chrono::steady_clock::now();
   st = SolveStats();
 
@@ -414,14 +427,15 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   DBuf<C> rt(cocg ? 0 : n), v(cocg ? 0 : n), phat(cocg ? 0 : n), s(cocg ? 0 : n),
       shat(cocg ? 0 : n), t(cocg ? 0 : n);
   C* xlo = lo ? xlo_buf.p : (C*)x;
-  DBuf<C> z(grad ? n : 0), w(nl), winv(nl);
+  DBuf<C> z(grad ? n : 0), w(nl), winv(nl), u(nn), vinv(nn);
   // z = M^-1 r, reducing r.z / |r|^2 into the site's scalars
   auto precondition = [&](const RedSite& rs) {
-    op.precond_lo(winv.p, dinv.p, r.p, w.p, z.p, G, rs);
+    op.precond_lo(winv.p, dinv.p, r.p, w.p, z.p, G, rs, aux ? vinv.p : nullptr, u.p);
   };
 
   op.dinv_lo(dinv.p);
   if (grad) op.winv_lo(winv.p);
+  if (aux) op.vinv_lo(vinv.p);
   SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, x);
   if (lo) SUCEM_LAUNCH((k_zero_c<C>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, xlo);
   SUCEM_LAUNCH(k_norm2, G, TB, 0, c->stream, n, G, b, op.mask, site(SITE_NORM, FIN_NORM));

@@ -48,9 +48,14 @@ void finish_reduction(Context* c, const RedSite& rs, int G);
 // gradient-space preconditioner (hiptmair.hip)
 template <typename R>
 void grad_winv(Context* c, System* sys, Coef k, bool masked, void* winv);
+// vinv (nullable): the auxiliary-space form, vinv = diag(Pi^T A Pi)^-1 (3 nv) from
+// nodal_winv, u a 3 nv work vector
 template <typename R>
 void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv, const void* r,
-                  void* w, void* z, const uint8_t* mask, int G, const RedSite& rs);
+                  void* w, void* z, const uint8_t* mask, int G, const RedSite& rs,
+                  const void* vinv = nullptr, void* u = nullptr);
+template <typename R>
+void nodal_winv(Context* c, System* sys, Coef k, bool masked, void* vinv);
 // Jacobi inverse diagonal (zero on masked rows).
 template <typename R>
 void sys_dinv(Context* c, const System* sys, Coef k, const uint8_t* mask, void* dinv);

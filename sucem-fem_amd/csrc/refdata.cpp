@@ -191,6 +191,18 @@ RefData build(int order) {
       }
     }
   }
+  R.dual_lam.assign(n * 4, 0.0);
+  R.dual_t.assign(n * 3, 0.0);
+  for (int i = 0; i < n; ++i) {
+    const double* p = duals[i].p;
+    R.dual_lam[4 * i] = 1.0 - p[0] - p[1] - p[2];
+    for (int k = 0; k < 3; ++k) {
+      R.dual_lam[4 * i + 1 + k] = p[k];
+      R.dual_t[3 * i + k] = duals[i].t[k];
+    }
+    for (int v = 0; v < 4; ++v)
+      if (std::fabs(R.dual_lam[4 * i + v]) < 1e-14) R.dual_lam[4 * i + v] = 0.0;
+  }
   std::vector<double> V(n * n);
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {

@@ -40,6 +40,9 @@ struct RefData {
   // edge order); the same on every cell (nloc x nlag).
   int nlag = 0;
   std::vector<double> grad;
+  // Nodal duals: dual_lam[i][v] = barycentric coordinate v of dual point i, dual_t[i][k]
+  // = its reference tangent (the physical tangent is J dual_t[i], J = [x1-x0 x2-x0 x3-x0]).
+  std::vector<double> dual_lam, dual_t;
 };
 
 const RefData& ref_data(int order);

@@ -86,6 +86,25 @@ class DeviceProblem(object):
                   0 if hist is None else len(hist))
         return st.as_dict()
 
+    def solve_coef(self, cs, ct, c0, b, x, precision=64, tol=1e-8, max_iter=50000,
+                   method=_hip.METHOD_COCG, check_every=16, ru_delta=0.01,
+                   precond=_hip.PRECOND_GRADIENT, eliminate_pec=False):
+        """Solve (cs S + ct T + i c0 S0) x = b (e.g. the shifted cavity operator with the
+        PEC DOFs eliminated)."""
+        import torch
+        self._check_vec(b, torch.complex128)
+        self._check_vec(x, torch.complex128)
+        o = _hip.SolveOpts()
+        _hip.load().sucem_solve_opts_default(ctypes.byref(o))
+        o.method, o.precision, o.tol, o.max_iter = method, precision, tol, max_iter
+        o.check_every, o.ru_delta, o.precond = check_every, ru_delta, precond
+        o.eliminate_pec = 1 if eliminate_pec else 0
+        st = _hip.SolveStats()
+        _hip.call("sucem_solve_system", self.ctx.handle, self.system.handle.h, float(cs),
+                  float(ct), float(c0), _hip.ptr(b), _hip.ptr(x), ctypes.byref(o),
+                  ctypes.byref(st), None, 0)
+        return st.as_dict()
+
     def solve_batch(self, k0s, B, X, tol=1e-8, max_iter=50000, check_every=16):
         """Solve len(k0s) systems at once: B, X are (nb, N) complex128 device tensors."""
         import torch

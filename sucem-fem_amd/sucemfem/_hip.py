@@ -16,7 +16,7 @@ ERR_INVALID, ERR_HIP, ERR_OOM, ERR_OUTSIDE, ERR_MATERIAL, ERR_UNSUPPORTED, ERR_I
     1, 2, 3, 4, 5, 6, 7)
 MAT_S, MAT_T, MAT_S0 = 1, 2, 4
 METHOD_COCG, METHOD_BICGSTAB = 0, 1
-PRECOND_NONE, PRECOND_JACOBI, PRECOND_GRADIENT = 0, 1, 2
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_GRADIENT, PRECOND_AUXILIARY = 0, 1, 2, 3
 
 
 class SolveOpts(ctypes.Structure):

@@ -363,6 +363,57 @@ def test_gradient_preconditioner_iterations_match_oracle(dev):
     assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) < 1e-7
 
 
+@pytest.mark.parametrize("order,pec", [(2, False), (1, False), (2, True)])
+def test_auxiliary_preconditioner_iterations_match_oracle(dev, order, pec):
+    """The FP64 device solve with the auxiliary-space preconditioner (gradient + vector
+    nodal terms) follows the oracle's auxiliary COCG: same iteration count up to rounding
+    (a wrong Pi, diag(Pi^T A Pi) or nodal kernel would change the count) and the same
+    solution; the FP32+IR solve reaches it too.  With PEC walls the shifted cavity
+    operator S - sigma T is solved on the eliminated DOFs."""
+    import torch
+    from sucemfem.Utilities.Workloads import DeviceProblem
+    from sucemfem import _hip
+    _say("auxiliary preconditioner order %d pec %s" % (order, pec))
+    which = _hip.MAT_S | _hip.MAT_T | (0 if pec else _hip.MAT_S0)
+    prob = DeviceProblem(4, 0.1, order, which, (1.0, 4.4), seed=1)
+    co, ce = O.unit_cube(4, 4, 4)
+    osp = O.Space(co * 0.1, ce, order)
+    eps = np.array([1.0, 4.4])[prob.region]
+    rng = np.random.default_rng(2)
+    if pec:
+        sigma = 1.1 * (np.pi / 0.1) ** 2
+        m = O.assemble_fast(osp, eps_r=eps, abc=False)
+        free = np.flatnonzero(~osp.pec_mask())
+        A = (m["S"] - sigma * m["M"]).tocsr()[free][:, free].astype(np.complex128)
+        bh = np.zeros(osp.ndofs, np.complex128)
+        bh[free] = rng.standard_normal(len(free)) + 1j * rng.standard_normal(len(free))
+        G = O.discrete_gradient(osp).tocsr()[free]
+        Pi = O.nodal_interpolation(osp).tocsr()[free]
+        cs, ct, c0 = 1.0, -sigma, 0.0
+    else:
+        k0 = 2 * np.pi * 1e9 / O.c0
+        A = O.driven_lhs(O.assemble_fast(osp, eps_r=eps), 1e9)
+        bh = rng.standard_normal(osp.ndofs) + 1j * rng.standard_normal(osp.ndofs)
+        G, Pi = O.discrete_gradient(osp), O.nodal_interpolation(osp)
+        free = np.arange(osp.ndofs)
+        cs, ct, c0 = 1.0, -k0 * k0, k0
+    xo, its, _ = O.cocg_auxiliary(A, bh[free], G, Pi, tol=1e-9, maxiter=20000)
+    b = torch.as_tensor(bh, device="cuda")
+    for precision in (64, 32):
+        x = torch.zeros_like(b)
+        st = prob.solve_coef(cs, ct, c0, b, x, precision=precision, tol=1e-9, max_iter=20000,
+                             check_every=1, precond=_hip.PRECOND_AUXILIARY,
+                             eliminate_pec=pec)
+        assert st["status"] == 0 and st["rel_residual"] <= 1e-9, (precision, st)
+        xs = x.cpu().numpy()
+        assert np.linalg.norm(xs[free] - xo) / np.linalg.norm(xo) < 1e-6, precision
+        if precision == 64:
+            # the shifted cavity operator is indefinite: relative perturbations of 1e-14 ..
+            # 1e-12 in A move the oracle's own count by 1 .. 4 of ~295, and the device sums
+            # in another order, so the count is compared to 5 %
+            assert abs(st["iterations"] - its) <= max(4, its // 20), (st["iterations"], its)
+
+
 @pytest.mark.parametrize("order", [1, 2])
 def test_matrix_free_operator_matches_assembled(dev, order):
     """The matrix-free operator (element products + row gather) equals the assembled fused
