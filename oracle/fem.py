@@ -843,10 +843,11 @@ def nodal_interpolation(space):
     return Pi
 
 
-def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000):
+def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000, gdiag=None, vdiag=None):
     """COCG with the auxiliary-space preconditioner, FP64 (scipy CSR):
     M^-1 r = D^-1 r + G diag(G^T A G)^-1 G^T r + Pi diag(Pi^T A Pi)^-1 Pi^T r
-    (zero diagonals dropped).  Returns ``(x, iterations, history)``."""
+    (zero diagonals dropped; ``gdiag`` / ``vdiag`` precomputed diagonals).  Returns
+    ``(x, iterations, history)``."""
     A = A.tocsr()
     D = 1.0 / A.diagonal()
 
@@ -855,8 +856,8 @@ def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000):
         nz = d != 0
         w[nz] = 1.0 / d[nz]
         return w
-    W = inv(gradient_diag(A, G))
-    V = inv(gradient_diag(A, Pi))
+    W = inv(gradient_diag(A, G) if gdiag is None else gdiag)
+    V = inv(gradient_diag(A, Pi) if vdiag is None else vdiag)
     G = G.tocsr().astype(np.complex128)
     Gt = G.T.tocsr()
     Pi = Pi.tocsr().astype(np.complex128)

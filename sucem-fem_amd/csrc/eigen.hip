@@ -203,7 +203,7 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
   SolveOpts so;
   so.method = SUCEM_METHOD_COCG;
   so.precision = o.inner_precision;
-  so.precond = SUCEM_PRECOND_GRADIENT;
+  so.precond = SUCEM_PRECOND_AUXILIARY;
   so.max_iter = o.inner_max_iter;
   so.check_every = 8;
   so.tol = o.inner_tol;

@@ -196,6 +196,44 @@ __global__ __launch_bounds__(TB) void k_grad_t(int64_t nv, int64_t nlag,
 }
 
 // z = dinv .* r + G w (masked rows zero); optional reduction of r.z and |r|^2
+// Pi applied without stored values: DOF i of edge e = (a, b), a < b, at 1/3 (k = 0) or
+// 2/3 (k = 1) from a: Pi_i u = (x_b - x_a) . (lambda_a u_a + lambda_b u_b), lambda_b =
+// (k + 1) / 3 (order 1: the midpoint, 1/2 each); DOF i of face f = (p, q, r), p < q < r,
+// at its centroid: Pi_i u = (x_q - x_p or x_r - x_p) . (u_p + u_q + u_r) / 3.
+template <typename R, int ORDER>
+__device__ __forceinline__ typename C2<R>::T nodal_row(int i, int nE, const int32_t* edges,
+                                                       const int32_t* faces, const R* xyz,
+                                                       const typename C2<R>::T* u) {
+  using C = typename C2<R>::T;
+  C s = mk(R(0), R(0));
+  const int ne_dofs = ORDER == 2 ? 2 * nE : nE;
+  if (i < ne_dofs) {
+    const int e = ORDER == 2 ? i >> 1 : i;
+    const int a = edges[2 * e], b = edges[2 * e + 1];
+    const R lb = ORDER == 2 ? R((i & 1) + 1) * R(1.0 / 3.0) : R(0.5);
+    const R la = R(1) - lb;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const R t = xyz[3 * b + d] - xyz[3 * a + d];
+      const C ua = u[3 * a + d], ub = u[3 * b + d];
+      s.x += t * (la * ua.x + lb * ub.x);
+      s.y += t * (la * ua.y + lb * ub.y);
+    }
+  } else {
+    const int f = (i - ne_dofs) >> 1;
+    const int p = faces[3 * f], q = faces[3 * f + 1], r = faces[3 * f + 2];
+    const int o = (i & 1) ? r : q;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const R t = (xyz[3 * o + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
+      const C up = u[3 * p + d], uq = u[3 * q + d], ur = u[3 * r + d];
+      s.x += t * (up.x + uq.x + ur.x);
+      s.y += t * (up.y + uq.y + ur.y);
+    }
+  }
+  return s;
+}
+
 template <typename R, bool NODAL>
 __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __restrict__ g_ptr,
                                                const uint32_t* __restrict__ g_pk, R inv_den,
@@ -204,10 +242,10 @@ __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __re
                                                const typename C2<R>::T* __restrict__ r,
                                                typename C2<R>::T* __restrict__ z,
                                                const uint8_t* __restrict__ mask, int reduce,
-                                               RedSite rs, int64_t nnz_pi,
-                                               const int32_t* __restrict__ pi_ptr,
-                                               const int32_t* __restrict__ pi_col,
-                                               const R* __restrict__ pi_val,
+                                               RedSite rs, int order, int nE,
+                                               const int32_t* __restrict__ edges,
+                                               const int32_t* __restrict__ faces,
+                                               const R* __restrict__ xyz,
                                                const typename C2<R>::T* __restrict__ u) {
   using C = typename C2<R>::T;
   const int lb = xcd_block(G);
@@ -227,15 +265,10 @@ __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __re
     const C di = cmul(dinv[i], ri);
     C zi = mk(di.x + inv_den * gw.x, di.y + inv_den * gw.y);
     if constexpr (NODAL) {
-      for (int q = pi_ptr[i]; q < pi_ptr[i + 1]; ++q) {
-        const C* uv = u + 3 * (int64_t)pi_col[q];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const R g = pi_val[d * nnz_pi + q];
-          zi.x += g * uv[d].x;
-          zi.y += g * uv[d].y;
-        }
-      }
+      const C pu = order == 2 ? nodal_row<R, 2>(i, nE, edges, faces, xyz, u)
+                              : nodal_row<R, 1>(i, nE, edges, faces, xyz, u);
+      zi.x += pu.x;
+      zi.y += pu.y;
     }
     if (mask && mask[i]) zi = mk(R(0), R(0));
     z[i] = zi;
@@ -267,11 +300,10 @@ __global__ void k_nodal_keys(int64_t nc, int nloc, int npairs, const int32_t* pa
   }
 }
 
-// Value of each unique entry from its first contribution (the smallest contribution id,
-// in both orders, so Pi and Pi^T hold the same numbers): lambda * (J t_ref)_d, SoA over d.
+// Value of each unique entry from its first contribution: lambda * (J t_ref)_d, SoA over d.
 __global__ void k_nodal_vals(int64_t nnz, const int32_t* seg, const uint32_t* perm, int npairs,
                              const int32_t* pairs, const double* plam, const double* dual_t,
-                             const double* xyz, const int32_t* cells, double* v64, float* v32) {
+                             const double* xyz, const int32_t* cells, double* v64) {
   for (int64_t z = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; z < nnz;
        z += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t t = perm[seg[z]];
@@ -287,7 +319,6 @@ __global__ void k_nodal_vals(int64_t nnz, const int32_t* seg, const uint32_t* pe
       const double td = tr[0] * (x1[d] - x0[d]) + tr[1] * (x2[d] - x0[d]) + tr[2] * (x3[d] - x0[d]);
       const double v = plam[q] * td;
       v64[d * nnz + z] = v;
-      v32[d * nnz + z] = (float)v;
     }
   }
 }
@@ -339,52 +370,100 @@ __global__ __launch_bounds__(TB) void k_nodal_diag(int64_t nv, int64_t nnz, cons
   }
 }
 
-// u[3v + d] = vinv[3v + d] (Pi^T r)_(v, d): L lanes per vertex row.
-template <typename R, int L>
-__global__ __launch_bounds__(TB) void k_nodal_t(int64_t nv, int64_t nnz,
-                                                const int32_t* __restrict__ pt_ptr,
-                                                const int32_t* __restrict__ pt_col,
-                                                const R* __restrict__ pt_val,
+// u[3v + d] = vinv[3v + d] (Pi^T r)_(v, d), L lanes per vertex: the edges at v (entries
+// (e << 1) | [v is the edge's second vertex]) and the faces at v (entries f << 2); the
+// values are formed from the coordinates as in nodal_row.
+template <typename R, int L, int ORDER>
+__global__ __launch_bounds__(TB) void k_nodal_t(int64_t nv, int nE,
+                                                const int32_t* __restrict__ ve_ptr,
+                                                const int32_t* __restrict__ ve_ent,
+                                                const int32_t* __restrict__ vf_ptr,
+                                                const int32_t* __restrict__ vf_ent,
+                                                const int32_t* __restrict__ edges,
+                                                const int32_t* __restrict__ faces,
+                                                const R* __restrict__ xyz,
                                                 const typename C2<R>::T* __restrict__ r,
                                                 const typename C2<R>::T* __restrict__ vinv,
                                                 typename C2<R>::T* __restrict__ u) {
   using C = typename C2<R>::T;
   const int lane = threadIdx.x % L;
-  const int64_t group = (blockIdx.x * (int64_t)TB + threadIdx.x) / L;
-  const int64_t ngroups = ((int64_t)gridDim.x * TB) / L;
-  for (int64_t vb = 0; vb < nv; vb += ngroups) {
-    const int64_t v = vb + group;
-    C acc[3];
-    for (int d = 0; d < 3; ++d) acc[d] = mk(R(0), R(0));
-    if (v < nv) {
-      const int q1 = pt_ptr[v + 1];
-      for (int q = pt_ptr[v] + lane; q < q1; q += L) {
-        const C rv = r[pt_col[q]];
+  const int64_t v = (blockIdx.x * (int64_t)TB + threadIdx.x) / L;
+  C acc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) acc[d] = mk(R(0), R(0));
+  if (v < nv) {
+    for (int q = ve_ptr[v] + lane; q < ve_ptr[v + 1]; q += L) {
+      const int ent = ve_ent[q];
+      const int e = ent >> 1;
+      const int a = edges[2 * e], b = edges[2 * e + 1];
+      C sr;
+      if constexpr (ORDER == 2) {
+        // lambda_v at the two DOF points: (2/3, 1/3) at a, (1/3, 2/3) at b
+        const C r0 = r[2 * e], r1 = r[2 * e + 1];
+        const R l0 = (ent & 1) ? R(1.0 / 3.0) : R(2.0 / 3.0), l1 = R(1) - l0;
+        sr = mk(l0 * r0.x + l1 * r1.x, l0 * r0.y + l1 * r1.y);
+      } else {
+        const C r0 = r[e];
+        sr = mk(R(0.5) * r0.x, R(0.5) * r0.y);
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const R t = xyz[3 * b + d] - xyz[3 * a + d];
+        acc[d].x += t * sr.x;
+        acc[d].y += t * sr.y;
+      }
+    }
+    if constexpr (ORDER == 2) {
+      for (int q = vf_ptr[v] + lane; q < vf_ptr[v + 1]; q += L) {
+        const int f = vf_ent[q] >> 2;
+        const int p = faces[3 * f], fq = faces[3 * f + 1], fr = faces[3 * f + 2];
+        const C r0 = r[2 * nE + 2 * f], r1 = r[2 * nE + 2 * f + 1];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const R g = pt_val[d * nnz + q];
-          acc[d].x += g * rv.x;
-          acc[d].y += g * rv.y;
+          const R t0 = (xyz[3 * fq + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
+          const R t1 = (xyz[3 * fr + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
+          acc[d].x += t0 * r0.x + t1 * r1.x;
+          acc[d].y += t0 * r0.y + t1 * r1.y;
         }
       }
     }
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int o = L / 2; o > 0; o >>= 1) {
-        acc[d].x += __shfl_xor(acc[d].x, o);
-        acc[d].y += __shfl_xor(acc[d].y, o);
-      }
-    if (lane == 0 && v < nv)
-      for (int d = 0; d < 3; ++d) u[3 * v + d] = cmul(vinv[3 * v + d], acc[d]);
   }
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) {
+      acc[d].x += __shfl_xor(acc[d].x, o);
+      acc[d].y += __shfl_xor(acc[d].y, o);
+    }
+  if (lane == 0 && v < nv)
+    for (int d = 0; d < 3; ++d) u[3 * v + d] = cmul(vinv[3 * v + d], acc[d]);
+}
+
+// vertex -> entity adjacency entries: edges (v, (e << 1) | role), faces (v, f << 2 | role)
+__global__ void k_adj_keys(int64_t nent, int nvper, const int32_t* ent_verts, int B,
+                           int shift, uint64_t* keys, uint32_t* vals) {
+  const int64_t n = nent * nvper;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = t / nvper;
+    const int role = (int)(t % nvper);
+    const uint64_t code = ((uint64_t)e << shift) | (uint64_t)role;
+    keys[t] = ((uint64_t)ent_verts[t] << B) | code;
+    vals[t] = (uint32_t)t;
+  }
+}
+
+__global__ void k_xyz32(int64_t n, const double* a, float* b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
 }
 
 }  // namespace
 
 void ensure_nodal(Context* c, Space* s) {
   std::lock_guard<std::mutex> hold(s->lazy);
-  if (s->pi_ptr.p) return;
+  if (s->pit_ptr.p) return;
   const RefData& R = ref_data(s->order);
   const Mesh* m = s->mesh;
   std::vector<int32_t> pairs;
@@ -405,26 +484,48 @@ void ensure_nodal(Context* c, Space* s) {
   h2d(c, plam_d.p, plam.data(), plam.size());
   h2d(c, dual_t_d.p, R.dual_t.data(), R.dual_t.size());
   const int B = key_bits(std::max<int64_t>(s->ndofs, m->nv));
-  for (int transpose = 0; transpose < 2; ++transpose) {
+  {
+    // Pi^T with stored FP64 values (for diag(Pi^T A Pi) only)
     DBuf<uint64_t> keys(n);
     DBuf<uint32_t> vals(n);
     SUCEM_LAUNCH(k_nodal_keys, grid_for(n, TB), TB, 0, c->stream, m->nc, s->nloc, npairs,
-                 pairs_d.p, s->cell_dofs.p, m->cells.p, B, transpose, keys.p, vals.p);
+                 pairs_d.p, s->cell_dofs.p, m->cells.p, B, 1, keys.p, vals.p);
     DBuf<int32_t> seg;
     DBuf<uint32_t> perm;
-    DBuf<int32_t>& ptr = transpose ? s->pit_ptr : s->pi_ptr;
-    DBuf<int32_t>& colb = transpose ? s->pit_col : s->pi_col;
-    DBuf<double>& v64 = transpose ? s->pit_val : s->pi_val;
-    DBuf<float>& v32 = transpose ? s->pit_val32 : s->pi_val32;
-    const int64_t nnz = build_csr_from_pairs(c, n, keys, vals, B, transpose ? m->nv : s->ndofs,
-                                             ptr, colb, seg, perm);
-    v64.alloc(3 * nnz);
-    v32.alloc(3 * nnz);
+    const int64_t nnz = build_csr_from_pairs(c, n, keys, vals, B, m->nv, s->pit_ptr, s->pit_col,
+                                             seg, perm);
+    s->pit_val.alloc(3 * nnz);
     SUCEM_LAUNCH(k_nodal_vals, grid_for(nnz, TB), TB, 0, c->stream, nnz, seg.p, perm.p, npairs,
-                 pairs_d.p, plam_d.p, dual_t_d.p, m->xyz.p, m->cells.p, v64.p, v32.p);
+                 pairs_d.p, plam_d.p, dual_t_d.p, m->xyz.p, m->cells.p, s->pit_val.p);
     s->nnz_pi = nnz;
     c->sync();
   }
+  // vertex -> edge / face adjacency and FP32 coordinates for the matrix-free Pi and Pi^T
+  for (int kind = 0; kind < (s->order == 2 ? 2 : 1); ++kind) {
+    const int64_t nent = kind == 0 ? s->nE : s->nF;
+    const int nvper = kind == 0 ? 2 : 3;
+    const int shift = kind == 0 ? 1 : 2;
+    const int Bk = key_bits(std::max<int64_t>(nent << shift, m->nv));
+    const int64_t na = nent * nvper;
+    DBuf<uint64_t> keys(na);
+    DBuf<uint32_t> vals(na);
+    SUCEM_LAUNCH(k_adj_keys, grid_for(na, TB), TB, 0, c->stream, nent, nvper,
+                 kind == 0 ? s->edges.p : s->faces.p, Bk, shift, keys.p, vals.p);
+    DBuf<int32_t> seg;
+    DBuf<uint32_t> perm;
+    build_csr_from_pairs(c, na, keys, vals, Bk, m->nv, kind == 0 ? s->ve_ptr : s->vf_ptr,
+                         kind == 0 ? s->ve_ent : s->vf_ent, seg, perm);
+    c->sync();
+  }
+  if (s->order == 1) {
+    s->vf_ptr.alloc(m->nv + 1);
+    SUCEM_HIP(hipMemsetAsync(s->vf_ptr.p, 0, (m->nv + 1) * sizeof(int32_t), c->stream));
+    s->vf_ent.alloc(1);
+  }
+  s->xyz32.alloc(3 * m->nv);
+  SUCEM_LAUNCH(k_xyz32, grid_for(3 * m->nv, TB), TB, 0, c->stream, 3 * m->nv, m->xyz.p,
+               s->xyz32.p);
+  c->sync();
 }
 
 const double* nodal_diag(Context* c, System* sys, bool masked) {
@@ -558,21 +659,26 @@ void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv
                s->gt_ptr.p, s->gt_pk.p, inv_den, (const C*)r, (const C*)winv, (C*)w);
   const int reduce = rs.mode != FIN_NONE ? 1 : 0;
   if (vinv) {
-    const R* pt_val;
-    const R* pi_val;
-    if constexpr (sizeof(R) == 4) { pt_val = s->pit_val32.p; pi_val = s->pi_val32.p; }
-    else { pt_val = s->pit_val.p; pi_val = s->pi_val.p; }
+    const R* xyz;
+    if constexpr (sizeof(R) == 4) xyz = s->xyz32.p; else xyz = s->mesh->xyz.p;
     const int64_t nv = s->mesh->nv;
-    SUCEM_LAUNCH((k_nodal_t<R, 16>), op_grid(c, nv * 16 / 64, 16), TB, 0, c->stream, nv,
-                 s->nnz_pi, s->pit_ptr.p, s->pit_col.p, pt_val, (const C*)r, (const C*)vinv,
-                 (C*)u);
+    constexpr int L = 16;
+    const int Gn = (int)((nv * L + TB - 1) / TB);
+    if (s->order == 2)
+      SUCEM_LAUNCH((k_nodal_t<R, L, 2>), Gn, TB, 0, c->stream, nv, (int)s->nE, s->ve_ptr.p,
+                   s->ve_ent.p, s->vf_ptr.p, s->vf_ent.p, s->edges.p, s->faces.p, xyz,
+                   (const C*)r, (const C*)vinv, (C*)u);
+    else
+      SUCEM_LAUNCH((k_nodal_t<R, L, 1>), Gn, TB, 0, c->stream, nv, (int)s->nE, s->ve_ptr.p,
+                   s->ve_ent.p, s->vf_ptr.p, s->vf_ent.p, s->edges.p, s->faces.p, xyz,
+                   (const C*)r, (const C*)vinv, (C*)u);
     SUCEM_LAUNCH((k_grad_z<R, true>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p,
                  s->g_pk.p, inv_den, (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
-                 reduce, rs, s->nnz_pi, s->pi_ptr.p, s->pi_col.p, pi_val, (const C*)u);
+                 reduce, rs, s->order, (int)s->nE, s->edges.p, s->faces.p, xyz, (const C*)u);
   } else {
     SUCEM_LAUNCH((k_grad_z<R, false>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p,
                  s->g_pk.p, inv_den, (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
-                 reduce, rs, (int64_t)0, nullptr, nullptr, (const R*)nullptr, (const C*)nullptr);
+                 reduce, rs, 0, 0, nullptr, nullptr, (const R*)nullptr, (const C*)nullptr);
   }
   finish_reduction(c, rs, G);
 }

@@ -78,13 +78,17 @@ struct Space {
   // integer k (the local gradient entries are k/3 for order 2, k for order 1)
   DBuf<uint32_t> g_pk, gt_pk;
   double g_den = 1.0;
-  // nodal interpolation Pi of P1 vector fields (N x 3 nv, built on first use), stored per
-  // (DOF, vertex) pair with its 3 components (SoA: val[d * nnz_pi + entry]), and Pi^T per
-  // (vertex, DOF) pair the same way
+  // nodal interpolation Pi of P1 vector fields (N x 3 nv, built on first use): Pi^T stored
+  // per (vertex, DOF) pair with its 3 components (SoA: val[d * nnz_pi + entry]) for the
+  // set-up of diag(Pi^T A Pi); the preconditioner applies Pi and Pi^T without stored values
+  // from these: vertex ->
+  // edge entries (e << 1) | [second vertex], vertex -> face entries f << 2, coordinates
+  // rounded to FP32 for the FP32 kernels
   int64_t nnz_pi = 0;
-  DBuf<int32_t> pi_ptr, pi_col, pit_ptr, pit_col;
-  DBuf<double> pi_val, pit_val;
-  DBuf<float> pi_val32, pit_val32;
+  DBuf<int32_t> pit_ptr, pit_col;
+  DBuf<double> pit_val;
+  DBuf<int32_t> ve_ptr, ve_ent, vf_ptr, vf_ent;
+  DBuf<float> xyz32;
   // matrix-free operator (built on first use): DOF ids per cell as SoA; the products of
   // cells (cell * nloc + local) and boundary faces (face * nfd + face DOF) sorted by DOF:
   // segment pointers per DOF and each product's position in that order

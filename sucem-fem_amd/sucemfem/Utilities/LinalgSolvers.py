@@ -5,8 +5,9 @@ iterations run in libsucem_hip: a matrix produced by
 ``DrivenProblemABC.get_LHS_matrix`` is solved with the fused S/T/S0 operator,
 any other matrix is uploaded as a complex CSR.  Preconditioners: ``None``,
 ``'diagonal'`` (Jacobi), ``'gradient'`` (Jacobi plus the discrete-gradient
-correction, COCG on a driven system only) and ``'ilu'``.  The device has no
-incomplete-LU factorisation: ``'ilu'`` selects the gradient preconditioner for a
+correction, COCG on a driven system only), ``'auxiliary'`` (the gradient form plus the
+vector nodal correction, COCG on a driven system only) and ``'ilu'``.  The device has no
+incomplete-LU factorisation: ``'ilu'`` selects the auxiliary-space preconditioner for a
 driven system solved by COCG and Jacobi otherwise, with a warning.  Tolerances default to
 SciPy's (relative residual 1e-5, at most 10*N iterations) as in the reference,
 which passes none.
@@ -93,12 +94,14 @@ class SystemSolverBase(object):
             self._precond = _hip.PRECOND_JACOBI
         elif M_type.lower() == 'gradient':
             self._precond = _hip.PRECOND_GRADIENT
+        elif M_type.lower() == 'auxiliary':
+            self._precond = _hip.PRECOND_AUXILIARY
         elif M_type.lower() == 'ilu':
             driven = getattr(self._A, "device_operator", None) is not None
             if driven and self.method == _hip.METHOD_COCG:
-                warnings.warn("incomplete LU is not available on the device; using the gradient "
-                              "preconditioner")
-                self._precond = _hip.PRECOND_GRADIENT
+                warnings.warn("incomplete LU is not available on the device; using the "
+                              "auxiliary-space preconditioner")
+                self._precond = _hip.PRECOND_AUXILIARY
             else:
                 warnings.warn("incomplete LU is not available on the device; using Jacobi "
                               "('diagonal')")

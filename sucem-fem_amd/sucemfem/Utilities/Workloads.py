@@ -70,7 +70,7 @@ class DeviceProblem(object):
                              % (dtype, nvec * self.ndofs, t.dtype, t.numel()))
 
     def solve(self, k0, b, x, precision=64, tol=1e-8, max_iter=50000, method=_hip.METHOD_COCG,
-              check_every=16, ru_delta=0.01, hist=None, precond=_hip.PRECOND_GRADIENT):
+              check_every=16, ru_delta=0.01, hist=None, precond=_hip.PRECOND_AUXILIARY):
         import torch
         self._check_vec(b, torch.complex128)
         self._check_vec(x, torch.complex128)
@@ -88,7 +88,7 @@ class DeviceProblem(object):
 
     def solve_coef(self, cs, ct, c0, b, x, precision=64, tol=1e-8, max_iter=50000,
                    method=_hip.METHOD_COCG, check_every=16, ru_delta=0.01,
-                   precond=_hip.PRECOND_GRADIENT, eliminate_pec=False):
+                   precond=_hip.PRECOND_AUXILIARY, eliminate_pec=False):
         """Solve (cs S + ct T + i c0 S0) x = b (e.g. the shifted cavity operator with the
         PEC DOFs eliminated)."""
         import torch

@@ -330,7 +330,8 @@ def test_gradient_preconditioner_solve_matches_direct(dev, precision):
     k0 = 2 * np.pi * 1e9 / O.c0
     b = prob.rhs_dipoles(np.array([[0.051, 0.049, 0.05]]), np.array([[0, 1e-3, 1e-3]]), k0)
     x = torch.zeros_like(b)
-    st = prob.solve(k0, b, x, precision=precision, tol=1e-9, max_iter=20000)
+    st = prob.solve(k0, b, x, precision=precision, tol=1e-9, max_iter=20000,
+                    precond=_hip.PRECOND_GRADIENT)
     assert st["status"] == 0 and st["rel_residual"] <= 1e-9, st
     co, ce = O.unit_cube(4, 4, 4)
     osp = O.Space(co * 0.1, ce, 2)
@@ -351,7 +352,8 @@ def test_gradient_preconditioner_iterations_match_oracle(dev):
     k0 = 2 * np.pi * 1e9 / O.c0
     b = prob.rhs_dipoles(np.array([[0.051, 0.049, 0.05]]), np.array([[0, 1e-3, 1e-3]]), k0)
     x = torch.zeros_like(b)
-    st = prob.solve(k0, b, x, precision=64, tol=1e-9, max_iter=20000, check_every=1)
+    st = prob.solve(k0, b, x, precision=64, tol=1e-9, max_iter=20000, check_every=1,
+                    precond=_hip.PRECOND_GRADIENT)
     co, ce = O.unit_cube(4, 4, 4)
     osp = O.Space(co * 0.1, ce, 2)
     A = O.driven_lhs(O.assemble_fast(osp, eps_r=np.array([1.0, 4.4])[prob.region]), 1e9)

@@ -129,6 +129,7 @@ def test_solver_preconditioner_selection_without_gpu():
         s = COCGSolver(A, 'unknown')
         assert s._precond == _hip.PRECOND_NONE
     assert COCGSolver(A, 'gradient')._precond == _hip.PRECOND_GRADIENT
+    assert COCGSolver(A, 'auxiliary')._precond == _hip.PRECOND_AUXILIARY
     log = s.get_logging_data()
     assert log['id'][:3] == ['init', 'preconditioner::start', 'preconditioner::end']
 
