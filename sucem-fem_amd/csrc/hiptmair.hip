@@ -136,117 +136,95 @@ __global__ void k_grad_winv(int64_t nlag, const double* d3, double cs, double ct
   }
 }
 
-// w = winv .* (G^T r): L lanes per Lagrange row, U loads per lane in flight.  Vertex
-// rows hold ~100 entries, edge-node rows ~12, and the two kinds are contiguous in the
-// node numbering [0, nv) + [nv, nlag).  Each XCD takes one contiguous eighth of each
-// kind (neighbouring rows share the r entries they gather, so they should share an
-// L2), and inside an XCD its blocks deal the rows out round-robin (balanced work).
-template <typename R, int L, int U>
-__global__ __launch_bounds__(TB) void k_grad_t(int64_t nv, int64_t nlag,
-                                               const int32_t* __restrict__ gt_ptr,
-                                               const uint32_t* __restrict__ gt_pk, R inv_den,
-                                               const typename C2<R>::T* __restrict__ r,
-                                               const typename C2<R>::T* __restrict__ winv,
-                                               typename C2<R>::T* __restrict__ w) {
-  using C = typename C2<R>::T;
-  const int lane = threadIdx.x % L;
-  const int per_xcd = gridDim.x / 8;                 // the grid is a multiple of 8
-  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
-  const int64_t group = ((int64_t)local * TB + threadIdx.x) / L;
-  const int64_t ngroups = ((int64_t)per_xcd * TB) / L;
-  const int64_t ne = nlag - nv;
-  const int64_t v0 = nv * xcd / 8, v1 = nv * (xcd + 1) / 8;
-  const int64_t e0 = nv + ne * xcd / 8, e1 = nv + ne * (xcd + 1) / 8;
-  const int64_t nmine = (v1 - v0) + (e1 - e0);
-  for (int64_t jb = 0; jb < nmine; jb += ngroups) {
-    const int64_t t = jb + group;
-    const int64_t j = t < v1 - v0 ? v0 + t : e0 + (t - (v1 - v0));
-    C acc = mk(R(0), R(0));
-    if (t < nmine) {
-      const int q1 = gt_ptr[j + 1];
-      for (int qb = gt_ptr[j] + lane; qb < q1; qb += L * U) {
-        int cc[U];
-        R gg[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int q = qb + u * L;
-          const uint32_t p = q < q1 ? gt_pk[q] : 0u;
-          cc[u] = q < q1 ? pk_col(p) : -1;
-          gg[u] = (R)pk_num(p);
-        }
-        C v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = cc[u] >= 0 ? r[cc[u]] : mk(R(0), R(0));
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          acc.x += gg[u] * v[u].x;
-          acc.y += gg[u] * v[u].y;
-        }
-      }
-    }
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) {
-      acc.x += __shfl_xor(acc.x, o);
-      acc.y += __shfl_xor(acc.y, o);
-    }
-    acc.x *= inv_den;
-    acc.y *= inv_den;
-    if (lane == 0 && t < nmine) w[j] = cmul(winv[j], acc);
-  }
-}
+// ---- application without stored matrices ------------------------------------------
+// Every entry of G and Pi follows from the DOF's entity (edge e = (a, b), a < b; face
+// f = (p, q, r), p < q < r; DOF layout 2e + k and 2 nE + 2f + k for order 2, e for
+// order 1) and the coordinates:
+//
+//   G, order 2 (units of 1/3):  edge DOF k = 0: -5 w_a + w_b + 4 w_e,  k = 1: -w_a + 5 w_b
+//     - 4 w_e; face DOF k = 0: -w_p + w_q - 4 w_(p,r) + 4 w_(q,r), k = 1: -w_p + w_r
+//     - 4 w_(p,q) + 4 w_(q,r) (w_e: the P2 node at the midpoint of edge e);
+//   G, order 1: w_b - w_a;
+//   Pi: edge DOF k: (x_b - x_a) . ((2 - k)/3 u_a + (1 + k)/3 u_b) (order 1: (u_a + u_b)/2);
+//     face DOF k: (x_q - x_p or x_r - x_p) . (u_p + u_q + u_r) / 3.
+//
+// These are the nodal duals (tangential derivative / value at the DOF point along its
+// tangent) of the P2 (P1) Lagrange basis and of P1 vector fields; G^T and Pi^T run over
+// vertex -> edge / face and edge -> face adjacency lists.  The stored CSR forms of G and
+// Pi^T are used only to form diag(G^T A G) and diag(Pi^T A Pi) once per system.
 
-// z = dinv .* r + G w (masked rows zero); optional reduction of r.z and |r|^2
-// Pi applied without stored values: DOF i of edge e = (a, b), a < b, at 1/3 (k = 0) or
-// 2/3 (k = 1) from a: Pi_i u = (x_b - x_a) . (lambda_a u_a + lambda_b u_b), lambda_b =
-// (k + 1) / 3 (order 1: the midpoint, 1/2 each); DOF i of face f = (p, q, r), p < q < r,
-// at its centroid: Pi_i u = (x_q - x_p or x_r - x_p) . (u_p + u_q + u_r) / 3.
-template <typename R, int ORDER>
-__device__ __forceinline__ typename C2<R>::T nodal_row(int i, int nE, const int32_t* edges,
-                                                       const int32_t* faces, const R* xyz,
-                                                       const typename C2<R>::T* u) {
+// (G w)_i and (Pi u)_i for DOF i
+template <typename R, int ORDER, bool NODAL>
+__device__ __forceinline__ void aux_row(int i, int nv, int nE, const int32_t* __restrict__ edges,
+                                        const int32_t* __restrict__ faces,
+                                        const int32_t* __restrict__ face_edges,
+                                        const R* __restrict__ xyz,
+                                        const typename C2<R>::T* __restrict__ w,
+                                        const typename C2<R>::T* __restrict__ u,
+                                        typename C2<R>::T& gw, typename C2<R>::T& pu) {
   using C = typename C2<R>::T;
-  C s = mk(R(0), R(0));
+  gw = mk(R(0), R(0));
+  pu = mk(R(0), R(0));
   const int ne_dofs = ORDER == 2 ? 2 * nE : nE;
   if (i < ne_dofs) {
     const int e = ORDER == 2 ? i >> 1 : i;
+    const int k = ORDER == 2 ? (i & 1) : 0;
     const int a = edges[2 * e], b = edges[2 * e + 1];
-    const R lb = ORDER == 2 ? R((i & 1) + 1) * R(1.0 / 3.0) : R(0.5);
-    const R la = R(1) - lb;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const R t = xyz[3 * b + d] - xyz[3 * a + d];
-      const C ua = u[3 * a + d], ub = u[3 * b + d];
-      s.x += t * (la * ua.x + lb * ub.x);
-      s.y += t * (la * ua.y + lb * ub.y);
+    const C wa = w[a], wb = w[b];
+    if constexpr (ORDER == 2) {
+      const C we = w[nv + e];
+      const R ca = k ? R(-1) : R(-5), cb = k ? R(5) : R(1), ce = k ? R(-4) : R(4);
+      gw = mk(ca * wa.x + cb * wb.x + ce * we.x, ca * wa.y + cb * wb.y + ce * we.y);
+    } else {
+      gw = mk(wb.x - wa.x, wb.y - wa.y);
     }
-  } else {
-    const int f = (i - ne_dofs) >> 1;
-    const int p = faces[3 * f], q = faces[3 * f + 1], r = faces[3 * f + 2];
-    const int o = (i & 1) ? r : q;
+    if constexpr (NODAL) {
+      const R lb = ORDER == 2 ? R(k + 1) * R(1.0 / 3.0) : R(0.5);
+      const R la = R(1) - lb;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const R t = (xyz[3 * o + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
-      const C up = u[3 * p + d], uq = u[3 * q + d], ur = u[3 * r + d];
-      s.x += t * (up.x + uq.x + ur.x);
-      s.y += t * (up.y + uq.y + ur.y);
+      for (int d = 0; d < 3; ++d) {
+        const R t = xyz[3 * b + d] - xyz[3 * a + d];
+        const C ua = u[3 * a + d], ub = u[3 * b + d];
+        pu.x += t * (la * ua.x + lb * ub.x);
+        pu.y += t * (la * ua.y + lb * ub.y);
+      }
+    }
+  } else if constexpr (ORDER == 2) {
+    const int f = (i - ne_dofs) >> 1;
+    const int k = i & 1;
+    const int p = faces[3 * f], q = faces[3 * f + 1], r = faces[3 * f + 2];
+    const int o = k ? r : q;
+    const C wp = w[p], wo = w[o];
+    const C we = w[nv + face_edges[3 * f + (k ? 0 : 1)]];   // (p,q) for k = 1, (p,r) for k = 0
+    const C wqr = w[nv + face_edges[3 * f + 2]];
+    gw = mk(wo.x - wp.x + R(4) * (wqr.x - we.x), wo.y - wp.y + R(4) * (wqr.y - we.y));
+    if constexpr (NODAL) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const R t = (xyz[3 * o + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
+        const C up = u[3 * p + d], uq = u[3 * q + d], ur = u[3 * r + d];
+        pu.x += t * (up.x + uq.x + ur.x);
+        pu.y += t * (up.y + uq.y + ur.y);
+      }
     }
   }
-  return s;
 }
 
-template <typename R, bool NODAL>
-__global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __restrict__ g_ptr,
-                                               const uint32_t* __restrict__ g_pk, R inv_den,
-                                               const typename C2<R>::T* __restrict__ w,
-                                               const typename C2<R>::T* __restrict__ dinv,
-                                               const typename C2<R>::T* __restrict__ r,
-                                               typename C2<R>::T* __restrict__ z,
-                                               const uint8_t* __restrict__ mask, int reduce,
-                                               RedSite rs, int order, int nE,
-                                               const int32_t* __restrict__ edges,
-                                               const int32_t* __restrict__ faces,
-                                               const R* __restrict__ xyz,
-                                               const typename C2<R>::T* __restrict__ u) {
+// z = dinv .* r + inv_den G w (+ Pi u) (masked rows zero); optional reduction of r.z and
+// |r|^2.
+template <typename R, int ORDER, bool NODAL>
+__global__ __launch_bounds__(TB) void k_aux_z(int n, int G, int nv, int nE, R inv_den,
+                                              const int32_t* __restrict__ edges,
+                                              const int32_t* __restrict__ faces,
+                                              const int32_t* __restrict__ face_edges,
+                                              const R* __restrict__ xyz,
+                                              const typename C2<R>::T* __restrict__ w,
+                                              const typename C2<R>::T* __restrict__ u,
+                                              const typename C2<R>::T* __restrict__ dinv,
+                                              const typename C2<R>::T* __restrict__ r,
+                                              typename C2<R>::T* __restrict__ z,
+                                              const uint8_t* __restrict__ mask, int reduce,
+                                              RedSite rs) {
   using C = typename C2<R>::T;
   const int lb = xcd_block(G);
   const int chunk = (n + G - 1) / G;
@@ -254,22 +232,10 @@ __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __re
   double2 d1 = make_double2(0, 0), d2 = make_double2(0, 0);
   for (int i = start + threadIdx.x; i < end; i += TB) {
     const C ri = r[i];
-    C gw = mk(R(0), R(0));
-    for (int q = g_ptr[i]; q < g_ptr[i + 1]; ++q) {
-      const uint32_t p = g_pk[q];
-      const R g = (R)pk_num(p);
-      const C v = w[pk_col(p)];
-      gw.x += g * v.x;
-      gw.y += g * v.y;
-    }
+    C gw, pu;
+    aux_row<R, ORDER, NODAL>(i, nv, nE, edges, faces, face_edges, xyz, w, u, gw, pu);
     const C di = cmul(dinv[i], ri);
-    C zi = mk(di.x + inv_den * gw.x, di.y + inv_den * gw.y);
-    if constexpr (NODAL) {
-      const C pu = order == 2 ? nodal_row<R, 2>(i, nE, edges, faces, xyz, u)
-                              : nodal_row<R, 1>(i, nE, edges, faces, xyz, u);
-      zi.x += pu.x;
-      zi.y += pu.y;
-    }
+    C zi = mk(di.x + inv_den * gw.x + pu.x, di.y + inv_den * gw.y + pu.y);
     if (mask && mask[i]) zi = mk(R(0), R(0));
     z[i] = zi;
     if (reduce) {
@@ -280,6 +246,140 @@ __global__ __launch_bounds__(TB) void k_grad_z(int n, int G, const int32_t* __re
     }
   }
   if (reduce) reduce_finalize<TB>(rs, G, d1, d2);
+}
+
+// Vertex rows of G^T and Pi^T, L lanes per vertex v:
+//   w_v = winv_v inv_den (G^T r)_v,  u_(v,d) = vinv_(v,d) (Pi^T r)_(v,d)
+// over the edges at v (entries (e << 1) | [v = b]) and the faces at v (entries
+// (f << 2) | position of v in (p, q, r)).
+template <typename R, int L, int ORDER, bool NODAL>
+__global__ __launch_bounds__(TB) void k_aux_vt(int nv, int nE, R inv_den,
+                                               const int32_t* __restrict__ ve_ptr,
+                                               const int32_t* __restrict__ ve_ent,
+                                               const int32_t* __restrict__ vf_ptr,
+                                               const int32_t* __restrict__ vf_ent,
+                                               const int32_t* __restrict__ edges,
+                                               const int32_t* __restrict__ faces,
+                                               const R* __restrict__ xyz,
+                                               const typename C2<R>::T* __restrict__ r,
+                                               const typename C2<R>::T* __restrict__ winv,
+                                               const typename C2<R>::T* __restrict__ vinv,
+                                               typename C2<R>::T* __restrict__ w,
+                                               typename C2<R>::T* __restrict__ u) {
+  using C = typename C2<R>::T;
+  const int lane = threadIdx.x % L;
+  const int64_t v = (blockIdx.x * (int64_t)TB + threadIdx.x) / L;
+  C g = mk(R(0), R(0));
+  C acc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) acc[d] = mk(R(0), R(0));
+  if (v < nv) {
+    for (int q = ve_ptr[v] + lane; q < ve_ptr[v + 1]; q += L) {
+      const int ent = ve_ent[q];
+      const int e = ent >> 1;
+      const bool second = ent & 1;
+      C sr;
+      if constexpr (ORDER == 2) {
+        const C r0 = r[2 * e], r1 = r[2 * e + 1];
+        // G: -5 r0 - r1 at a, r0 + 5 r1 at b; Pi: lambda_v = (2/3, 1/3) at a, (1/3, 2/3) at b
+        const R g0 = second ? R(1) : R(-5), g1 = second ? R(5) : R(-1);
+        g.x += g0 * r0.x + g1 * r1.x;
+        g.y += g0 * r0.y + g1 * r1.y;
+        const R l0 = second ? R(1.0 / 3.0) : R(2.0 / 3.0), l1 = R(1) - l0;
+        sr = mk(l0 * r0.x + l1 * r1.x, l0 * r0.y + l1 * r1.y);
+      } else {
+        const C r0 = r[e];
+        g.x += second ? r0.x : -r0.x;
+        g.y += second ? r0.y : -r0.y;
+        sr = mk(R(0.5) * r0.x, R(0.5) * r0.y);
+      }
+      if constexpr (NODAL) {
+        const int a = edges[2 * e], b = edges[2 * e + 1];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const R t = xyz[3 * b + d] - xyz[3 * a + d];
+          acc[d].x += t * sr.x;
+          acc[d].y += t * sr.y;
+        }
+      }
+    }
+    if constexpr (ORDER == 2) {
+      for (int q = vf_ptr[v] + lane; q < vf_ptr[v + 1]; q += L) {
+        const int ent = vf_ent[q];
+        const int f = ent >> 2, pos = ent & 3;
+        const C r0 = r[2 * nE + 2 * f], r1 = r[2 * nE + 2 * f + 1];
+        // G: -(r0 + r1) at p, r0 at q, r1 at r
+        if (pos == 0) { g.x -= r0.x + r1.x; g.y -= r0.y + r1.y; }
+        else if (pos == 1) { g.x += r0.x; g.y += r0.y; }
+        else { g.x += r1.x; g.y += r1.y; }
+        if constexpr (NODAL) {
+          const int p = faces[3 * f], fq = faces[3 * f + 1], fr = faces[3 * f + 2];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const R t0 = (xyz[3 * fq + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
+            const R t1 = (xyz[3 * fr + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
+            acc[d].x += t0 * r0.x + t1 * r1.x;
+            acc[d].y += t0 * r0.y + t1 * r1.y;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) {
+    g.x += __shfl_xor(g.x, o);
+    g.y += __shfl_xor(g.y, o);
+  }
+  if constexpr (NODAL) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int o = L / 2; o > 0; o >>= 1) {
+        acc[d].x += __shfl_xor(acc[d].x, o);
+        acc[d].y += __shfl_xor(acc[d].y, o);
+      }
+  }
+  if (lane == 0 && v < nv) {
+    w[v] = cmul(winv[v], mk(inv_den * g.x, inv_den * g.y));
+    if constexpr (NODAL)
+      for (int d = 0; d < 3; ++d) u[3 * v + d] = cmul(vinv[3 * v + d], acc[d]);
+  }
+}
+
+// Edge-node rows of G^T (order 2), L lanes per edge e: its own two DOFs (4 r0 - 4 r1) and
+// the faces containing e (entries (f << 2) | position of e among (p,q), (p,r), (q,r)):
+// -4 r1 at (p,q), -4 r0 at (p,r), 4 (r0 + r1) at (q,r); w_(nv+e) = winv inv_den (.).
+template <typename R, int L>
+__global__ __launch_bounds__(TB) void k_aux_et(int nv, int nE, R inv_den,
+                                               const int32_t* __restrict__ ef_ptr,
+                                               const int32_t* __restrict__ ef_ent,
+                                               const typename C2<R>::T* __restrict__ r,
+                                               const typename C2<R>::T* __restrict__ winv,
+                                               typename C2<R>::T* __restrict__ w) {
+  using C = typename C2<R>::T;
+  const int lane = threadIdx.x % L;
+  const int64_t e = (blockIdx.x * (int64_t)TB + threadIdx.x) / L;
+  C g = mk(R(0), R(0));
+  if (e < nE) {
+    if (lane == 0) {
+      const C r0 = r[2 * e], r1 = r[2 * e + 1];
+      g = mk(R(4) * (r0.x - r1.x), R(4) * (r0.y - r1.y));
+    }
+    for (int q = ef_ptr[e] + lane; q < ef_ptr[e + 1]; q += L) {
+      const int ent = ef_ent[q];
+      const int f = ent >> 2, pos = ent & 3;
+      const C r0 = r[2 * nE + 2 * f], r1 = r[2 * nE + 2 * f + 1];
+      if (pos == 0) { g.x -= R(4) * r1.x; g.y -= R(4) * r1.y; }
+      else if (pos == 1) { g.x -= R(4) * r0.x; g.y -= R(4) * r0.y; }
+      else { g.x += R(4) * (r0.x + r1.x); g.y += R(4) * (r0.y + r1.y); }
+    }
+  }
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) {
+    g.x += __shfl_xor(g.x, o);
+    g.y += __shfl_xor(g.y, o);
+  }
+  if (lane == 0 && e < nE) w[nv + e] = cmul(winv[nv + e], mk(inv_den * g.x, inv_den * g.y));
 }
 
 // ---- vector nodal term --------------------------------------------------------------
@@ -370,75 +470,6 @@ __global__ __launch_bounds__(TB) void k_nodal_diag(int64_t nv, int64_t nnz, cons
   }
 }
 
-// u[3v + d] = vinv[3v + d] (Pi^T r)_(v, d), L lanes per vertex: the edges at v (entries
-// (e << 1) | [v is the edge's second vertex]) and the faces at v (entries f << 2); the
-// values are formed from the coordinates as in nodal_row.
-template <typename R, int L, int ORDER>
-__global__ __launch_bounds__(TB) void k_nodal_t(int64_t nv, int nE,
-                                                const int32_t* __restrict__ ve_ptr,
-                                                const int32_t* __restrict__ ve_ent,
-                                                const int32_t* __restrict__ vf_ptr,
-                                                const int32_t* __restrict__ vf_ent,
-                                                const int32_t* __restrict__ edges,
-                                                const int32_t* __restrict__ faces,
-                                                const R* __restrict__ xyz,
-                                                const typename C2<R>::T* __restrict__ r,
-                                                const typename C2<R>::T* __restrict__ vinv,
-                                                typename C2<R>::T* __restrict__ u) {
-  using C = typename C2<R>::T;
-  const int lane = threadIdx.x % L;
-  const int64_t v = (blockIdx.x * (int64_t)TB + threadIdx.x) / L;
-  C acc[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) acc[d] = mk(R(0), R(0));
-  if (v < nv) {
-    for (int q = ve_ptr[v] + lane; q < ve_ptr[v + 1]; q += L) {
-      const int ent = ve_ent[q];
-      const int e = ent >> 1;
-      const int a = edges[2 * e], b = edges[2 * e + 1];
-      C sr;
-      if constexpr (ORDER == 2) {
-        // lambda_v at the two DOF points: (2/3, 1/3) at a, (1/3, 2/3) at b
-        const C r0 = r[2 * e], r1 = r[2 * e + 1];
-        const R l0 = (ent & 1) ? R(1.0 / 3.0) : R(2.0 / 3.0), l1 = R(1) - l0;
-        sr = mk(l0 * r0.x + l1 * r1.x, l0 * r0.y + l1 * r1.y);
-      } else {
-        const C r0 = r[e];
-        sr = mk(R(0.5) * r0.x, R(0.5) * r0.y);
-      }
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const R t = xyz[3 * b + d] - xyz[3 * a + d];
-        acc[d].x += t * sr.x;
-        acc[d].y += t * sr.y;
-      }
-    }
-    if constexpr (ORDER == 2) {
-      for (int q = vf_ptr[v] + lane; q < vf_ptr[v + 1]; q += L) {
-        const int f = vf_ent[q] >> 2;
-        const int p = faces[3 * f], fq = faces[3 * f + 1], fr = faces[3 * f + 2];
-        const C r0 = r[2 * nE + 2 * f], r1 = r[2 * nE + 2 * f + 1];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const R t0 = (xyz[3 * fq + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
-          const R t1 = (xyz[3 * fr + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
-          acc[d].x += t0 * r0.x + t1 * r1.x;
-          acc[d].y += t0 * r0.y + t1 * r1.y;
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) {
-      acc[d].x += __shfl_xor(acc[d].x, o);
-      acc[d].y += __shfl_xor(acc[d].y, o);
-    }
-  if (lane == 0 && v < nv)
-    for (int d = 0; d < 3; ++d) u[3 * v + d] = cmul(vinv[3 * v + d], acc[d]);
-}
-
 // vertex -> entity adjacency entries: edges (v, (e << 1) | role), faces (v, f << 2 | role)
 __global__ void k_adj_keys(int64_t nent, int nvper, const int32_t* ent_verts, int B,
                            int shift, uint64_t* keys, uint32_t* vals) {
@@ -457,6 +488,31 @@ __global__ void k_xyz32(int64_t n, const double* a, float* b) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     b[i] = (float)a[i];
+}
+
+// face_edges[3F + j] = the edges (p,q), (p,r), (q,r) of face F = (p, q, r), from each cell's
+// local face f and UFC local edges (every cell that holds F writes the same ids)
+__constant__ int c_face_edge[4][3] = {{2, 1, 0}, {4, 3, 0}, {5, 3, 1}, {5, 4, 2}};
+__global__ void k_face_edges(int64_t nc, const int32_t* cell_faces, const int32_t* cell_edges,
+                             int32_t* face_edges) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nc * 4;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t / 4;
+    const int f = (int)(t % 4);
+    const int64_t F = cell_faces[t];
+    for (int j = 0; j < 3; ++j) face_edges[3 * F + j] = cell_edges[6 * c + c_face_edge[f][j]];
+  }
+}
+
+// edge -> face adjacency entries (e, (f << 2) | position of e among (p,q), (p,r), (q,r))
+__global__ void k_ef_keys(int64_t nF, const int32_t* face_edges, int B, uint64_t* keys,
+                          uint32_t* vals) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < 3 * nF;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t code = ((uint64_t)(t / 3) << 2) | (uint64_t)(t % 3);
+    keys[t] = ((uint64_t)face_edges[t] << B) | code;
+    vals[t] = (uint32_t)t;
+  }
 }
 
 }  // namespace
@@ -500,7 +556,14 @@ void ensure_nodal(Context* c, Space* s) {
     s->nnz_pi = nnz;
     c->sync();
   }
-  // vertex -> edge / face adjacency and FP32 coordinates for the matrix-free Pi and Pi^T
+}
+
+// Vertex -> edge / face and edge -> face adjacency, face -> edge ids and FP32 coordinates:
+// what the preconditioner kernels read in place of stored G, G^T, Pi and Pi^T.
+void ensure_entity_maps(Context* c, Space* s) {
+  std::lock_guard<std::mutex> hold(s->lazy);
+  if (s->xyz32.p) return;
+  const Mesh* m = s->mesh;
   for (int kind = 0; kind < (s->order == 2 ? 2 : 1); ++kind) {
     const int64_t nent = kind == 0 ? s->nE : s->nF;
     const int nvper = kind == 0 ? 2 : 3;
@@ -517,10 +580,28 @@ void ensure_nodal(Context* c, Space* s) {
                          kind == 0 ? s->ve_ent : s->vf_ent, seg, perm);
     c->sync();
   }
-  if (s->order == 1) {
+  if (s->order == 2) {
+    s->face_edges.alloc(3 * s->nF);
+    SUCEM_LAUNCH(k_face_edges, grid_for(m->nc * 4, TB), TB, 0, c->stream, m->nc,
+                 s->cell_faces.p, s->cell_edges.p, s->face_edges.p);
+    const int64_t na = 3 * s->nF;
+    const int Bk = key_bits(std::max<int64_t>(s->nF << 2, s->nE));
+    DBuf<uint64_t> keys(na);
+    DBuf<uint32_t> vals(na);
+    SUCEM_LAUNCH(k_ef_keys, grid_for(na, TB), TB, 0, c->stream, s->nF, s->face_edges.p, Bk,
+                 keys.p, vals.p);
+    DBuf<int32_t> seg;
+    DBuf<uint32_t> perm;
+    build_csr_from_pairs(c, na, keys, vals, Bk, s->nE, s->ef_ptr, s->ef_ent, seg, perm);
+    c->sync();
+  } else {
     s->vf_ptr.alloc(m->nv + 1);
     SUCEM_HIP(hipMemsetAsync(s->vf_ptr.p, 0, (m->nv + 1) * sizeof(int32_t), c->stream));
     s->vf_ent.alloc(1);
+    s->face_edges.alloc(1);
+    s->ef_ptr.alloc(s->nE + 1);
+    SUCEM_HIP(hipMemsetAsync(s->ef_ptr.p, 0, (s->nE + 1) * sizeof(int32_t), c->stream));
+    s->ef_ent.alloc(1);
   }
   s->xyz32.alloc(3 * m->nv);
   SUCEM_LAUNCH(k_xyz32, grid_for(3 * m->nv, TB), TB, 0, c->stream, 3 * m->nv, m->xyz.p,
@@ -653,33 +734,34 @@ void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv
                   void* w, void* z, const uint8_t* mask, int G, const RedSite& rs,
                   const void* vinv, void* u) {
   using C = typename C2<R>::T;
+  SUCEM_CHECK(s->xyz32.p, SUCEM_ERR_INVALID, "preconditioner maps not built");
   const R inv_den = (R)(1.0 / s->g_den);
-  const int Gt = op_grid(c, s->nlag, 16);               // multiple of 8
-  SUCEM_LAUNCH((k_grad_t<R, 8, 4>), Gt, TB, 0, c->stream, (int64_t)s->mesh->nv, s->nlag,
-               s->gt_ptr.p, s->gt_pk.p, inv_den, (const C*)r, (const C*)winv, (C*)w);
+  const R* xyz;
+  if constexpr (sizeof(R) == 4) xyz = s->xyz32.p; else xyz = s->mesh->xyz.p;
+  const int nv = (int)s->mesh->nv, nE = (int)s->nE;
   const int reduce = rs.mode != FIN_NONE ? 1 : 0;
-  if (vinv) {
-    const R* xyz;
-    if constexpr (sizeof(R) == 4) xyz = s->xyz32.p; else xyz = s->mesh->xyz.p;
-    const int64_t nv = s->mesh->nv;
-    constexpr int L = 16;
-    const int Gn = (int)((nv * L + TB - 1) / TB);
-    if (s->order == 2)
-      SUCEM_LAUNCH((k_nodal_t<R, L, 2>), Gn, TB, 0, c->stream, nv, (int)s->nE, s->ve_ptr.p,
-                   s->ve_ent.p, s->vf_ptr.p, s->vf_ent.p, s->edges.p, s->faces.p, xyz,
-                   (const C*)r, (const C*)vinv, (C*)u);
-    else
-      SUCEM_LAUNCH((k_nodal_t<R, L, 1>), Gn, TB, 0, c->stream, nv, (int)s->nE, s->ve_ptr.p,
-                   s->ve_ent.p, s->vf_ptr.p, s->vf_ent.p, s->edges.p, s->faces.p, xyz,
-                   (const C*)r, (const C*)vinv, (C*)u);
-    SUCEM_LAUNCH((k_grad_z<R, true>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p,
-                 s->g_pk.p, inv_den, (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
-                 reduce, rs, s->order, (int)s->nE, s->edges.p, s->faces.p, xyz, (const C*)u);
+  constexpr int LV = 16, LE = 4;
+  const int Gv = (int)(((int64_t)nv * LV + TB - 1) / TB);
+#define VT_ARGS nv, nE, inv_den, s->ve_ptr.p, s->ve_ent.p, s->vf_ptr.p, s->vf_ent.p, s->edges.p, \
+                s->faces.p, xyz, (const C*)r, (const C*)winv, (const C*)vinv, (C*)w, (C*)u
+#define Z_ARGS (int)s->ndofs, G, nv, nE, inv_den, s->edges.p, s->faces.p, s->face_edges.p, xyz, \
+               (const C*)w, (const C*)u, (const C*)dinv, (const C*)r, (C*)z, mask, reduce, rs
+  if (s->order == 2) {
+    if (vinv) SUCEM_LAUNCH((k_aux_vt<R, LV, 2, true>), Gv, TB, 0, c->stream, VT_ARGS);
+    else SUCEM_LAUNCH((k_aux_vt<R, LV, 2, false>), Gv, TB, 0, c->stream, VT_ARGS);
+    const int Ge = (int)(((int64_t)nE * LE + TB - 1) / TB);
+    SUCEM_LAUNCH((k_aux_et<R, LE>), Ge, TB, 0, c->stream, nv, nE, inv_den, s->ef_ptr.p,
+                 s->ef_ent.p, (const C*)r, (const C*)winv, (C*)w);
+    if (vinv) SUCEM_LAUNCH((k_aux_z<R, 2, true>), G, TB, 0, c->stream, Z_ARGS);
+    else SUCEM_LAUNCH((k_aux_z<R, 2, false>), G, TB, 0, c->stream, Z_ARGS);
   } else {
-    SUCEM_LAUNCH((k_grad_z<R, false>), G, TB, 0, c->stream, (int)s->ndofs, G, s->g_ptr.p,
-                 s->g_pk.p, inv_den, (const C*)w, (const C*)dinv, (const C*)r, (C*)z, mask,
-                 reduce, rs, 0, 0, nullptr, nullptr, (const R*)nullptr, (const C*)nullptr);
+    if (vinv) SUCEM_LAUNCH((k_aux_vt<R, LV, 1, true>), Gv, TB, 0, c->stream, VT_ARGS);
+    else SUCEM_LAUNCH((k_aux_vt<R, LV, 1, false>), Gv, TB, 0, c->stream, VT_ARGS);
+    if (vinv) SUCEM_LAUNCH((k_aux_z<R, 1, true>), G, TB, 0, c->stream, Z_ARGS);
+    else SUCEM_LAUNCH((k_aux_z<R, 1, false>), G, TB, 0, c->stream, Z_ARGS);
   }
+#undef VT_ARGS
+#undef Z_ARGS
   finish_reduction(c, rs, G);
 }
 template void grad_precond<float>(Context*, const Space*, const void*, const void*, const void*,

@@ -88,6 +88,9 @@ struct Space {
   DBuf<int32_t> pit_ptr, pit_col;
   DBuf<double> pit_val;
   DBuf<int32_t> ve_ptr, ve_ent, vf_ptr, vf_ent;
+  // the discrete gradient without stored values: face -> edge ids ((p,q), (p,r), (q,r)) and
+  // edge -> face entries (f << 2) | position of the edge in the face
+  DBuf<int32_t> face_edges, ef_ptr, ef_ent;
   DBuf<float> xyz32;
   // matrix-free operator (built on first use): DOF ids per cell as SoA; the products of
   // cells (cell * nloc + local) and boundary faces (face * nfd + face DOF) sorted by DOF:
@@ -219,6 +222,7 @@ void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const Sol
 //   M^-1 r = D^-1 r + G (diag(G^T A G))^-1 G^T r
 void ensure_gradient(Context* c, Space* s);
 void ensure_nodal(Context* c, Space* s);
+void ensure_entity_maps(Context* c, Space* s);
 const double* gradient_diag(Context* c, System* sys, bool masked);
 
 // batch.hip: nb driven systems (one k0 each) on one assembled system

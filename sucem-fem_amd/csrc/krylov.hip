@@ -305,6 +305,7 @@ struct SysOp : Op {
   }
   int64_t nlag() override {
     ensure_gradient(c, sys->space);
+    ensure_entity_maps(c, sys->space);
     return sys->space->nlag;
   }
   void winv_lo(void* winv) override {
