@@ -174,7 +174,8 @@ def main():
     mf = int(tuned[a.precision].get("matrix_free", 0))
     cb = int(tuned[a.precision].get("loads", 256))
     spmv_bytes = prob.operator_bytes(a.precision, mf, cb=cb)
-    op_bytes = prob.operator_bytes(a.precision, mf, epilogue=True, cb=cb)
+    # inside COCG the patch form leaves the shared rows' partial sums to the x/r update
+    op_bytes = prob.operator_bytes(a.precision, mf, epilogue=True, cb=cb, deferred=(mf == 2))
     if a.copy_probe_mb > 0:
         src = torch.ones(a.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -250,8 +251,9 @@ def main():
                      "kernel": ("k_sys_spmv (assembled S - k0^2 T + i k0 S0",
                                 "k_mf_cells + k_mf_faces + k_mf_gather_lds (matrix-free "
                                 "S - k0^2 T + i k0 S0",
-                                "k_mf_patch + k_mf_psum (matrix-free S - k0^2 T + i k0 S0, "
-                                "patches of %d cells" % cb)[mf]
+                                "k_mf_patch (matrix-free S - k0^2 T + i k0 S0, patches of "
+                                "%d cells, shared rows left as partial sums for the x/r "
+                                "update" % cb)[mf]
                                + " with the fused p.Ap epilogue)"
                                + (", FP32, complex64 x" if a.precision == 32 else ", FP64"),
                      "bytes_per_launch": op_bytes, "avg_ms": op_ms, "samples": nsamp,

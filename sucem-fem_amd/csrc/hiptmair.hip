@@ -253,7 +253,7 @@ __global__ __launch_bounds__(TB) void k_aux_z(int n, int G, int nv, int nE, R in
 // over the edges at v (entries (e << 1) | [v = b]) and the faces at v (entries
 // (f << 2) | position of v in (p, q, r)).
 template <typename R, int L, int ORDER, bool NODAL>
-__global__ __launch_bounds__(TB) void k_aux_vt(int nv, int nE, R inv_den,
+__device__ __forceinline__ void aux_vertex_rows(int64_t blk, int nv, int nE, R inv_den,
                                                const int32_t* __restrict__ ve_ptr,
                                                const int32_t* __restrict__ ve_ent,
                                                const int32_t* __restrict__ vf_ptr,
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(TB) void k_aux_vt(int nv, int nE, R inv_den,
                                                typename C2<R>::T* __restrict__ u) {
   using C = typename C2<R>::T;
   const int lane = threadIdx.x % L;
-  const int64_t v = (blockIdx.x * (int64_t)TB + threadIdx.x) / L;
+  const int64_t v = (blk * TB + threadIdx.x) / L;
   C g = mk(R(0), R(0));
   C acc[3];
 #pragma unroll
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(TB) void k_aux_vt(int nv, int nE, R inv_den,
 // the faces containing e (entries (f << 2) | position of e among (p,q), (p,r), (q,r)):
 // -4 r1 at (p,q), -4 r0 at (p,r), 4 (r0 + r1) at (q,r); w_(nv+e) = winv inv_den (.).
 template <typename R, int L>
-__global__ __launch_bounds__(TB) void k_aux_et(int nv, int nE, R inv_den,
+__device__ __forceinline__ void aux_edge_rows(int64_t blk, int nv, int nE, R inv_den,
                                                const int32_t* __restrict__ ef_ptr,
                                                const int32_t* __restrict__ ef_ent,
                                                const typename C2<R>::T* __restrict__ r,
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(TB) void k_aux_et(int nv, int nE, R inv_den,
                                                typename C2<R>::T* __restrict__ w) {
   using C = typename C2<R>::T;
   const int lane = threadIdx.x % L;
-  const int64_t e = (blockIdx.x * (int64_t)TB + threadIdx.x) / L;
+  const int64_t e = (blk * TB + threadIdx.x) / L;
   C g = mk(R(0), R(0));
   if (e < nE) {
     if (lane == 0) {
@@ -380,6 +380,31 @@ __global__ __launch_bounds__(TB) void k_aux_et(int nv, int nE, R inv_den,
     g.y += __shfl_xor(g.y, o);
   }
   if (lane == 0 && e < nE) w[nv + e] = cmul(winv[nv + e], mk(inv_den * g.x, inv_den * g.y));
+}
+
+// One launch for both kinds of G^T rows: blocks [0, Gv) take the vertex rows (with Pi^T),
+// blocks [Gv, gridDim) the edge-node rows, so the two gathers overlap on the device.
+template <typename R, int LV, int LE, int ORDER, bool NODAL>
+__global__ __launch_bounds__(TB) void k_aux_t(int Gv, int nv, int nE, R inv_den,
+                                              const int32_t* __restrict__ ve_ptr,
+                                              const int32_t* __restrict__ ve_ent,
+                                              const int32_t* __restrict__ vf_ptr,
+                                              const int32_t* __restrict__ vf_ent,
+                                              const int32_t* __restrict__ ef_ptr,
+                                              const int32_t* __restrict__ ef_ent,
+                                              const int32_t* __restrict__ edges,
+                                              const int32_t* __restrict__ faces,
+                                              const R* __restrict__ xyz,
+                                              const typename C2<R>::T* __restrict__ r,
+                                              const typename C2<R>::T* __restrict__ winv,
+                                              const typename C2<R>::T* __restrict__ vinv,
+                                              typename C2<R>::T* __restrict__ w,
+                                              typename C2<R>::T* __restrict__ u) {
+  if ((int)blockIdx.x < Gv)
+    aux_vertex_rows<R, LV, ORDER, NODAL>(blockIdx.x, nv, nE, inv_den, ve_ptr, ve_ent, vf_ptr,
+                                         vf_ent, edges, faces, xyz, r, winv, vinv, w, u);
+  else if constexpr (ORDER == 2)
+    aux_edge_rows<R, LE>(blockIdx.x - Gv, nv, nE, inv_den, ef_ptr, ef_ent, r, winv, w);
 }
 
 // ---- vector nodal term --------------------------------------------------------------
@@ -742,21 +767,20 @@ void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv
   const int reduce = rs.mode != FIN_NONE ? 1 : 0;
   constexpr int LV = 16, LE = 4;
   const int Gv = (int)(((int64_t)nv * LV + TB - 1) / TB);
-#define VT_ARGS nv, nE, inv_den, s->ve_ptr.p, s->ve_ent.p, s->vf_ptr.p, s->vf_ent.p, s->edges.p, \
-                s->faces.p, xyz, (const C*)r, (const C*)winv, (const C*)vinv, (C*)w, (C*)u
+#define VT_ARGS Gv, nv, nE, inv_den, s->ve_ptr.p, s->ve_ent.p, s->vf_ptr.p, s->vf_ent.p, \
+                s->ef_ptr.p, s->ef_ent.p, s->edges.p, s->faces.p, xyz, (const C*)r, \
+                (const C*)winv, (const C*)vinv, (C*)w, (C*)u
 #define Z_ARGS (int)s->ndofs, G, nv, nE, inv_den, s->edges.p, s->faces.p, s->face_edges.p, xyz, \
                (const C*)w, (const C*)u, (const C*)dinv, (const C*)r, (C*)z, mask, reduce, rs
   if (s->order == 2) {
-    if (vinv) SUCEM_LAUNCH((k_aux_vt<R, LV, 2, true>), Gv, TB, 0, c->stream, VT_ARGS);
-    else SUCEM_LAUNCH((k_aux_vt<R, LV, 2, false>), Gv, TB, 0, c->stream, VT_ARGS);
-    const int Ge = (int)(((int64_t)nE * LE + TB - 1) / TB);
-    SUCEM_LAUNCH((k_aux_et<R, LE>), Ge, TB, 0, c->stream, nv, nE, inv_den, s->ef_ptr.p,
-                 s->ef_ent.p, (const C*)r, (const C*)winv, (C*)w);
+    const int Gt = Gv + (int)(((int64_t)nE * LE + TB - 1) / TB);
+    if (vinv) SUCEM_LAUNCH((k_aux_t<R, LV, LE, 2, true>), Gt, TB, 0, c->stream, VT_ARGS);
+    else SUCEM_LAUNCH((k_aux_t<R, LV, LE, 2, false>), Gt, TB, 0, c->stream, VT_ARGS);
     if (vinv) SUCEM_LAUNCH((k_aux_z<R, 2, true>), G, TB, 0, c->stream, Z_ARGS);
     else SUCEM_LAUNCH((k_aux_z<R, 2, false>), G, TB, 0, c->stream, Z_ARGS);
   } else {
-    if (vinv) SUCEM_LAUNCH((k_aux_vt<R, LV, 1, true>), Gv, TB, 0, c->stream, VT_ARGS);
-    else SUCEM_LAUNCH((k_aux_vt<R, LV, 1, false>), Gv, TB, 0, c->stream, VT_ARGS);
+    if (vinv) SUCEM_LAUNCH((k_aux_t<R, LV, LE, 1, true>), Gv, TB, 0, c->stream, VT_ARGS);
+    else SUCEM_LAUNCH((k_aux_t<R, LV, LE, 1, false>), Gv, TB, 0, c->stream, VT_ARGS);
     if (vinv) SUCEM_LAUNCH((k_aux_z<R, 1, true>), G, TB, 0, c->stream, Z_ARGS);
     else SUCEM_LAUNCH((k_aux_z<R, 1, false>), G, TB, 0, c->stream, Z_ARGS);
   }

@@ -44,6 +44,10 @@ struct PatchMaps {
   DBuf<uint32_t> lp;                    // nloc * nc
   DBuf<int32_t> uptr, dof, out, fptr, qptr, qdof;
   DBuf<uint16_t> seg, fpos;
+  // per DOF: -1 (the DOF lies in one patch) or (first partial sum << 4) | their count,
+  // for consumers that add a shared DOF's partial sums themselves (empty when a DOF is
+  // shared by more than 15 patches)
+  DBuf<int32_t> qcode;
 };
 
 struct Space {

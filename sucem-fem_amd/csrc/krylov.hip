@@ -158,6 +158,39 @@ __global__ void k_cocg_xr_plain(int n, int G, const double* scal, C* x, C* r, co
   }
 }
 
+// The same update when the operator left the sums of DOFs shared by several patches
+// unsummed: for those rows (qcode >= 0) q is the sum of their partial sums, in patch order
+// (zero on masked rows).
+template <typename C>
+__global__ void k_cocg_xr_def(int n, int G, const double* scal, C* x, C* r, const C* p,
+                              const C* q, const int32_t* __restrict__ qcode,
+                              const C* __restrict__ part, const uint8_t* __restrict__ mask) {
+  const C a = from_d<C>(scal2(scal, S_ALPHA));
+  Range rr = block_rows(n, G);
+  for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
+    const int code = qcode[i];
+    C qi;
+    if (code < 0) {
+      qi = q[i];
+    } else {
+      qi = mk(decltype(qi.x)(0), decltype(qi.x)(0));
+      const int q0 = code >> 4, q1 = q0 + (code & 15);
+      for (int j = q0; j < q1; ++j) {
+        const C t = part[j];
+        qi.x += t.x;
+        qi.y += t.y;
+      }
+      if (mask && mask[i]) qi = mk(decltype(qi.x)(0), decltype(qi.x)(0));
+    }
+    const C ap = cmul(a, p[i]), aq = cmul(a, qi);
+    C xi = x[i], ri = r[i];
+    xi.x += ap.x; xi.y += ap.y;
+    ri.x -= aq.x; ri.y -= aq.y;
+    x[i] = xi;
+    r[i] = ri;
+  }
+}
+
 // p = z + beta p
 template <typename C>
 __global__ void k_cocg_pz(int n, int G, const double* scal, const C* z, C* p) {
@@ -284,6 +317,11 @@ struct Op {
   // vector nodal term of the auxiliary-space preconditioner (3 nv entries)
   virtual int64_t nnod() { return 0; }
   virtual void vinv_lo(void* vinv) {}
+  // operator form that leaves shared rows as partial sums (COCG): block partials, or 0
+  virtual int deferred_parts() { return 0; }
+  virtual void apply_deferred(const void* x, void* y, const void* w, const RedSite& rs) {}
+  virtual const int32_t* qcode() { return nullptr; }
+  virtual const void* partials() { return nullptr; }
   virtual ~Op() {}
 };
 
@@ -322,6 +360,12 @@ struct SysOp : Op {
   void vinv_lo(void* vinv) override {
     nodal_winv<R>(c, const_cast<System*>(sys), k, mask != nullptr, vinv);
   }
+  int deferred_parts() override { return pt_deferred_parts(sys, flo); }
+  void apply_deferred(const void* x, void* y, const void* w, const RedSite& rs) override {
+    pt_apply_deferred<R>(c, sys, k, mask, x, y, w, rs, flo);
+  }
+  const int32_t* qcode() override { return pt_qcode(sys, flo); }
+  const void* partials() override { return pt_partials(sys); }
 };
 
 template <typename R>
@@ -478,8 +522,24 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     if (sample) timer.stop(c);
     finish_reduction(c, rs, op.flo.parts());
   };
+  // COCG with the patch operator: the shared rows' partial sums are added by the x/r update
+  const int Gd = (cocg && grad) ? op.deferred_parts() : 0;
+  auto apply_deferred = [&](const void* xv, void* yv, const RedSite& rs) {
+    const bool sample = it >= o.check_every && it % o.check_every == 0;
+    if (sample) timer.start(c);
+    op.apply_deferred(xv, yv, xv, rs);
+    if (sample) timer.stop(c);
+    finish_reduction(c, rs, Gd);
+  };
   for (it = 0; it < o.max_iter; ++it) {
-    if (cocg && grad) {
+    if (cocg && grad && Gd > 0) {
+      apply_deferred(p.p, q.p, site(SITE_SPMV1, FIN_COCG_ALPHA));
+      SUCEM_LAUNCH((k_cocg_xr_def<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, p.p, q.p,
+                   op.qcode(), (const C*)op.partials(), op.mask);
+      precondition(site(SITE_UPDATE, FIN_COCG_BETA, it));
+      SUCEM_LAUNCH((k_cocg_pz<C>), G, TB, 0, c->stream, n, G, scal.p, z.p, p.p);
+      st.spmv_lo += 1;
+    } else if (cocg && grad) {
       apply_epi(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
       SUCEM_LAUNCH((k_cocg_xr_plain<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, p.p, q.p);
       precondition(site(SITE_UPDATE, FIN_COCG_BETA, it));

@@ -35,6 +35,16 @@ bool pt_prepare(Context* c, System* sys, int cb);
 template <typename R>
 void pt_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x, void* y,
               const void* w, int epi, const RedSite& rs, const SpmvCfg& f);
+// Patch operator for COCG with the shared DOFs' partial sums left unsummed: y holds the
+// rows of DOFs that lie in one patch, the partial sums stay in the system's buffer (rows
+// marked by pt_qcode), and the p.y epilogue (linear in y) covers both; rs gets
+// pt_deferred_parts(sys, f) block partials (0: the form is not available for f).
+int pt_deferred_parts(const System* sys, const SpmvCfg& f);
+template <typename R>
+void pt_apply_deferred(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
+                       void* y, const void* w, const RedSite& rs, const SpmvCfg& f);
+const int32_t* pt_qcode(const System* sys, const SpmvCfg& f);
+const void* pt_partials(const System* sys);
 // whether the patch kernel's LDS need in precision R fits the device
 template <typename R>
 bool pt_fits(const Space* s, int cb);

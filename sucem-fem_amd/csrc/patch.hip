@@ -225,6 +225,22 @@ __global__ void k_pt_outs(int64_t nu, const uint32_t* dk, const uint32_t* du,
   }
 }
 
+__global__ void k_set_i32_pt(int64_t n, int32_t* a, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+__global__ void k_pt_qcode(int64_t npd, const int32_t* qptr, const int32_t* qdof, int32_t* qcode,
+                           int32_t* too_many) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < npd;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int cnt = qptr[k + 1] - qptr[k];
+    if (cnt > 15) atomicMax(too_many, 1);
+    qcode[qdof[k]] = (qptr[k] << 4) | (cnt & 15);
+  }
+}
+
 template <typename T>
 __global__ void k_pt_permute(int64_t n, int k, const int32_t* from, const T* src, T* dst) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * k;
@@ -373,10 +389,16 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
         if (hi[q] > 0) {   // an entry of this patch (segments are never empty)
           if (out[q] >= 0) {
             part[out[q]] = v;
+            // EPI 3: the consumer adds the partial sums; the dot product is linear in y
+            if (EPI == 3 && !(mask && mask[dv[q]])) {
+              const double2 yd = to_d(v), wd = to_d(w[dv[q]]);
+              d1.x += wd.x * yd.x - wd.y * yd.y;
+              d1.y += wd.x * yd.y + wd.y * yd.x;
+            }
           } else {
             if (mask && mask[dv[q]]) v = mk(R(0), R(0));
             y[dv[q]] = v;
-            if (EPI) epilogue<EPI>(d1, d2, v, w[dv[q]]);
+            if (EPI) epilogue<EPI == 3 ? 1 : EPI>(d1, d2, v, w[dv[q]]);
           }
         }
       }
@@ -618,6 +640,16 @@ void pt_build_maps(Context* c, Space* s, PatchMaps& pm, int cb) {
   pm.maxfr = (int)maxfr;
   pm.cb = cb;
   pt_validate(c, s, pm, ps, up, fp);
+  if (npe < (int64_t(1) << 27)) {
+    pm.qcode.alloc(s->ndofs);
+    DBuf<int32_t> too_many(1);
+    SUCEM_LAUNCH(k_set_i32_pt, grid_for(s->ndofs, TB), TB, 0, c->stream, s->ndofs, pm.qcode.p, -1);
+    SUCEM_LAUNCH(k_set_i32_pt, 1, 64, 0, c->stream, (int64_t)1, too_many.p, 0);
+    if (npd)
+      SUCEM_LAUNCH(k_pt_qcode, grid_for(npd, TB), TB, 0, c->stream, npd, pm.qptr.p, pm.qdof.p,
+                   pm.qcode.p, too_many.p);
+    if (read_i32(c, too_many.p)) pm.qcode.release();
+  }
 }
 
 // Host check of every index the patch kernels follow (one pass over the maps when they
@@ -696,7 +728,8 @@ void launch_patch(Context* c, const System* sys, const PatchMaps& pm, Coef k,
                 (const C*)w, rs
   if (epi == 0) SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 0, CB>), Gp, CB, lds, c->stream, PT_ARGS);
   else if (epi == 1) SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 1, CB>), Gp, CB, lds, c->stream, PT_ARGS);
-  else SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 2, CB>), Gp, CB, lds, c->stream, PT_ARGS);
+  else if (epi == 2) SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 2, CB>), Gp, CB, lds, c->stream, PT_ARGS);
+  else SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 3, CB>), Gp, CB, lds, c->stream, PT_ARGS);
 #undef PT_ARGS
 }
 
@@ -796,6 +829,43 @@ void pt_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
 }
 template void pt_apply<float>(Context*, const System*, Coef, const uint8_t*, const void*, void*,
                               const void*, int, const RedSite&, const SpmvCfg&);
+
+int pt_deferred_parts(const System* sys, const SpmvCfg& f) {
+  if (f.mf != 2 || !sys->pt_ready) return 0;
+  const int slot = cb_slot(f.U);
+  if (slot < 0) return 0;
+  const PatchMaps& pm = sys->space->pt[slot];
+  if (pm.cb != f.U || !pm.qcode.p) return 0;
+  return (int)((pm.np + 7) / 8 * 8);
+}
+
+template <typename R>
+void pt_apply_deferred(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,
+                       void* y, const void* w, const RedSite& rs, const SpmvCfg& f) {
+  const int Gp = pt_deferred_parts(sys, f);
+  SUCEM_CHECK(Gp > 0, SUCEM_ERR_INVALID, "deferred patch operator not available");
+  SUCEM_CHECK(pt_fits<R>(sys->space, f.U), SUCEM_ERR_UNSUPPORTED,
+              "the patch operator needs more LDS per workgroup than the device has");
+  const PatchMaps& pm = sys->space->pt[cb_slot(f.U)];
+  if (sys->space->order == 2) {
+    if (f.U == 128) launch_patch<R, 2, 8, 128>(c, sys, pm, k, mask, x, y, w, 3, rs, Gp, Gp);
+    else if (f.U == 192) launch_patch<R, 2, 8, 192>(c, sys, pm, k, mask, x, y, w, 3, rs, Gp, Gp);
+    else launch_patch<R, 2, 8, 256>(c, sys, pm, k, mask, x, y, w, 3, rs, Gp, Gp);
+  } else {
+    launch_patch<R, 1, 3, 256>(c, sys, pm, k, mask, x, y, w, 3, rs, Gp, Gp);
+  }
+}
+template void pt_apply_deferred<float>(Context*, const System*, Coef, const uint8_t*,
+                                       const void*, void*, const void*, const RedSite&,
+                                       const SpmvCfg&);
+template void pt_apply_deferred<double>(Context*, const System*, Coef, const uint8_t*,
+                                        const void*, void*, const void*, const RedSite&,
+                                        const SpmvCfg&);
+
+const int32_t* pt_qcode(const System* sys, const SpmvCfg& f) {
+  return sys->space->pt[cb_slot(f.U)].qcode.p;
+}
+const void* pt_partials(const System* sys) { return sys->pt_part.p; }
 template void pt_apply<double>(Context*, const System*, Coef, const uint8_t*, const void*, void*,
                                const void*, int, const RedSite&, const SpmvCfg&);
 

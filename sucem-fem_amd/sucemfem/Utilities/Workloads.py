@@ -174,14 +174,29 @@ class DeviceProblem(object):
         return dict(zip(("patches", "patch_dofs", "shared_dofs", "partial_sums", "lds",
                          "face_rows"), (int(v) for v in info)))
 
-    def operator_bytes(self, precision, matrix_free, epilogue=False, cb=256):
+    def operator_bytes(self, precision, matrix_free, epilogue=False, cb=256, deferred=False):
         """Algorithmic HBM bytes of one application of the fused operator (one vector);
         ``epilogue``: plus the read of w for the fused dot product sum_r w_r y_r;
         ``matrix_free`` 2 (patches of ``cb`` cells): per cell 12 coefficients and nloc
         (patch-DOF index, product place) pairs, per patch DOF its id, segment end,
         destination and x value, per shared DOF its id and segment pointer, the partial
         sums written and read, per boundary face 6 coefficients, its cell and local face
-        and its products' places, y once."""
+        and its products' places, y once.  ``deferred`` (patch form inside COCG): the shared
+        DOFs' partial sums are written but summed by the x/r update, so the bytes are the
+        patch kernel's: no shared-DOF segments or partial reads, y written for the DOFs of
+        one patch only, and w read at every finished row and every partial sum."""
+        if deferred and matrix_free == 2:
+            s = 4 if precision == 32 else 8
+            sx = 2 * s
+            nc = int(np.prod(self.n)) * 6
+            nbf = int(self.space.info()[4])
+            nloc, nfd = (6, 3) if self.order == 1 else (20, 8)
+            pi = self.patch_info(cb)
+            own = self.ndofs - pi["shared_dofs"]
+            return (nc * (12 * s + nloc * 4) + pi["patch_dofs"] * (4 + 2 + 4 + sx)
+                    + pi["partial_sums"] * sx + nbf * (6 * s + 4 + nfd * 2)
+                    + 2 * (pi["patches"] + 1) * 4 + own * sx
+                    + ((own + pi["partial_sums"]) * sx if epilogue else 0))
         if epilogue:
             return (self.operator_bytes(precision, matrix_free, cb=cb)
                     + self.ndofs * (8 if precision == 32 else 16))
