@@ -217,9 +217,11 @@ int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const doubl
                        double* res_hist_h, int64_t hist_cap);
 
 /* nb driven systems on one assembled system, system j at wavenumber k0_h[j]:
- * (S - k0_j^2 T + i k0_j S0) x_j = b_j.  B_d, X_d: nb x N complex128 (system-major).  The
- * matrix is read once per iteration for a batch of up to 16 systems (vectors interleaved
- * internally).  FP64 COCG with SUCEM_PRECOND_GRADIENT only.  stats_h (nullable): nb entries.
+ * (S - k0_j^2 T + i k0_j S0) x_j = b_j.  B_d, X_d: nb x N complex128 (system-major).  With
+ * FP64 COCG and SUCEM_PRECOND_GRADIENT the assembled matrix is read once per iteration for
+ * a batch of up to 16 systems (vectors interleaved internally); any other options (the
+ * default, opts = NULL: SUCEM_PRECOND_AUXILIARY) solve the systems one after another with
+ * the system's operator.  stats_h (nullable): nb entries.
  * (Frequency sweeps of the reference re-form and re-solve the LHS per frequency,
  * sandbox/patch/patch_driver.py:64-129.) */
 int sucem_solve_driven_batch(sucem_ctx* ctx, sucem_system* sys, int64_t nb, const double* k0_h,

@@ -651,7 +651,7 @@ int sucem_solve_driven_batch(sucem_ctx* ctx, sucem_system* sys, int64_t nb, cons
   c->use();
   std::vector<SolveStats> st(nb);
   SolveOpts o = convert(opts);
-  if (!opts) o.precond = SUCEM_PRECOND_GRADIENT;
+  if (!opts) o.precond = SUCEM_PRECOND_AUXILIARY;
   solve_system_batch(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, st.data());
   if (stats_h)
     for (int64_t j = 0; j < nb; ++j) convert_stats(st[j], stats_h + j);

@@ -383,7 +383,10 @@ __device__ __forceinline__ void aux_edge_rows(int64_t blk, int nv, int nE, R inv
 }
 
 // One launch for both kinds of G^T rows: blocks [0, Gv) take the vertex rows (with Pi^T),
-// blocks [Gv, gridDim) the edge-node rows, so the two gathers overlap on the device.
+// blocks [Gv, gridDim) the edge-node rows, so the two gathers overlap on the device.  Gv
+// and gridDim - Gv are multiples of 8 and each part is mapped XCD-aware (each XCD takes a
+// contiguous eighth of the vertices and of the edges): neighbouring rows gather the same
+// r entries, and only blocks on one XCD share an L2.
 template <typename R, int LV, int LE, int ORDER, bool NODAL>
 __global__ __launch_bounds__(TB) void k_aux_t(int Gv, int nv, int nE, R inv_den,
                                               const int32_t* __restrict__ ve_ptr,
@@ -400,11 +403,16 @@ __global__ __launch_bounds__(TB) void k_aux_t(int Gv, int nv, int nE, R inv_den,
                                               const typename C2<R>::T* __restrict__ vinv,
                                               typename C2<R>::T* __restrict__ w,
                                               typename C2<R>::T* __restrict__ u) {
-  if ((int)blockIdx.x < Gv)
-    aux_vertex_rows<R, LV, ORDER, NODAL>(blockIdx.x, nv, nE, inv_den, ve_ptr, ve_ent, vf_ptr,
-                                         vf_ent, edges, faces, xyz, r, winv, vinv, w, u);
-  else if constexpr (ORDER == 2)
-    aux_edge_rows<R, LE>(blockIdx.x - Gv, nv, nE, inv_den, ef_ptr, ef_ent, r, winv, w);
+  const int b = blockIdx.x;
+  if (b < Gv) {
+    const int lb = (b & 7) * (Gv >> 3) + (b >> 3);
+    aux_vertex_rows<R, LV, ORDER, NODAL>(lb, nv, nE, inv_den, ve_ptr, ve_ent, vf_ptr, vf_ent,
+                                         edges, faces, xyz, r, winv, vinv, w, u);
+  } else if constexpr (ORDER == 2) {
+    const int be = b - Gv, Ge = (int)gridDim.x - Gv;
+    const int lb = (be & 7) * (Ge >> 3) + (be >> 3);
+    aux_edge_rows<R, LE>(lb, nv, nE, inv_den, ef_ptr, ef_ent, r, winv, w);
+  }
 }
 
 // ---- vector nodal term --------------------------------------------------------------
@@ -766,14 +774,14 @@ void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv
   const int nv = (int)s->mesh->nv, nE = (int)s->nE;
   const int reduce = rs.mode != FIN_NONE ? 1 : 0;
   constexpr int LV = 16, LE = 4;   // lanes per vertex / edge row (measured: 32, 64 and 8 slower)
-  const int Gv = (int)(((int64_t)nv * LV + TB - 1) / TB);
+  const int Gv = (int)(((int64_t)nv * LV + 8 * TB - 1) / (8 * TB) * 8);   // multiple of 8
 #define VT_ARGS Gv, nv, nE, inv_den, s->ve_ptr.p, s->ve_ent.p, s->vf_ptr.p, s->vf_ent.p, \
                 s->ef_ptr.p, s->ef_ent.p, s->edges.p, s->faces.p, xyz, (const C*)r, \
                 (const C*)winv, (const C*)vinv, (C*)w, (C*)u
 #define Z_ARGS (int)s->ndofs, G, nv, nE, inv_den, s->edges.p, s->faces.p, s->face_edges.p, xyz, \
                (const C*)w, (const C*)u, (const C*)dinv, (const C*)r, (C*)z, mask, reduce, rs
   if (s->order == 2) {
-    const int Gt = Gv + (int)(((int64_t)nE * LE + TB - 1) / TB);
+    const int Gt = Gv + (int)(((int64_t)nE * LE + 8 * TB - 1) / (8 * TB) * 8);
     if (vinv) SUCEM_LAUNCH((k_aux_t<R, LV, LE, 2, true>), Gt, TB, 0, c->stream, VT_ARGS);
     else SUCEM_LAUNCH((k_aux_t<R, LV, LE, 2, false>), Gt, TB, 0, c->stream, VT_ARGS);
     if (vinv) SUCEM_LAUNCH((k_aux_z<R, 2, true>), G, TB, 0, c->stream, Z_ARGS);

@@ -105,16 +105,19 @@ class DeviceProblem(object):
                   ctypes.byref(st), None, 0)
         return st.as_dict()
 
-    def solve_batch(self, k0s, B, X, tol=1e-8, max_iter=50000, check_every=16):
-        """Solve len(k0s) systems at once: B, X are (nb, N) complex128 device tensors."""
+    def solve_batch(self, k0s, B, X, tol=1e-8, max_iter=50000, check_every=16,
+                    precond=_hip.PRECOND_AUXILIARY, precision=64):
+        """Solve len(k0s) systems: B, X are (nb, N) complex128 device tensors.  FP64 with
+        the gradient preconditioner runs them interleaved on the assembled matrix; other
+        options one after another with the system's operator."""
         import torch
         nb = len(k0s)
         self._check_vec(B, torch.complex128, nb)
         self._check_vec(X, torch.complex128, nb)
         o = _hip.SolveOpts()
         _hip.load().sucem_solve_opts_default(ctypes.byref(o))
-        o.method, o.precision, o.tol, o.max_iter = _hip.METHOD_COCG, 64, tol, max_iter
-        o.check_every, o.precond = check_every, _hip.PRECOND_GRADIENT
+        o.method, o.precision, o.tol, o.max_iter = _hip.METHOD_COCG, precision, tol, max_iter
+        o.check_every, o.precond = check_every, precond
         st = (_hip.SolveStats * nb)()
         k0 = np.ascontiguousarray(k0s, np.float64)
         _hip.call("sucem_solve_driven_batch", self.ctx.handle, self.system.handle.h, nb,
