@@ -769,10 +769,10 @@ def gradient_diag(A, G):
     return np.asarray(G.multiply(AG).sum(axis=0)).ravel()
 
 
-def cocg_gradient(A, b, G, tol=1e-8, maxiter=10000, gdiag=None):
+def cocg_gradient(A, b, G, tol=1e-8, maxiter=10000, gdiag=None, theta_g=2.0):
     """COCG with the additive gradient-space preconditioner, FP64 (scipy CSR).
 
-    M^-1 r = D^-1 r + G diag(G^T A G)^-1 G^T r, D = diag(A); Lagrange nodes whose
+    M^-1 r = D^-1 r + theta_g G diag(G^T A G)^-1 G^T r, D = diag(A); Lagrange nodes whose
     diagonal is zero (none on a mesh without PEC walls) are dropped. Same
     recurrences as ``cocg_jacobi`` with z = M^-1 r. Returns ``(x, iterations, history)``.
     """
@@ -784,7 +784,7 @@ def cocg_gradient(A, b, G, tol=1e-8, maxiter=10000, gdiag=None):
     D = 1.0 / A.diagonal()
     W = np.zeros(len(gdiag), np.complex128)
     nz = gdiag != 0
-    W[nz] = 1.0 / gdiag[nz]
+    W[nz] = theta_g / gdiag[nz]
 
     def prec(r):
         return D * r + G @ (W * (Gt @ r))
@@ -843,11 +843,12 @@ def nodal_interpolation(space):
     return Pi
 
 
-def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000, gdiag=None, vdiag=None):
+def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000, gdiag=None, vdiag=None,
+                   theta_g=3.0):
     """COCG with the auxiliary-space preconditioner, FP64 (scipy CSR):
-    M^-1 r = D^-1 r + G diag(G^T A G)^-1 G^T r + Pi diag(Pi^T A Pi)^-1 Pi^T r
-    (zero diagonals dropped; ``gdiag`` / ``vdiag`` precomputed diagonals).  Returns
-    ``(x, iterations, history)``."""
+    M^-1 r = D^-1 r + theta_g G diag(G^T A G)^-1 G^T r + Pi diag(Pi^T A Pi)^-1 Pi^T r
+    (zero diagonals dropped; ``gdiag`` / ``vdiag`` precomputed diagonals; the device uses
+    theta_g = 3 at order 2 and 2 at order 1).  Returns ``(x, iterations, history)``."""
     A = A.tocsr()
     D = 1.0 / A.diagonal()
 
@@ -856,7 +857,7 @@ def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000, gdiag=None, vdiag=None)
         nz = d != 0
         w[nz] = 1.0 / d[nz]
         return w
-    W = inv(gradient_diag(A, G) if gdiag is None else gdiag)
+    W = theta_g * inv(gradient_diag(A, G) if gdiag is None else gdiag)
     V = inv(gradient_diag(A, Pi) if vdiag is None else vdiag)
     G = G.tocsr().astype(np.complex128)
     Gt = G.T.tocsr()

@@ -230,7 +230,8 @@ __global__ void k_bdinv(int64_t n, const int32_t* diag, const double2* st, const
 }
 
 template <int NB>
-__global__ void k_bwinv(int64_t nlag, const double* d3, const double* coef, C* winv) {
+__global__ void k_bwinv(int64_t nlag, const double* d3, const double* coef, double theta,
+                        C* winv) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nlag * NB;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = e / NB;
@@ -238,7 +239,7 @@ __global__ void k_bwinv(int64_t nlag, const double* d3, const double* coef, C* w
     const double re = coef[0] * d3[3 * v] + coef[1 + j] * d3[3 * v + 1];
     const double im = coef[1 + NB + j] * d3[3 * v + 2];
     const double den = re * re + im * im;
-    winv[e] = den > 0.0 ? make_double2(re / den, -im / den) : make_double2(0, 0);
+    winv[e] = den > 0.0 ? make_double2(theta * re / den, -theta * im / den) : make_double2(0, 0);
   }
 }
 
@@ -377,7 +378,8 @@ void solve_chunk(Context* c, System* sys, int nb, const double* k0, const double
   SUCEM_LAUNCH((k_interleave<NB>), grid_for(ne, TB), TB, 0, c->stream, (int64_t)n, nb, B, b.p);
   SUCEM_LAUNCH((k_bdinv<NB>), grid_for(ne, TB), TB, 0, c->stream, (int64_t)n, s->diag.p, sys->st64.p,
                has_s0 ? s->s0_diag.p : nullptr, sys->s0_64.p, coef_d.p, mask, dinv.p);
-  SUCEM_LAUNCH((k_bwinv<NB>), grid_for(nl * NB, TB), TB, 0, c->stream, nl, d3, coef_d.p, winv.p);
+  SUCEM_LAUNCH((k_bwinv<NB>), grid_for(nl * NB, TB), TB, 0, c->stream, nl, d3, coef_d.p,
+               gradient_weight(s->order, false), winv.p);
   auto finish = [&](const RedSite& rs) {
     SUCEM_LAUNCH((k_finalize_nb<NB>), 1, TB, 0, c->stream, rs, G);
   };

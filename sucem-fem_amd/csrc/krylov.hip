@@ -312,6 +312,7 @@ struct Op {
   // gradient-space preconditioner (driven / shifted systems only)
   virtual int64_t nlag() { return 0; }
   virtual void winv_lo(void* winv) {}
+  bool aux = false;   // the auxiliary-space form (weights of the gradient term differ)
   virtual void precond_lo(const void* winv, const void* dinv, const void* r, void* w, void* z,
                           int G, const RedSite& rs, const void* vinv, void* u) {}
   // vector nodal term of the auxiliary-space preconditioner (3 nv entries)
@@ -347,7 +348,7 @@ struct SysOp : Op {
     return sys->space->nlag;
   }
   void winv_lo(void* winv) override {
-    grad_winv<R>(c, const_cast<System*>(sys), k, mask != nullptr, winv);
+    grad_winv<R>(c, const_cast<System*>(sys), k, mask != nullptr, aux, winv);
   }
   void precond_lo(const void* winv, const void* dinv, const void* r, void* w, void* z, int G,
                   const RedSite& rs, const void* vinv, void* u) override {
@@ -479,6 +480,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   };
 
   op.dinv_lo(dinv.p);
+  op.aux = aux;
   if (grad) op.winv_lo(winv.p);
   if (aux) op.vinv_lo(vinv.p);
   SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, x);

@@ -57,7 +57,8 @@ void finish_reduction(Context* c, const RedSite& rs, int G);
 
 // gradient-space preconditioner (hiptmair.hip)
 template <typename R>
-void grad_winv(Context* c, System* sys, Coef k, bool masked, void* winv);
+void grad_winv(Context* c, System* sys, Coef k, bool masked, bool aux, void* winv);
+double gradient_weight(int order, bool aux);
 // vinv (nullable): the auxiliary-space form, vinv = diag(Pi^T A Pi)^-1 (3 nv) from
 // nodal_winv, u a 3 nv work vector
 template <typename R>

@@ -402,7 +402,8 @@ def test_auxiliary_preconditioner_iterations_match_oracle(dev, order, pec):
         G, Pi = O.discrete_gradient(osp), O.nodal_interpolation(osp)
         free = np.arange(osp.ndofs)
         cs, ct, c0 = 1.0, -k0 * k0, k0
-    xo, its, _ = O.cocg_auxiliary(A, bh[free], G, Pi, tol=1e-9, maxiter=20000)
+    xo, its, _ = O.cocg_auxiliary(A, bh[free], G, Pi, tol=1e-9, maxiter=20000,
+                                  theta_g=3.0 if order == 2 else 2.0)
     b = torch.as_tensor(bh, device="cuda")
     for precision in (64, 32):
         x = torch.zeros_like(b)
