@@ -37,8 +37,10 @@ void pt_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
               const void* w, int epi, const RedSite& rs, const SpmvCfg& f);
 // Patch operator for COCG with the shared DOFs' partial sums left unsummed: y holds the
 // rows of DOFs that lie in one patch, the partial sums stay in the system's buffer (rows
-// marked by pt_qcode), and the p.y epilogue (linear in y) covers both; rs gets
-// pt_deferred_parts(sys, f) block partials (0: the form is not available for f).
+// marked by pt_qcode), and the epilogue is x.y (FP64: the sum of the element quadratic
+// forms; FP32: over the finished rows and the partial sums) -- the caller's w must be x,
+// zero on masked rows; rs gets pt_deferred_parts(sys, f) block partials (0: the form is
+// not available for f).
 int pt_deferred_parts(const System* sys, const SpmvCfg& f);
 template <typename R>
 void pt_apply_deferred(Context* c, const System* sys, Coef k, const uint8_t* mask, const void* x,

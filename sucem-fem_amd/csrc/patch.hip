@@ -38,7 +38,8 @@ namespace sucem {
 namespace {
 
 constexpr int TB = 256;    // threads of the set-up kernels and of k_mf_psum
-constexpr int UB = 8;      // patch DOFs per thread and batch in k_mf_patch
+constexpr int UB = 8;      // patch DOFs per thread and batch in k_mf_patch (16: one wave
+                           // per SIMD fewer in FP32)
 constexpr int kCbs[3] = {128, 192, 256};   // cells per patch = threads of k_mf_patch
 
 int cb_slot(int cb) { return cb == 128 ? 0 : (cb == 192 ? 1 : (cb == 256 ? 2 : -1)); }
@@ -330,6 +331,18 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
 #pragma unroll
       for (int i = 0; i < NLOC; ++i) acc[i] = mk(R(0), R(0));
       element_product<R, ORDER>(a, xl, acc, std::make_integer_sequence<int, MfTab<ORDER>::NP>{});
+      if constexpr (EPI == 3 && sizeof(R) == 8) {
+        // p.Ap as the sum of the element quadratic forms x_c^T K_c x_c (w = x in COCG;
+        // x is zero on masked rows there, so the masked rows add nothing either way).
+        // FP64 only: in FP32 the extra registers cost a wave per SIMD, and the sums of
+        // the finished rows and partial sums below are cheaper.
+#pragma unroll
+        for (int i = 0; i < NLOC; ++i) {
+          const double2 xd = to_d(xl[i]), yd = to_d(acc[i]);
+          d1.x += xd.x * yd.x - xd.y * yd.y;
+          d1.y += xd.x * yd.y + xd.y * yd.x;
+        }
+      }
     }
     // the patch's boundary faces, one (face, row) pair per thread and pass, staged in LDS
     // behind the x copy (stage[id]) until the x copy may be overwritten
@@ -358,6 +371,12 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
           v.y += m * xb.y;
         }
         stage[id] = mk(-c0 * v.y, c0 * v.x);
+        if constexpr (EPI == 3 && sizeof(R) == 8) {
+          const C xa = sm[lp[(int64_t)face_dofs[lf * NFD + a] * nc + rk] & 0xffffu];
+          const double2 xd = to_d(xa), yd = to_d(stage[id]);
+          d1.x += xd.x * yd.x - xd.y * yd.y;
+          d1.y += xd.x * yd.y + xd.y * yd.x;
+        }
       }
     }
     __syncthreads();   // every read of the x copy is done: the products overwrite it
@@ -389,16 +408,13 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
         if (hi[q] > 0) {   // an entry of this patch (segments are never empty)
           if (out[q] >= 0) {
             part[out[q]] = v;
-            // EPI 3: the consumer adds the partial sums; the dot product is linear in y
-            if (EPI == 3 && !(mask && mask[dv[q]])) {
-              const double2 yd = to_d(v), wd = to_d(w[dv[q]]);
-              d1.x += wd.x * yd.x - wd.y * yd.y;
-              d1.y += wd.x * yd.y + wd.y * yd.x;
-            }
+            // FP32: x.y over the partial sums (linear in y; x is zero on masked rows)
+            if constexpr (EPI == 3 && sizeof(R) == 4) epilogue<1>(d1, d2, v, w[dv[q]]);
           } else {
             if (mask && mask[dv[q]]) v = mk(R(0), R(0));
             y[dv[q]] = v;
-            if (EPI) epilogue<EPI == 3 ? 1 : EPI>(d1, d2, v, w[dv[q]]);
+            if constexpr (EPI == 1 || EPI == 2) epilogue<EPI>(d1, d2, v, w[dv[q]]);
+            else if constexpr (EPI == 3 && sizeof(R) == 4) epilogue<1>(d1, d2, v, w[dv[q]]);
           }
         }
       }

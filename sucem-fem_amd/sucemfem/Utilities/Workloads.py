@@ -187,7 +187,8 @@ class DeviceProblem(object):
         and its products' places, y once.  ``deferred`` (patch form inside COCG): the shared
         DOFs' partial sums are written but summed by the x/r update, so the bytes are the
         patch kernel's: no shared-DOF segments or partial reads, y written for the DOFs of
-        one patch only, and w read at every finished row and every partial sum."""
+        one patch only, and (FP32) w read at every finished row and every partial sum;
+        FP64 forms the epilogue from the element quadratic forms and reads no w."""
         if deferred and matrix_free == 2:
             s = 4 if precision == 32 else 8
             sx = 2 * s
@@ -199,7 +200,8 @@ class DeviceProblem(object):
             return (nc * (12 * s + nloc * 4) + pi["patch_dofs"] * (4 + 2 + 4 + sx)
                     + pi["partial_sums"] * sx + nbf * (6 * s + 4 + nfd * 2)
                     + 2 * (pi["patches"] + 1) * 4 + own * sx
-                    + ((own + pi["partial_sums"]) * sx if epilogue else 0))
+                    + ((own + pi["partial_sums"]) * sx if epilogue and precision == 32
+                       else 0))
         if epilogue:
             return (self.operator_bytes(precision, matrix_free, cb=cb)
                     + self.ndofs * (8 if precision == 32 else 16))
