@@ -248,6 +248,108 @@ __global__ __launch_bounds__(TB) void k_aux_z(int n, int G, int nv, int nE, R in
   if (reduce) reduce_finalize<TB>(rs, G, d1, d2);
 }
 
+// Order 2: the two DOFs of an edge or a face read the same entity vertices, w, u and
+// coordinates, so one thread forms both rows: entity j < nE is the edge j, otherwise the
+// face j - nE, and its rows are 2j and 2j + 1 in both cases (r, dinv and z move as one
+// 16-byte (FP32) pair per thread).  Same sums as aux_row for k = 0 and k = 1.
+template <typename C>
+__device__ __forceinline__ void ld_pair(const C* __restrict__ a, int64_t j, C& v0, C& v1) {
+  if constexpr (sizeof(C) == 8) {
+    const float4 t = reinterpret_cast<const float4*>(a)[j];
+    v0 = mk(t.x, t.y);
+    v1 = mk(t.z, t.w);
+  } else {
+    v0 = a[2 * j];
+    v1 = a[2 * j + 1];
+  }
+}
+
+template <typename R, bool NODAL>
+__global__ __launch_bounds__(TB) void k_aux_z2(int n2, int G, int nv, int nE, R inv_den,
+                                               const int32_t* __restrict__ edges,
+                                               const int32_t* __restrict__ faces,
+                                               const int32_t* __restrict__ face_edges,
+                                               const R* __restrict__ xyz,
+                                               const typename C2<R>::T* __restrict__ w,
+                                               const typename C2<R>::T* __restrict__ u,
+                                               const typename C2<R>::T* __restrict__ dinv,
+                                               const typename C2<R>::T* __restrict__ r,
+                                               typename C2<R>::T* __restrict__ z,
+                                               const uint8_t* __restrict__ mask, int reduce,
+                                               RedSite rs) {
+  using C = typename C2<R>::T;
+  const int lb = xcd_block(G);
+  const int chunk = (n2 + G - 1) / G;
+  const int start = lb * chunk, end = min(n2, start + chunk);
+  double2 d1 = make_double2(0, 0), d2 = make_double2(0, 0);
+  for (int j = start + threadIdx.x; j < end; j += TB) {
+    C g0, g1, p0 = mk(R(0), R(0)), p1 = mk(R(0), R(0));
+    if (j < nE) {
+      const int a = edges[2 * j], b = edges[2 * j + 1];
+      const C wa = w[a], wb = w[b], we = w[nv + j];
+      g0 = mk(R(-5) * wa.x + wb.x + R(4) * we.x, R(-5) * wa.y + wb.y + R(4) * we.y);
+      g1 = mk(R(5) * wb.x - wa.x - R(4) * we.x, R(5) * wb.y - wa.y - R(4) * we.y);
+      if constexpr (NODAL) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const R t = xyz[3 * b + d] - xyz[3 * a + d];
+          const C ua = u[3 * a + d], ub = u[3 * b + d];
+          const R t1 = t * R(1.0 / 3.0), t2 = t * R(2.0 / 3.0);
+          p0.x += t2 * ua.x + t1 * ub.x;
+          p0.y += t2 * ua.y + t1 * ub.y;
+          p1.x += t1 * ua.x + t2 * ub.x;
+          p1.y += t1 * ua.y + t2 * ub.y;
+        }
+      }
+    } else {
+      const int f = j - nE;
+      const int fp = faces[3 * f], fq = faces[3 * f + 1], fr = faces[3 * f + 2];
+      const C wp = w[fp], wq = w[fq], wr = w[fr];
+      const C wpq = w[nv + face_edges[3 * f]], wpr = w[nv + face_edges[3 * f + 1]];
+      const C wqr = w[nv + face_edges[3 * f + 2]];
+      g0 = mk(wq.x - wp.x + R(4) * (wqr.x - wpr.x), wq.y - wp.y + R(4) * (wqr.y - wpr.y));
+      g1 = mk(wr.x - wp.x + R(4) * (wqr.x - wpq.x), wr.y - wp.y + R(4) * (wqr.y - wpq.y));
+      if constexpr (NODAL) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const R xp = xyz[3 * fp + d];
+          const R t0 = (xyz[3 * fq + d] - xp) * R(1.0 / 3.0);
+          const R t1 = (xyz[3 * fr + d] - xp) * R(1.0 / 3.0);
+          const C up = u[3 * fp + d], uq = u[3 * fq + d], ur = u[3 * fr + d];
+          const R sx = up.x + uq.x + ur.x, sy = up.y + uq.y + ur.y;
+          p0.x += t0 * sx;
+          p0.y += t0 * sy;
+          p1.x += t1 * sx;
+          p1.y += t1 * sy;
+        }
+      }
+    }
+    C r0, r1, e0, e1;
+    ld_pair(r, j, r0, r1);
+    ld_pair(dinv, j, e0, e1);
+    const C q0 = cmul(e0, r0), q1 = cmul(e1, r1);
+    C z0 = mk(q0.x + inv_den * g0.x + p0.x, q0.y + inv_den * g0.y + p0.y);
+    C z1 = mk(q1.x + inv_den * g1.x + p1.x, q1.y + inv_den * g1.y + p1.y);
+    if (mask) {
+      if (mask[2 * j]) z0 = mk(R(0), R(0));
+      if (mask[2 * j + 1]) z1 = mk(R(0), R(0));
+    }
+    if constexpr (sizeof(C) == 8) {
+      reinterpret_cast<float4*>(z)[j] = make_float4(z0.x, z0.y, z1.x, z1.y);
+    } else {
+      z[2 * j] = z0;
+      z[2 * j + 1] = z1;
+    }
+    if (reduce) {
+      const double2 ra = to_d(r0), za = to_d(z0), rb = to_d(r1), zb = to_d(z1);
+      d1.x += ra.x * za.x - ra.y * za.y + rb.x * zb.x - rb.y * zb.y;
+      d1.y += ra.x * za.y + ra.y * za.x + rb.x * zb.y + rb.y * zb.x;
+      d2.x += ra.x * ra.x + ra.y * ra.y + rb.x * rb.x + rb.y * rb.y;
+    }
+  }
+  if (reduce) reduce_finalize<TB>(rs, G, d1, d2);
+}
+
 // Vertex rows of G^T and Pi^T, L lanes per vertex v:
 //   w_v = winv_v inv_den (G^T r)_v,  u_(v,d) = vinv_(v,d) (Pi^T r)_(v,d)
 // over the edges at v (entries (e << 1) | [v = b]) and the faces at v (entries
@@ -794,8 +896,12 @@ void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv
     const int Gt = Gv + (int)(((int64_t)nE * LE + 8 * TB - 1) / (8 * TB) * 8);
     if (vinv) SUCEM_LAUNCH((k_aux_t<R, LV, LE, 2, true>), Gt, TB, 0, c->stream, VT_ARGS);
     else SUCEM_LAUNCH((k_aux_t<R, LV, LE, 2, false>), Gt, TB, 0, c->stream, VT_ARGS);
-    if (vinv) SUCEM_LAUNCH((k_aux_z<R, 2, true>), G, TB, 0, c->stream, Z_ARGS);
-    else SUCEM_LAUNCH((k_aux_z<R, 2, false>), G, TB, 0, c->stream, Z_ARGS);
+    const int n2 = (int)(s->ndofs / 2);   // 2 nE + 2 nF rows: one thread per edge or face
+#define Z2_ARGS n2, G, nv, nE, inv_den, s->edges.p, s->faces.p, s->face_edges.p, xyz, \
+                (const C*)w, (const C*)u, (const C*)dinv, (const C*)r, (C*)z, mask, reduce, rs
+    if (vinv) SUCEM_LAUNCH((k_aux_z2<R, true>), G, TB, 0, c->stream, Z2_ARGS);
+    else SUCEM_LAUNCH((k_aux_z2<R, false>), G, TB, 0, c->stream, Z2_ARGS);
+#undef Z2_ARGS
   } else {
     if (vinv) SUCEM_LAUNCH((k_aux_t<R, LV, LE, 1, true>), Gv, TB, 0, c->stream, VT_ARGS);
     else SUCEM_LAUNCH((k_aux_t<R, LV, LE, 1, false>), Gv, TB, 0, c->stream, VT_ARGS);

@@ -38,8 +38,12 @@ namespace sucem {
 namespace {
 
 constexpr int TB = 256;    // threads of the set-up kernels and of k_mf_psum
-constexpr int UB = 8;      // patch DOFs per thread and batch in k_mf_patch (16: one wave
-                           // per SIMD fewer in FP32)
+// patch DOFs per thread and batch in k_mf_patch: of the x gather (UB) and of the segment
+// sums (UD).  Patches of 256 cells hold about 2,300 DOFs at order 2, so 10 per thread
+// takes them in one batch; measured per FP32 application at 40^3, order 2: 8 / 10 / 12 /
+// 14 / 16 -> 64.6 / 59.9 / 60.5 / 60.5 / 62.0 us.
+constexpr int UB = 10;
+constexpr int UD = 10;
 constexpr int kCbs[3] = {128, 192, 256};   // cells per patch = threads of k_mf_patch
 
 int cb_slot(int cb) { return cb == 128 ? 0 : (cb == 192 ? 1 : (cb == 256 ? 2 : -1)); }
@@ -297,6 +301,20 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
   double2 d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0);
   if (p < np) {   // uniform over the workgroup
     const int u0 = uptr[p], nu = uptr[p + 1] - u0;
+    const int64_t r = p * CB + threadIdx.x;
+    const bool has = r < nc;
+    // FP32: the cell's coefficients and LDS places do not depend on x, so their loads go
+    // out with the x gather below and the two round trips overlap (in FP64 the held
+    // registers would cost a wave per SIMD)
+    constexpr bool PF = sizeof(R) == 4;
+    R pa[12];
+    uint32_t pli[NLOC];
+    if (PF && has) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) pa[k] = coef[k * nc + r];
+#pragma unroll
+      for (int i = 0; i < NLOC; ++i) pli[i] = lp[(int64_t)i * nc + r];
+    }
     // batches of UB entries per thread: all id loads, then all x loads, in flight together
     for (int j0 = 0; j0 < nu; j0 += UB * CB) {
       int dv[UB];
@@ -315,18 +333,21 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
       }
     }
     __syncthreads();
-    const int64_t r = p * CB + threadIdx.x;
-    const bool has = r < nc;
     C acc[NLOC];
     if (has) {
       R a[12];
-#pragma unroll
-      for (int k = 0; k < 12; ++k)
-        a[k] = coef[k * nc + r] * (k < 6 ? ct : cs) * R(1.0 / MfTab<ORDER>::den);
       C xl[NLOC];
+      if constexpr (PF) {
 #pragma unroll
-      for (int i = 0; i < NLOC; ++i) {
-        xl[i] = sm[lp[(int64_t)i * nc + r] & 0xffffu];
+        for (int k = 0; k < 12; ++k) a[k] = pa[k] * (k < 6 ? ct : cs) * R(1.0 / MfTab<ORDER>::den);
+#pragma unroll
+        for (int i = 0; i < NLOC; ++i) xl[i] = sm[pli[i] & 0xffffu];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+          a[k] = coef[k * nc + r] * (k < 6 ? ct : cs) * R(1.0 / MfTab<ORDER>::den);
+#pragma unroll
+        for (int i = 0; i < NLOC; ++i) xl[i] = sm[lp[(int64_t)i * nc + r] & 0xffffu];
       }
 #pragma unroll
       for (int i = 0; i < NLOC; ++i) acc[i] = mk(R(0), R(0));
@@ -384,12 +405,10 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
 #pragma unroll
       for (int i = 0; i < NLOC; ++i) sm[lp[(int64_t)i * nc + r] >> 16] = acc[i];
     }
-    for (int id = threadIdx.x; id < nfr; id += CB) sm[fpos[(int64_t)f0 * NFD + id]] = stage[id];
-    __syncthreads();
-    for (int j0 = 0; j0 < nu; j0 += UB * CB) {
-      int lo[UB], hi[UB], out[UB], dv[UB];
+    int lo[UD], hi[UD], out[UD], dv[UD];
+    auto load_ids = [&](int j0) {
 #pragma unroll
-      for (int q = 0; q < UB; ++q) {
+      for (int q = 0; q < UD; ++q) {
         const int j = j0 + q * CB + threadIdx.x;
         const bool in = j < nu;
         lo[q] = (in && j) ? (int)pseg[u0 + j - 1] : 0;
@@ -397,8 +416,14 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
         out[q] = in ? pout[u0 + j] : 0;
         dv[q] = in ? pdof[u0 + j] : 0;
       }
+    };
+    if constexpr (PF) load_ids(0);   // before the barrier: the loads overlap the wait
+    for (int id = threadIdx.x; id < nfr; id += CB) sm[fpos[(int64_t)f0 * NFD + id]] = stage[id];
+    __syncthreads();
+    for (int j0 = 0; j0 < nu; j0 += UD * CB) {
+      if (!PF || j0) load_ids(j0);
 #pragma unroll
-      for (int q = 0; q < UB; ++q) {
+      for (int q = 0; q < UD; ++q) {
         C v = mk(R(0), R(0));
         for (int t = lo[q]; t < hi[q]; ++t) {
           const C a = sm[t];
