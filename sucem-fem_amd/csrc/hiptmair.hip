@@ -376,6 +376,12 @@ __device__ __forceinline__ void aux_vertex_rows(int64_t blk, int nv, int nE, R i
 #pragma unroll
   for (int d = 0; d < 3; ++d) acc[d] = mk(R(0), R(0));
   if (v < nv) {
+    // this vertex's coordinates once: each edge or face then needs only its other vertices'
+    R xv[3];
+    if constexpr (NODAL) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xv[d] = xyz[3 * v + d];
+    }
     for (int q = ve_ptr[v] + lane; q < ve_ptr[v + 1]; q += L) {
       const int ent = ve_ent[q];
       const int e = ent >> 1;
@@ -396,10 +402,11 @@ __device__ __forceinline__ void aux_vertex_rows(int64_t blk, int nv, int nE, R i
         sr = mk(R(0.5) * r0.x, R(0.5) * r0.y);
       }
       if constexpr (NODAL) {
-        const int a = edges[2 * e], b = edges[2 * e + 1];
+        const int o = edges[2 * e + (second ? 0 : 1)];   // the edge's other vertex
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const R t = xyz[3 * b + d] - xyz[3 * a + d];
+          const R xo = xyz[3 * o + d];
+          const R t = second ? xv[d] - xo : xo - xv[d];   // x_b - x_a
           acc[d].x += t * sr.x;
           acc[d].y += t * sr.y;
         }
@@ -415,11 +422,16 @@ __device__ __forceinline__ void aux_vertex_rows(int64_t blk, int nv, int nE, R i
         else if (pos == 1) { g.x += r0.x; g.y += r0.y; }
         else { g.x += r1.x; g.y += r1.y; }
         if constexpr (NODAL) {
-          const int p = faces[3 * f], fq = faces[3 * f + 1], fr = faces[3 * f + 2];
+          // the face's other two vertices, in (p, q, r) order
+          const int o1 = faces[3 * f + (pos == 0 ? 1 : 0)], o2 = faces[3 * f + (pos == 2 ? 1 : 2)];
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            const R t0 = (xyz[3 * fq + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
-            const R t1 = (xyz[3 * fr + d] - xyz[3 * p + d]) * R(1.0 / 3.0);
+            const R y1 = xyz[3 * o1 + d], y2 = xyz[3 * o2 + d];
+            const R xp = pos == 0 ? xv[d] : y1;
+            const R xq = pos == 0 ? y1 : (pos == 1 ? xv[d] : y2);
+            const R xr = pos == 2 ? xv[d] : y2;
+            const R t0 = (xq - xp) * R(1.0 / 3.0);
+            const R t1 = (xr - xp) * R(1.0 / 3.0);
             acc[d].x += t0 * r0.x + t1 * r1.x;
             acc[d].y += t0 * r0.y + t1 * r1.y;
           }
