@@ -131,6 +131,11 @@ int sucem_mesh_translate(sucem_mesh* m, const double* shift3_h);
 int sucem_mesh_set_materials(sucem_mesh* m, const int64_t* region_h, int64_t nregions,
                              const int64_t* region_ids_h, const double* eps_r_h,
                              const double* mu_r_h);
+/* Per-cell eps_r and 1/mu_r given directly (nc doubles each, finite; zero weights allowed):
+ * the cellwise-constant material functions of CalcEMFunctional.set_epsr_function /
+ * set_mur_function (PostProcessing/PostProcessing.py:103-110), with 1/mu_r = eps_r = 0
+ * outside a cell-domain mark (set_cell_domains, PostProcessing.py:93-101). */
+int sucem_mesh_set_cell_materials(sucem_mesh* m, const double* eps_r_h, const double* inv_mu_r_h);
 int sucem_mesh_destroy(sucem_mesh* m);
 
 /* ---- function space (replaces dolfin.FunctionSpace(mesh, "Nedelec 1st kind H(curl)", order),

@@ -1,5 +1,6 @@
 // extern "C" entry points of libsucem_hip (declared in include/sucem_hip.h).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -223,6 +224,24 @@ int sucem_mesh_set_materials(sucem_mesh* m, const int64_t* region_h, int64_t nre
     int64_t rid = region_h ? region_h[bad] : 0;
     throw Error(SUCEM_ERR_MATERIAL, "Material number " + std::to_string(rid) + " not found");
   }
+  API_END
+}
+
+int sucem_mesh_set_cell_materials(sucem_mesh* m, const double* eps_r_h,
+                                  const double* inv_mu_r_h) {
+  API_BEGIN
+  NOT_NULL(m); NOT_NULL(eps_r_h); NOT_NULL(inv_mu_r_h);
+  Context* c = m->m.ctx;
+  c->use();
+  const int64_t nc = m->m.nc;
+  for (int64_t i = 0; i < nc; ++i)
+    SUCEM_CHECK(std::isfinite(eps_r_h[i]) && std::isfinite(inv_mu_r_h[i]), SUCEM_ERR_INVALID,
+                "cell materials must be finite");
+  m->m.eps_r.alloc(nc);
+  m->m.inv_mu_r.alloc(nc);
+  h2d(c, m->m.eps_r.p, eps_r_h, nc);
+  h2d(c, m->m.inv_mu_r.p, inv_mu_r_h, nc);
+  c->sync();
   API_END
 }
 

@@ -8,6 +8,7 @@
 // occurrence first) and rank entities by it.
 #include <string.h>
 
+#include <cmath>
 #include <cstring>
 #include <rocprim/rocprim.hpp>
 
@@ -528,8 +529,11 @@ void mesh_sort_cells(Context* c, Mesh* m) {
 int64_t mesh_set_materials(Context* c, Mesh* m, const int32_t* region_d, int64_t nreg,
                            const int64_t* ids_h, const double* eps_h, const double* mu_h) {
   SUCEM_CHECK(nreg > 0, SUCEM_ERR_INVALID, "empty material table");
-  for (int64_t i = 0; i < nreg; ++i)
+  for (int64_t i = 0; i < nreg; ++i) {
     SUCEM_CHECK(mu_h[i] != 0.0, SUCEM_ERR_INVALID, "mu_r must be nonzero");
+    SUCEM_CHECK(std::isfinite(mu_h[i]) && std::isfinite(eps_h[i]), SUCEM_ERR_INVALID,
+                "eps_r and mu_r must be finite");
+  }
   DBuf<int64_t> ids(nreg);
   DBuf<double> eps(nreg), mu(nreg);
   h2d(c, ids.p, ids_h, nreg);

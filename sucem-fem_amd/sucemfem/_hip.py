@@ -76,6 +76,7 @@ SIGNATURES = {
     "sucem_mesh_export": (ctypes.c_int, [_P, _P, _P]),
     "sucem_mesh_translate": (ctypes.c_int, [_P, _P]),
     "sucem_mesh_set_materials": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
+    "sucem_mesh_set_cell_materials": (ctypes.c_int, [_P, _P, _P]),
     "sucem_mesh_destroy": (ctypes.c_int, [_P]),
     "sucem_space_create": (ctypes.c_int, [_P, _P, ctypes.c_int, _PP]),
     "sucem_space_info": (ctypes.c_int, [_P, _P]),

@@ -123,3 +123,94 @@ def test_flux_of_device_solution_matches_oracle(dev):
     E = nt.calc_pts(th, ph)
     want = O.surface_ntff(osp, xh, 1e9, th, ph)
     assert np.abs(E - want).max() < 1e-10 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_reconstruct_points_matches_oracle(dev, order):
+    """Reconstruct.reconstruct_points (PostProcessing.py:38-60): the device's point
+    location and basis evaluation against the oracle's (lowest-index containing cell,
+    fixture-pinned basis) for random complex DOFs at random points, and exact
+    reproduction of a field the space contains (order 2: linear; order 1: a + b x x),
+    interpolated through Pi, which the reference's own basis pins independently."""
+    from sucemfem.PostProcessing import Reconstruct
+    _say("reconstruct points order %d" % order)
+    co, ce = O.box_mesh([0.3, 0.2, 0.25], 3)
+    rng = np.random.default_rng(11)
+    inner = np.all((co > 1e-9) & (co < np.array([0.3, 0.2, 0.25]) - 1e-9), axis=1)
+    co = co.copy()
+    co[inner] += 0.01 * rng.uniform(-1, 1, size=(inner.sum(), 3))
+    V = _space(co, ce, order)
+    osp = O.Space(co, ce, order)
+    assert np.array_equal(V.cell_dofs(), osp.cell_dofs)
+    pts = rng.uniform([0.01, 0.01, 0.01], [0.29, 0.19, 0.24], size=(40, 3))
+    x = rng.standard_normal(osp.ndofs) + 1j * rng.standard_normal(osp.ndofs)
+    dut = Reconstruct(V)
+    dut.set_dof_values(x)
+    got = dut.reconstruct_points(pts)
+    want = np.zeros((len(pts), 3), np.complex128)
+    for k, p in enumerate(pts):
+        c = O.locate_cell(osp.coords, osp.cells, p)
+        vals, _ = O.eval_basis(order, osp.coords[osp.cells[c]], p[None])
+        want[k] = vals[0].T @ x[osp.cell_dofs[c]]
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    a, bvec = rng.standard_normal(3), rng.standard_normal(3)
+    M = rng.standard_normal((3, 3))
+    field = (lambda q: a + np.cross(bvec, q)) if order == 1 else (lambda q: a + q @ M.T)
+    dofs = O.nodal_interpolation(osp) @ np.array([field(v) for v in co]).ravel()
+    dut.set_dof_values(dofs)
+    got = dut.reconstruct_points(pts)
+    assert got.dtype == np.float64
+    assert np.allclose(got, np.array([field(p) for p in pts]), atol=1e-11)
+
+
+def test_em_functional_symmetry_and_oracle(dev):
+    """CalcEMFunctional (PostProcessing.py:62-172): the reference's symmetry test
+    (test_Postprocessing.py:51-60, UnitCube(2,2,2), order 1, random E, g, k0) to rounding,
+    the free-space value against the oracle's em_functional, and, with per-cell eps_r and
+    mu_r restricted to one cell-domain mark, against the oracle's element matrices summed
+    over the marked cells (order 2)."""
+    from sucemfem.Mesh import UnitCube, FunctionSpace
+    from sucemfem.PostProcessing import CalcEMFunctional
+    _say("EM functional")
+    rng = np.random.default_rng(5)
+    V = FunctionSpace(UnitCube(2, 2, 2), "Nedelec 1st kind H(curl)", 1)
+    n = V.dim()
+    E = rng.random(n) + 1j * rng.random(n)
+    g = rng.random(n) + 1j * rng.random(n)
+    k0 = rng.random() * 10
+    dut = CalcEMFunctional(V)
+    dut.set_k0(k0)
+    dut.set_g_dofs(g)
+    dut.set_E_dofs(E)
+    val1 = dut.calc_functional()
+    dut.set_g_dofs(E)
+    dut.set_E_dofs(g)
+    val2 = dut.calc_functional()
+    assert abs(val1 - val2) <= 1e-13 * abs(val1), (val1, val2)
+    co, ce = O.unit_cube(2, 2, 2)
+    osp = O.Space(co, ce, 1)
+    want = O.em_functional(osp, E, g, k0, range(len(ce)))
+    assert abs(val1 - want) <= 1e-12 * abs(want), (val1, want)
+    # order 2, materials per cell, half the cells marked
+    V2 = FunctionSpace(UnitCube(2, 2, 2), "Nedelec 1st kind H(curl)", 2)
+    osp2 = O.Space(co, ce, 2)
+    nc = len(ce)
+    eps = rng.choice([1.0, 2.2, 4.4], nc)
+    mu = rng.choice([1.0, 1.5], nc)
+    marks = (np.arange(nc) % 2).astype(np.int64)
+    E2 = rng.standard_normal(osp2.ndofs) + 1j * rng.standard_normal(osp2.ndofs)
+    g2 = rng.standard_normal(osp2.ndofs) + 1j * rng.standard_normal(osp2.ndofs)
+    dut = CalcEMFunctional(V2)
+    dut.set_k0(k0)
+    dut.set_epsr_function(eps)
+    dut.set_mur_function(mu)
+    dut.set_cell_domains(marks, 1)
+    dut.set_E_dofs(E2)
+    dut.set_g_dofs(g2)
+    got = dut.calc_functional()
+    want = 0j
+    for c in np.flatnonzero(marks == 1):
+        T, S = O.element_matrices(2, co[ce[c]], eps_r=eps[c], inv_mu_r=1.0 / mu[c])
+        d = osp2.cell_dofs[c]
+        want += g2[d] @ ((S - k0 ** 2 * T) @ E2[d])
+    assert abs(got - want) <= 1e-12 * abs(want), (got, want)
