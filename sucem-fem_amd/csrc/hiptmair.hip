@@ -897,7 +897,10 @@ void grad_precond(Context* c, const Space* s, const void* winv, const void* dinv
   if constexpr (sizeof(R) == 4) xyz = s->xyz32.p; else xyz = s->mesh->xyz.p;
   const int nv = (int)s->mesh->nv, nE = (int)s->nE;
   const int reduce = rs.mode != FIN_NONE ? 1 : 0;
-  constexpr int LV = 16, LE = 4;   // lanes per vertex / edge row (measured: 32, 64 and 8 slower)
+  // lanes per vertex row / edge-node row; k_aux_t per FP32 application at 40^3 order 2:
+  // (LV, LE) = (16, 4) 37.8 us, (16, 2) 33.8, (16, 1) 33.9, (32, 2) 44.7, (8, 2) 31.5,
+  // (4, 2) 35.4, (2, 2) 47.0
+  constexpr int LV = 8, LE = 2;
   const int Gv = (int)(((int64_t)nv * LV + 8 * TB - 1) / (8 * TB) * 8);   // multiple of 8
 #define VT_ARGS Gv, nv, nE, inv_den, s->ve_ptr.p, s->ve_ent.p, s->vf_ptr.p, s->vf_ent.p, \
                 s->ef_ptr.p, s->ef_ent.p, s->edges.p, s->faces.p, xyz, (const C*)r, \
