@@ -191,6 +191,54 @@ __global__ void k_cocg_xr_def(int n, int G, const double* scal, C* x, C* r, cons
   }
 }
 
+// The same update with alpha = rho / (p.Ap) formed by every block from the operator's Gd
+// block partials (its p.Ap epilogue): all blocks sum them in one fixed order, so they agree
+// bit for bit, and the one-block finalising launch between the two kernels is saved.
+// Block 0 also stores alpha.
+template <typename C>
+__global__ __launch_bounds__(TB) void k_cocg_xr_def_a(int n, int G, int Gd, RedSite rs, C* x,
+                                                      C* r, const C* p, const C* q,
+                                                      const int32_t* __restrict__ qcode,
+                                                      const C* __restrict__ part,
+                                                      const uint8_t* __restrict__ mask) {
+  double2 s0 = make_double2(0, 0);
+  for (int k = threadIdx.x; k < Gd; k += TB) {
+    const double2 t = rs.part[k];
+    s0.x += t.x;
+    s0.y += t.y;
+  }
+  s0 = block_sum<TB>(s0);
+  const double2 al = cdiv0(scal2(rs.scal, S_RHO), s0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    rs.scal[S_ALPHA] = al.x;
+    rs.scal[S_ALPHA + 1] = al.y;
+  }
+  const C a = from_d<C>(al);
+  Range rr = block_rows(n, G);
+  for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
+    const int code = qcode[i];
+    C qi;
+    if (code < 0) {
+      qi = q[i];
+    } else {
+      qi = mk(decltype(qi.x)(0), decltype(qi.x)(0));
+      const int q0 = code >> 4, q1 = q0 + (code & 15);
+      for (int j = q0; j < q1; ++j) {
+        const C t = part[j];
+        qi.x += t.x;
+        qi.y += t.y;
+      }
+      if (mask && mask[i]) qi = mk(decltype(qi.x)(0), decltype(qi.x)(0));
+    }
+    const C ap = cmul(a, p[i]), aq = cmul(a, qi);
+    C xi = x[i], ri = r[i];
+    xi.x += ap.x; xi.y += ap.y;
+    ri.x -= aq.x; ri.y -= aq.y;
+    x[i] = xi;
+    r[i] = ri;
+  }
+}
+
 // p = z + beta p
 template <typename C>
 __global__ void k_cocg_pz(int n, int G, const double* scal, const C* z, C* p) {
@@ -524,19 +572,17 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     if (sample) timer.stop(c);
     finish_reduction(c, rs, op.flo.parts());
   };
-  // COCG with the patch operator: the shared rows' partial sums are added by the x/r update
+  // COCG with the patch operator: the shared rows' partial sums are added by the x/r update,
+  // which also forms alpha from the operator's Gd block partials
   const int Gd = (cocg && grad) ? op.deferred_parts() : 0;
-  auto apply_deferred = [&](const void* xv, void* yv, const RedSite& rs) {
-    const bool sample = it >= o.check_every && it % o.check_every == 0;
-    if (sample) timer.start(c);
-    op.apply_deferred(xv, yv, xv, rs);
-    if (sample) timer.stop(c);
-    finish_reduction(c, rs, Gd);
-  };
   for (it = 0; it < o.max_iter; ++it) {
     if (cocg && grad && Gd > 0) {
-      apply_deferred(p.p, q.p, site(SITE_SPMV1, FIN_COCG_ALPHA));
-      SUCEM_LAUNCH((k_cocg_xr_def<C>), G, TB, 0, c->stream, n, G, scal.p, xlo, r.p, p.p, q.p,
+      const RedSite ra = site(SITE_SPMV1, FIN_COCG_ALPHA);
+      const bool sample = it >= o.check_every && it % o.check_every == 0;
+      if (sample) timer.start(c);
+      op.apply_deferred(p.p, q.p, p.p, ra);
+      if (sample) timer.stop(c);
+      SUCEM_LAUNCH((k_cocg_xr_def_a<C>), G, TB, 0, c->stream, n, G, Gd, ra, xlo, r.p, p.p, q.p,
                    op.qcode(), (const C*)op.partials(), op.mask);
       precondition(site(SITE_UPDATE, FIN_COCG_BETA, it));
       SUCEM_LAUNCH((k_cocg_pz<C>), G, TB, 0, c->stream, n, G, scal.p, z.p, p.p);
