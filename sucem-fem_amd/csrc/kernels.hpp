@@ -71,7 +71,9 @@ enum ScalarSlot {
   S_RR = 8,       // |r|^2 of the recursion
   S_TRUE = 9,     // |b - A x|^2 (FP64)
   S_BNORM = 10,   // |b|^2
-  S_COUNT = 12
+  S_RHO_NEXT = 12,  // complex: rho for the next alpha (equal to S_RHO except between the
+                    // fused p update and the next x/r update of patch-operator COCG)
+  S_COUNT = 14
 };
 
 enum FinalizeMode {
@@ -84,6 +86,7 @@ enum FinalizeMode {
   FIN_BICG_OMEGA,   // s0 = conj(s).t, s1 = |t|^2 -> omega = conj(s0)/s1
   FIN_BICG_BETA,    // s0 = conj(rt).r, s1 = |r|^2 -> beta = (s0/rho)(alpha/omega), rho = s0
   FIN_SET_RHO,      // s0 = r.z, s1 = |r|^2     -> rho = s0 (start / restart)
+  FIN_DEFER,        // partials only: the consuming kernel forms the scalars itself
 };
 
 // One reduction site: per-block partial sums and the scalar array they are finalised
@@ -111,18 +114,21 @@ __device__ __forceinline__ void finalize(const RedSite& rs, double2 s0, double2 
       out = cdiv0(s0, rho);
       sc[S_BETA] = out.x; sc[S_BETA + 1] = out.y;
       sc[S_RHO] = s0.x; sc[S_RHO + 1] = s0.y;
+      sc[S_RHO_NEXT] = s0.x; sc[S_RHO_NEXT + 1] = s0.y;
       sc[S_RR] = s1.x;
       if (rs.hist) rs.hist[rs.it] = sqrt(s1.x);
       break;
     case FIN_RESID:
       sc[S_TRUE] = s0.x;
       sc[S_RHO] = s1.x; sc[S_RHO + 1] = s1.y;
+      sc[S_RHO_NEXT] = s1.x; sc[S_RHO_NEXT + 1] = s1.y;
       break;
     case FIN_NORM:
       sc[S_BNORM] = s0.x;
       break;
     case FIN_SET_RHO:
       sc[S_RHO] = s0.x; sc[S_RHO + 1] = s0.y;
+      sc[S_RHO_NEXT] = s0.x; sc[S_RHO_NEXT + 1] = s0.y;
       sc[S_RR] = s1.x;
       break;
     case FIN_BICG_OMEGA:
@@ -135,6 +141,7 @@ __device__ __forceinline__ void finalize(const RedSite& rs, double2 s0, double2 
       out = cmuld(cdiv0(s0, rho), cdiv0(al, om));
       sc[S_BETA] = out.x; sc[S_BETA + 1] = out.y;
       sc[S_RHO] = s0.x; sc[S_RHO + 1] = s0.y;
+      sc[S_RHO_NEXT] = s0.x; sc[S_RHO_NEXT + 1] = s0.y;
       sc[S_RR] = s1.x;
       if (rs.hist) rs.hist[rs.it] = sqrt(s1.x);
       break;

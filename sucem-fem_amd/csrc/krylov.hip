@@ -194,7 +194,8 @@ __global__ void k_cocg_xr_def(int n, int G, const double* scal, C* x, C* r, cons
 // The same update with alpha = rho / (p.Ap) formed by every block from the operator's Gd
 // block partials (its p.Ap epilogue): all blocks sum them in one fixed order, so they agree
 // bit for bit, and the one-block finalising launch between the two kernels is saved.
-// Block 0 also stores alpha.
+// rho is read from S_RHO_NEXT, which no block of this kernel writes; block 0 stores alpha
+// and copies that rho to S_RHO, which the next p update reads as the previous rho.
 template <typename C>
 __global__ __launch_bounds__(TB) void k_cocg_xr_def_a(int n, int G, int Gd, RedSite rs, C* x,
                                                       C* r, const C* p, const C* q,
@@ -208,10 +209,13 @@ __global__ __launch_bounds__(TB) void k_cocg_xr_def_a(int n, int G, int Gd, RedS
     s0.y += t.y;
   }
   s0 = block_sum<TB>(s0);
-  const double2 al = cdiv0(scal2(rs.scal, S_RHO), s0);
+  const double2 rho = scal2(rs.scal, S_RHO_NEXT);
+  const double2 al = cdiv0(rho, s0);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     rs.scal[S_ALPHA] = al.x;
     rs.scal[S_ALPHA + 1] = al.y;
+    rs.scal[S_RHO] = rho.x;
+    rs.scal[S_RHO + 1] = rho.y;
   }
   const C a = from_d<C>(al);
   Range rr = block_rows(n, G);
@@ -236,6 +240,43 @@ __global__ __launch_bounds__(TB) void k_cocg_xr_def_a(int n, int G, int Gd, RedS
     ri.x -= aq.x; ri.y -= aq.y;
     x[i] = xi;
     r[i] = ri;
+  }
+}
+
+// p = z + beta p with beta = (r.z) / rho formed by every block from the preconditioner's G
+// block partials (r.z at part[0, G), |r|^2 at part[G, 2G)) in one fixed order; rho is read
+// from S_RHO, which no block of this kernel writes.  Block 0 stores beta, the new rho (in
+// S_RHO_NEXT, for the next alpha), |r|^2 and the residual history entry.
+template <typename C>
+__global__ __launch_bounds__(TB) void k_cocg_pz_b(int n, int G, RedSite rs, const C* z, C* p) {
+  double2 s0 = make_double2(0, 0), s1 = make_double2(0, 0);
+  for (int k = threadIdx.x; k < G; k += TB) {
+    const double2 t = rs.part[k];
+    s0.x += t.x;
+    s0.y += t.y;
+  }
+  s0 = block_sum<TB>(s0);
+  const double2 be = cdiv0(s0, scal2(rs.scal, S_RHO));
+  if (blockIdx.x == 0) {
+    for (int k = threadIdx.x; k < G; k += TB) {
+      const double2 t = rs.part[G + k];
+      s1.x += t.x;
+      s1.y += t.y;
+    }
+    s1 = block_sum<TB>(s1);
+    if (threadIdx.x == 0) {
+      double* sc = rs.scal;
+      sc[S_BETA] = be.x; sc[S_BETA + 1] = be.y;
+      sc[S_RHO_NEXT] = s0.x; sc[S_RHO_NEXT + 1] = s0.y;
+      sc[S_RR] = s1.x;
+      if (rs.hist) rs.hist[rs.it] = sqrt(s1.x);
+    }
+  }
+  const C b = from_d<C>(be);
+  Range rr = block_rows(n, G);
+  for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
+    const C bp = cmul(b, p[i]), zi = z[i];
+    p[i] = mk(zi.x + bp.x, zi.y + bp.y);
   }
 }
 
@@ -584,8 +625,9 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
       if (sample) timer.stop(c);
       SUCEM_LAUNCH((k_cocg_xr_def_a<C>), G, TB, 0, c->stream, n, G, Gd, ra, xlo, r.p, p.p, q.p,
                    op.qcode(), (const C*)op.partials(), op.mask);
-      precondition(site(SITE_UPDATE, FIN_COCG_BETA, it));
-      SUCEM_LAUNCH((k_cocg_pz<C>), G, TB, 0, c->stream, n, G, scal.p, z.p, p.p);
+      const RedSite rb = site(SITE_UPDATE, FIN_DEFER, it);
+      precondition(rb);
+      SUCEM_LAUNCH((k_cocg_pz_b<C>), G, TB, 0, c->stream, n, G, rb, z.p, p.p);
       st.spmv_lo += 1;
     } else if (cocg && grad) {
       apply_epi(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
@@ -689,7 +731,7 @@ void check_opts(const SolveOpts& o) {
 }  // namespace
 
 void finish_reduction(Context* c, const RedSite& rs, int G) {
-  if (rs.mode == FIN_NONE) return;
+  if (rs.mode == FIN_NONE || rs.mode == FIN_DEFER) return;
   SUCEM_LAUNCH(k_finalize, 1, FIN_TB, 0, c->stream, rs, G);
 }
 
