@@ -243,14 +243,15 @@ __global__ __launch_bounds__(TB) void k_cocg_xr_def_a(int n, int G, int Gd, RedS
   }
 }
 
-// p = z + beta p with beta = (r.z) / rho formed by every block from the preconditioner's G
-// block partials (r.z at part[0, G), |r|^2 at part[G, 2G)) in one fixed order; rho is read
+// p = z + beta p with beta = (r.z) / rho formed by every block from the preconditioner's Gr
+// block partials (r.z at part[0, Gr), |r|^2 at part[Gr, 2Gr)) in one fixed order; rho is read
 // from S_RHO, which no block of this kernel writes.  Block 0 stores beta, the new rho (in
 // S_RHO_NEXT, for the next alpha), |r|^2 and the residual history entry.
 template <typename C>
-__global__ __launch_bounds__(TB) void k_cocg_pz_b(int n, int G, RedSite rs, const C* z, C* p) {
+__global__ __launch_bounds__(TB) void k_cocg_pz_b(int n, int G, int Gr, RedSite rs, const C* z,
+                                                  C* p) {
   double2 s0 = make_double2(0, 0), s1 = make_double2(0, 0);
-  for (int k = threadIdx.x; k < G; k += TB) {
+  for (int k = threadIdx.x; k < Gr; k += TB) {
     const double2 t = rs.part[k];
     s0.x += t.x;
     s0.y += t.y;
@@ -258,8 +259,8 @@ __global__ __launch_bounds__(TB) void k_cocg_pz_b(int n, int G, RedSite rs, cons
   s0 = block_sum<TB>(s0);
   const double2 be = cdiv0(s0, scal2(rs.scal, S_RHO));
   if (blockIdx.x == 0) {
-    for (int k = threadIdx.x; k < G; k += TB) {
-      const double2 t = rs.part[G + k];
+    for (int k = threadIdx.x; k < Gr; k += TB) {
+      const double2 t = rs.part[Gr + k];
       s1.x += t.x;
       s1.y += t.y;
     }
@@ -616,6 +617,10 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   // COCG with the patch operator: the shared rows' partial sums are added by the x/r update,
   // which also forms alpha from the operator's Gd block partials
   const int Gd = (cocg && grad) ? op.deferred_parts() : 0;
+  // grid of the preconditioner's z kernel in that loop (4 blocks per CU: 23.4 against
+  // 24.7 us per FP32 application with the 8 of the other vector kernels, which lose more
+  // at 4: the x/r update 34.3 against 28.5 us)
+  const int Gz = op_grid(c, n, 4);
   for (it = 0; it < o.max_iter; ++it) {
     if (cocg && grad && Gd > 0) {
       const RedSite ra = site(SITE_SPMV1, FIN_COCG_ALPHA);
@@ -626,8 +631,8 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
       SUCEM_LAUNCH((k_cocg_xr_def_a<C>), G, TB, 0, c->stream, n, G, Gd, ra, xlo, r.p, p.p, q.p,
                    op.qcode(), (const C*)op.partials(), op.mask);
       const RedSite rb = site(SITE_UPDATE, FIN_DEFER, it);
-      precondition(rb);
-      SUCEM_LAUNCH((k_cocg_pz_b<C>), G, TB, 0, c->stream, n, G, rb, z.p, p.p);
+      op.precond_lo(winv.p, dinv.p, r.p, w.p, z.p, Gz, rb, aux ? vinv.p : nullptr, u.p);
+      SUCEM_LAUNCH((k_cocg_pz_b<C>), G, TB, 0, c->stream, n, G, Gz, rb, z.p, p.p);
       st.spmv_lo += 1;
     } else if (cocg && grad) {
       apply_epi(p.p, q.p, p.p, 1, site(SITE_SPMV1, FIN_COCG_ALPHA));
