@@ -12,7 +12,7 @@ mesh data, matrices and b resident in HBM.  The headline path is FP32+IR
 accumulation of x and FP64 reliable updates of the residual); plain FP64 is timed once
 beside it for the comparison.
 
-Prints one JSON line.  ``--gpus`` above 1 is refused: this build runs on one GPU.
+Prints one JSON line.  ``--gpus`` above 1 is refused: the solver runs on one GPU.
 """
 import argparse
 import json
@@ -104,7 +104,7 @@ def main():
                          "known-size kernel that scales HBM byte counters")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d"
+        print("bench.py runs on one GPU only (the solver has no multi-GPU path); got --gpus %d"
               % a.gpus, file=sys.stderr)
         sys.exit(2)
 
