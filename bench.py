@@ -9,8 +9,8 @@ auxiliary-space preconditioner (Jacobi + gradient + vector nodal terms), operato
 autotuned (matrix-free per patch of cells or per cell, or assembled CSR),
 mesh data, matrices and b resident in HBM.  The headline path is FP32+IR
 (--precision 32: FP32 operator, preconditioner and vectors, FP64 dot products, FP64
-accumulation of x and FP64 reliable updates of the residual); plain FP64 is timed once
-beside it for the comparison.
+accumulation of x and FP64 reliable updates of the residual); plain FP64 is timed beside
+it over the same warmup and timed steps for the comparison.
 
 Prints one JSON line.  ``--gpus`` above 1 is refused: the solver runs on one GPU.
 """
@@ -28,6 +28,24 @@ for p in (ROOT, os.path.join(ROOT, "sucem-fem_amd")):
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_facts():
+    """CPU model and the core counts of the machine the baseline runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": usable, "cpu_count": os.cpu_count()}
 
 
 def cpu_baseline(prob, fp64_iterations, budget_s=20.0):
@@ -58,12 +76,14 @@ def cpu_baseline(prob, fp64_iterations, budget_s=20.0):
     _, its, _ = O.cocg_auxiliary(A, b, G, Pi, tol=0.0, maxiter=k, gdiag=gd, vdiag=vd)
     per_it = (time.perf_counter() - t0) / its
     solve_s = setup_s + per_it * fp64_iterations
-    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "port",
+    out = {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": "preconditioner set-up (Pi, diag(G^T A G), diag(Pi^T A Pi): %.2f s) plus "
                       "%d FP64 auxiliary-space preconditioned COCG iterations of the same "
                       "%d-DOF system on the host (scipy CSR, %.3f s/iteration), iterations "
                       "scaled to the %d the FP64 device solve needs"
                       % (setup_s, its, n, per_it, fp64_iterations)}
+    out.update(cpu_facts())
+    return out
 
 
 def read_traffic(matrix_free, precision, cb):
@@ -176,12 +196,45 @@ def main():
     spmv_bytes = prob.operator_bytes(a.precision, mf, cb=cb)
     # inside COCG the patch form leaves the shared rows' partial sums to the x/r update
     op_bytes = prob.operator_bytes(a.precision, mf, epilogue=True, cb=cb, deferred=(mf == 2))
+    copy_gbps = None
     if a.copy_probe_mb > 0:
+        # one device copy of known size (scales the HBM byte counters of a profiled run),
+        # then the same copy repeated and event-timed: the measured HBM peak (read + write)
         src = torch.ones(a.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
+        # the timed copy is 8x the 256 MB last-level cache, so it measures HBM
         del src, dst
+        src = torch.ones(2048 << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        reps = 10
+        c0_, c1_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0_.record(stream)
+        for _ in range(reps):
+            dst.copy_(src)
+        c1_.record(stream)
+        torch.cuda.synchronize()
+        copy_gbps = 2.0 * src.numel() * 4 * reps / (c0_.elapsed_time(c1_) * 1e-3) / 1e9
+        del src, dst
+        log("device copy 2048 MiB: %.0f GB/s (read + write)" % copy_gbps)
+
+    # assembly: S and T re-assembled on the resident space (second call, so the gather map
+    # and reference tensors exist), event-timed around the whole call
+    from sucemfem.SystemMatrices import AssembledSystem
+    from sucemfem import _hip
+    AssembledSystem(prob.space, _hip.MAT_S | _hip.MAT_T)
+    torch.cuda.synchronize()
+    a0_, a1_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a0_.record(stream)
+    again = AssembledSystem(prob.space, _hip.MAT_S | _hip.MAT_T)
+    a1_.record(stream)
+    torch.cuda.synchronize()
+    asm_ms = a0_.elapsed_time(a1_)
+    del again
+    asm_bytes = prob.assembly_bytes()
+    log("assembly of S and T: %.3f ms, %.1f GB/s of gather-kernel bytes" % (
+        asm_ms, asm_bytes / (asm_ms * 1e-3) / 1e9))
     isolated = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     achieved = op_bytes / (op_ms * 1e-3) / 1e9 if op_ms else 0.0
     form = ("assembled CSR", "matrix-free per cell", "matrix-free per patch of %d cells" % cb)[mf]
@@ -194,21 +247,33 @@ def main():
     if a.precision == 64:
         fp64_iters = stats[-1]["iterations"]
     if not a.no_compare:
-        t1 = time.perf_counter()
-        sc = step(other)
+        # the comparison path is timed exactly like the headline: the same warmup and
+        # number of timed solves, bracketed by device synchronisations
+        for i in range(a.warmup):
+            step(other)
         torch.cuda.synchronize()
-        other_s = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        other_stats = [step(other) for _ in range(a.steps)]
+        torch.cuda.synchronize()
+        other_s = (time.perf_counter() - t1) / a.steps
+        sc = other_stats[-1]
+        other_ok = all(q["status"] == 0 and q["rel_residual"] <= a.tol for q in other_stats)
         name = "fp64" if other == 64 else "fp32ir"
-        extra["%s_solves_per_s" % name] = 1.0 / other_s if sc["status"] == 0 else 0.0
-        extra["%s_iterations" % name] = sc["iterations"]
-        extra["%s_rel_residual" % name] = sc["rel_residual"]
+        extra["%s_solves_per_s" % name] = 1.0 / other_s if other_ok else 0.0
+        extra["%s_iterations" % name] = [q["iterations"] for q in other_stats]
+        extra["%s_rel_residual" % name] = [q["rel_residual"] for q in other_stats]
+        extra["%s_latency_ms_p50" % name] = float(np.percentile(
+            [q["elapsed_ms"] for q in other_stats], 50))
+        extra["%s_latency_ms_p99" % name] = float(np.percentile(
+            [q["elapsed_ms"] for q in other_stats], 99))
         mine = a.steps / elapsed
         extra["fp32ir_vs_fp64_speedup"] = ((1.0 / other_s) / mine if other == 32
                                            else mine / (1.0 / other_s))
         if other == 64:
             fp64_iters = sc["iterations"]
-        log("%s solve %.2fs iters %d rel %.3e" % (name, other_s, sc["iterations"],
-                                                  sc["rel_residual"]))
+        log("%s solves %.3fs each, iters %s rel %.3e" % (name, other_s,
+                                                       extra["%s_iterations" % name],
+                                                       sc["rel_residual"]))
 
     traffic = read_traffic(mf, a.precision, cb) if (a.n, a.order) == (40, 2) else None
 
@@ -261,6 +326,13 @@ def main():
                      "isolated": {"avg_ms": spmv_ms, "bytes_per_launch": spmv_bytes,
                                   "achieved": isolated}},
         "cpu_baseline": cpu,
+        "hbm_copy_measured_gbps": copy_gbps,
+        "assembly": {"ms": asm_ms, "bytes": asm_bytes,
+                     "gbps": asm_bytes / (asm_ms * 1e-3) / 1e9,
+                     "what": "sucem_assemble of S and T on the resident space (cell "
+                             "coefficients, allocation of the value arrays and the "
+                             "atomic-free gather), event-timed; bytes = the gather kernel's "
+                             "algorithmic bytes"},
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

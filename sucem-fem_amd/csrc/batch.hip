@@ -182,10 +182,15 @@ __global__ __launch_bounds__(TB) void k_bresid(int n, int G, const C* b, const C
   reduce_finalize_nb<NB>(rs, G, s, make_double2(0, 0));
 }
 
+// Systems with act[j] == 0 (converged or broken down) are left as they are, so the
+// returned x_j is the one whose residual was checked.  Element e = row*NB + j with TB a
+// multiple of NB, so a thread always handles the same system.
 template <int NB>
-__global__ void k_bxr(int n, int G, const double* scal, C* x, C* r, const C* p, const C* q) {
+__global__ void k_bxr(int n, int G, const double* scal, const int* act, C* x, C* r, const C* p,
+                      const C* q) {
   const Rows rr = rows_of(n, G);
   const int j = threadIdx.x % NB;
+  if (!act[j]) return;
   const C a = scal2(scal + j * S_COUNT, S_ALPHA);
   for (int64_t e = (int64_t)rr.start * NB + threadIdx.x; e < (int64_t)rr.end * NB; e += TB) {
     const C ap = cmul(a, p[e]), aq = cmul(a, q[e]);
@@ -195,9 +200,10 @@ __global__ void k_bxr(int n, int G, const double* scal, C* x, C* r, const C* p, 
 }
 
 template <int NB>
-__global__ void k_bpz(int n, int G, const double* scal, const C* z, C* p) {
+__global__ void k_bpz(int n, int G, const double* scal, const int* act, const C* z, C* p) {
   const Rows rr = rows_of(n, G);
   const int j = threadIdx.x % NB;
+  if (!act[j]) return;
   const C b = scal2(scal + j * S_COUNT, S_BETA);
   for (int64_t e = (int64_t)rr.start * NB + threadIdx.x; e < (int64_t)rr.end * NB; e += TB) {
     const C bp = cmul(b, p[e]);
@@ -422,31 +428,49 @@ void solve_chunk(Context* c, System* sys, int nb, const double* k0, const double
   SUCEM_LAUNCH((k_bcopy<NB>), G, TB, 0, c->stream, n, G, b.p, r.p);
   precondition(site(1, FIN_SET_RHO));
   SUCEM_LAUNCH((k_bcopy<NB>), G, TB, 0, c->stream, n, G, z.p, p.p);
-  std::vector<int> done(NB, 0);
+  std::vector<int> done(NB, 0), broken(NB, 0);
   for (int j = 0; j < NB; ++j)
     if (j >= nb || bn[j] == 0.0) done[j] = 1;
+  // device copy of the systems still iterating (the update kernels skip the others)
+  DBuf<int> act(NB);
+  auto upload_act = [&] {
+    std::vector<int> a(NB);
+    for (int j = 0; j < NB; ++j) a[j] = done[j] ? 0 : 1;
+    h2d(c, act.p, a.data(), NB);
+    c->sync();
+  };
+  upload_act();
   std::vector<double> rel(NB, 0.0);
   std::vector<int> iters(NB, 0);
   int it = 0;
   bool all_done = std::all_of(done.begin(), done.end(), [](int d) { return d != 0; });
   for (it = 0; it < o.max_iter && !all_done; ++it) {
     spmm(p.p, q.p, p.p, 1, site(2, FIN_COCG_ALPHA));
-    SUCEM_LAUNCH((k_bxr<NB>), G, TB, 0, c->stream, n, G, scal.p, x.p, r.p, p.p, q.p);
+    SUCEM_LAUNCH((k_bxr<NB>), G, TB, 0, c->stream, n, G, scal.p, act.p, x.p, r.p, p.p, q.p);
     precondition(site(3, FIN_COCG_BETA, it));
-    SUCEM_LAUNCH((k_bpz<NB>), G, TB, 0, c->stream, n, G, scal.p, z.p, p.p);
+    SUCEM_LAUNCH((k_bpz<NB>), G, TB, 0, c->stream, n, G, scal.p, act.p, z.p, p.p);
     if ((it + 1) % o.check_every != 0 && it + 1 != o.max_iter) continue;
     std::vector<double> h = read(hist.p + (int64_t)it * NB, NB);
-    bool cand = true;
+    bool cand = true, froze = false;
     for (int j = 0; j < NB; ++j) {
       if (done[j]) continue;
       rel[j] = h[j] / bn[j];
+      if (!std::isfinite(rel[j])) {
+        // breakdown of this system's recursion: stop it, keep the others going
+        done[j] = broken[j] = 1;
+        iters[j] = it + 1;
+        froze = true;
+        continue;
+      }
       if (rel[j] <= o.tol) {
         if (!iters[j]) iters[j] = it + 1;
       } else {
         cand = false;
       }
     }
-    if (!cand) continue;
+    if (froze) upload_act();
+    all_done = std::all_of(done.begin(), done.end(), [](int d) { return d != 0; });
+    if (!cand || all_done) continue;
     // every recursion is below tol: confirm with the true residuals, restart the others
     spmm(x.p, ax.p, nullptr, 0, RedSite{});
     SUCEM_LAUNCH((k_bresid<NB>), G, TB, 0, c->stream, n, G, b.p, ax.p, mask, r.p,
@@ -455,11 +479,12 @@ void solve_chunk(Context* c, System* sys, int nb, const double* k0, const double
     std::vector<double> tr = scal_slot(S_BNORM);
     all_done = true;
     for (int j = 0; j < NB; ++j) {
-      if (j >= nb || bn[j] == 0.0) continue;
+      if (j >= nb || bn[j] == 0.0 || broken[j]) continue;
       rel[j] = std::sqrt(tr[j]) / bn[j];
       if (rel[j] <= o.tol) done[j] = 1;
-      else { all_done = false; iters[j] = 0; }
+      else { all_done = false; iters[j] = 0; done[j] = 0; }
     }
+    upload_act();
     if (!all_done) {
       precondition(site(1, FIN_SET_RHO));
       SUCEM_LAUNCH((k_bcopy<NB>), G, TB, 0, c->stream, n, G, z.p, p.p);
@@ -471,7 +496,7 @@ void solve_chunk(Context* c, System* sys, int nb, const double* k0, const double
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   for (int j = 0; j < nb; ++j) {
     st[j] = SolveStats();
-    st[j].status = done[j] ? 0 : it;
+    st[j].status = broken[j] ? -1 : (done[j] ? 0 : it);
     st[j].iterations = iters[j] ? iters[j] : it;
     st[j].rel_residual = rel[j];
     st[j].elapsed_ms = ms;

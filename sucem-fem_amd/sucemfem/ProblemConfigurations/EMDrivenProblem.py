@@ -49,6 +49,21 @@ class DrivenProblemABC(EMProblem):
         """The device-resident assembled system (S, M and S_0 share it)."""
         return self.system_matrices['S'].system
 
+    def pec_diagonal(self, k0):
+        """Diagonal of the PEC rows of A, or None when no essential condition was applied.
+
+        DOLFIN's ``DirichletBC.apply`` (reached through ``calc_system_matrices``,
+        SystemMatrices.py:46-58) zeroes each constrained row of S, M and S_0 separately and
+        puts 1 on its diagonal, so a PEC row of S - k0^2 M + j k0 S_0 is
+        (1 - k0^2 + j k0) e_i (terms of matrices left unconstrained drop out)."""
+        mats = self.system_matrices
+        on = dict((k, mats.get(k) is not None and mats[k].pec_constrained)
+                  for k in ('S', 'M', 'S_0'))
+        if not any(on.values()):
+            return None
+        return complex(1.0 if on['S'] else 0.0, 0.0) - (k0 ** 2 if on['M'] else 0.0) \
+            + (1j * k0 if on['S_0'] else 0.0)
+
     def get_LHS_matrix(self):
         k0 = self.get_k0()
         system = self.get_system()
@@ -56,11 +71,18 @@ class DrivenProblemABC(EMProblem):
         vals = system.lhs_values(k0)
         if self.system_matrices.get('S_0') is None:
             vals = vals.real.astype(N.complex128)
+        d = self.pec_diagonal(k0)
+        if d is not None:
+            mask = self.function_space.pec_mask()
+            rows = N.repeat(N.arange(len(indptr) - 1), N.diff(indptr))
+            kill = mask[rows]
+            vals[kill] = 0
+            vals[kill & (indices == rows)] = d
         A = DrivenLHS((vals, indices.astype(N.int32), indptr.astype(N.int32)),
                       shape=(self.get_global_dimension(),) * 2)
         # SciPy's sparse sum in the reference drops exact zeros
         A.eliminate_zeros()
-        A.device_operator = (system, k0, self.system_matrices.get('S_0') is not None)
+        A.device_operator = (system, k0, self.system_matrices.get('S_0') is not None, d)
         return A
 
     def get_RHS_device(self, nrhs=1):

@@ -124,13 +124,31 @@ class SystemSolverBase(object):
         hist = np.zeros(maxiter, np.float64)
         op = getattr(A, "device_operator", None)
         if op is not None:
-            system, k0, _ = op
+            system, k0 = op[0], op[1]
+            pec_d = op[3] if len(op) > 3 else None
             ctx = system.ctx
             b_d = ctx.to_device(b)
             x_d = ctx.tensor(n)
+            xp_d = None
+            if pec_d is not None:
+                # PEC rows are pec_d * x_i = b_i (DirichletBC rows); the other rows keep
+                # their couplings to the PEC columns, so b_i / pec_d on the PEC DOFs is moved
+                # to the right-hand side and the solve runs on the free DOFs only
+                mask = system.function_space.pec_mask()
+                opts.eliminate_pec = 1
+                if np.any(b[mask] != 0):
+                    x_pec = np.zeros(n, np.complex128)
+                    x_pec[mask] = b[mask] / pec_d
+                    xp_d = ctx.to_device(x_pec)
+                    y_d = ctx.tensor(n)
+                    _hip.call("sucem_system_apply", ctx.handle, system.handle.h, float(k0), 64,
+                              0, 1, _hip.ptr(xp_d), _hip.ptr(y_d))
+                    b_d = b_d - y_d
             _hip.call("sucem_solve_driven", ctx.handle, system.handle.h, float(k0), _hip.ptr(b_d),
                       _hip.ptr(x_d), ctypes.byref(opts), ctypes.byref(stats), _hip.ptr(hist),
                       maxiter)
+            if xp_d is not None:
+                x_d = x_d + xp_d
         else:
             dev = DeviceCSR(A)
             ctx = dev.ctx
@@ -204,13 +222,30 @@ class UMFPACKSolver(SystemSolverBase):
     """Stand-in for the direct UMFPACK solve (LinalgSolvers.py:262-277).
 
     There is no sparse direct factorisation on the device; the system is solved
-    by BiCGStab to a relative residual of 1e-12 instead.
+    iteratively to a relative residual of 1e-12 instead: COCG with the auxiliary-space
+    preconditioner for a driven operator (complex symmetric), Jacobi BiCGStab for any
+    other matrix.  A direct solve always returns the solution, so a solve that misses
+    the tolerance is reported with a ``RuntimeWarning`` (the iterate is still returned,
+    and ``self.stats`` holds the final relative residual).
     """
     method = _hip.METHOD_BICGSTAB
     default_tol = 1e-12
 
-    def __init__(self, A, preconditioner_type='diagonal', **kw):
-        SystemSolverBase.__init__(self, A, preconditioner_type, **kw)
+    def __init__(self, A, preconditioner_type=None, **kw):
+        if getattr(A, "device_operator", None) is not None:
+            self.method = _hip.METHOD_COCG
+            preconditioner_type = preconditioner_type or 'auxiliary'
+        SystemSolverBase.__init__(self, A, preconditioner_type or 'diagonal', **kw)
+
+    def solve(self, b):
+        x = SystemSolverBase.solve(self, b)
+        if self.stats is not None and self.stats['status'] != 0:
+            warnings.warn("UMFPACKSolver stand-in: the iterative solve stopped at relative "
+                          "residual %.3e (status %d) above the tolerance %.1e; the iterate is "
+                          "not the direct solution"
+                          % (self.stats['rel_residual'], self.stats['status'], self.tol),
+                          RuntimeWarning)
+        return x
 
 
 def calculate_residual(A, x, b):

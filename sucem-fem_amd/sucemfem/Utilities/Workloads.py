@@ -229,6 +229,15 @@ class DeviceProblem(object):
         products = 2 * (nc * nloc + nbf * nfd) * sx   # products written, then read
         return cells + faces + products + (n + 1) * 4 + 2 * n * sx   # segments; x once, y once
 
+    def assembly_bytes(self):
+        """Algorithmic HBM bytes of the S + T assembly gather (``k_gather_volume``): the
+        segment pointers (nnz + 1 int32), every contribution id of the (row, col)-sorted
+        gather map once (nc * nloc^2 uint32), the 12 FP64 geometric coefficients of each cell
+        once, and the FP64 and FP32 (S, T) pairs written per nonzero (16 + 8 bytes)."""
+        nc = int(np.prod(self.n)) * 6
+        nloc = 6 if self.order == 1 else 20
+        return 4 * (self.nnz + 1) + 4 * nc * nloc * nloc + 96 * nc + 24 * self.nnz
+
     def spmv_bytes(self, precision):
         """Algorithmic HBM bytes of one assembled fused SpMV (one vector)."""
         sv = 8 if precision == 32 else 16          # (S, T) pair

@@ -149,3 +149,43 @@ def test_femmesh_sphere_driven(tmp_path, golden):
     osp = O.Space(d["nodes"], d["tets"], 1)
     xo = O.solve_direct(O.driven_lhs(O.assemble_fast(osp), freq), b)
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-7
+
+
+def test_gmsh_rectangular_prism_cavity(tmp_path):
+    """rectangular_prism.msh read by Utilities.GmshIO (the gmsh conversion of
+    sandbox/pec_labels/pec_labels.py:27-443): its facet_region marks exactly the boundary
+    facets, PEC on region 1 gives the oracle's order-2 cavity modes on the same mesh, and
+    writing the mesh and facet function to DOLFIN XML and reading them back (the XML path)
+    gives the same PEC DOFs and eigenvalues."""
+    import os
+    from sucemfem.Mesh import Mesh, MeshFunction, boundary_facets
+    from sucemfem.Utilities.GmshIO import gmsh_to_dolfin_mesh
+    from sucemfem.BoundaryConditions import PECWallsBoundaryCondition
+    print("gmsh rectangular prism cavity", flush=True)
+    path = os.path.join(os.path.dirname(__file__), "golden", "rectangular_prism.msh")
+    mesh, cell_mf, facet_mf = gmsh_to_dolfin_mesh(path)
+    assert cell_mf is None and facet_mf is not None
+    marked = np.flatnonzero(facet_mf.array() == 1)
+    assert np.array_equal(marked, np.sort(boundary_facets(mesh)))
+    bc = PECWallsBoundaryCondition()
+    bc.init_with_meshfunction(facet_mf, 1)
+    ep, k2, _ = _eigs(mesh, 2, bc, 1.1, 5)
+    co, ce = mesh.coordinates(), mesh.cells()
+    osp = O.Space(co, ce, 2)
+    flags = np.zeros(len(facet_mf.array()), bool)
+    flags[marked] = True
+    assert np.array_equal(ep.function_space.pec_mask(), osp.pec_mask(flags))
+    ref, _, _ = O.cavity_eigs(osp, 5, 1.1, facet_flags=flags)
+    assert np.all(np.abs(k2 - ref) / ref <= 1e-8), (k2, ref)
+    # discretisation check against the analytic modes of the 1 x 0.5 x 0.25 box
+    ana = O.analytic_cavity_k2([1.0, 0.5, 0.25], 5)
+    assert np.all(np.abs(k2 - ana) / ana < 0.01), (k2, ana)
+    write_dolfin_xml(str(tmp_path / "p.xml"), co, ce)
+    write_dolfin_meshfunction(str(tmp_path / "p_facet_region.xml"), 2, facet_mf.array())
+    mesh2 = Mesh(str(tmp_path / "p.xml"))
+    mf2 = MeshFunction("uint", mesh2, str(tmp_path / "p_facet_region.xml"))
+    bc2 = PECWallsBoundaryCondition()
+    bc2.init_with_meshfunction(mf2, 1)
+    ep2, k2b, _ = _eigs(mesh2, 2, bc2, 1.1, 5)
+    assert np.array_equal(ep2.function_space.pec_mask(), ep.function_space.pec_mask())
+    assert np.all(np.abs(k2b - k2) / k2 <= 1e-12), (k2b, k2)

@@ -86,3 +86,66 @@ def test_femmesh_accessors_need_read(tmp_path):
     r = MeshIO.FemmeshReader(str(tmp_path / "none.femmesh"))
     with pytest.raises(MeshIO.UninitialisedException):
         r.get_nodes()
+
+
+def _boundary_triangles(cells):
+    loc = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
+    tri = np.sort(np.concatenate([np.sort(cells, axis=1)[:, list(l)] for l in loc]), axis=1)
+    u, cnt = np.unique(tri, axis=0, return_counts=True)
+    return u[cnt == 1]
+
+
+def test_gmsh_reader_rectangular_prism(golden):
+    """functional_tests/data/meshes/rectangular_prism.msh (gmsh 2.2, the 1 x 0.5 x 0.25 prism
+    of rectangular_prism.geo with the walls in physical group 1), read by the conversion rules
+    of sandbox/pec_labels/pec_labels.py:27-443: tetrahedra make the mesh, the tagged triangles
+    are exactly its boundary facets."""
+    import os
+    from sucemfem.Utilities.GmshIO import GmshMesh
+    g = GmshMesh(os.path.join(os.path.dirname(__file__), "golden", "rectangular_prism.msh"))
+    xyz, cells = g.coordinates_and_cells()
+    assert xyz.shape == (107, 3) and cells.shape == (329, 4)
+    # vertices numbered by ascending gmsh node id
+    used = np.unique(g.elements[4][1])
+    ref = dict(zip(g.node_ids.tolist(), g.node_xyz))
+    assert np.array_equal(xyz, np.array([ref[i] for i in used]))
+    assert np.all(np.diff(cells, axis=1) > 0)
+    X = xyz[cells]
+    J = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]], axis=2)
+    vol = np.abs(np.linalg.det(J)).sum() / 6
+    assert abs(vol - 0.125) < 1e-12
+    assert np.allclose(xyz.min(axis=0), 0) and np.allclose(xyz.max(axis=0), [1, 0.5, 0.25])
+    assert g.cell_regions() is None                     # every tetrahedron in region 0
+    tri, tag = g.tagged_triangles()
+    assert len(tri) == 156 and set(tag.tolist()) == {1}
+    bnd = _boundary_triangles(cells)
+    assert np.array_equal(np.unique(np.sort(tri, axis=1), axis=0), bnd)
+
+
+def test_gmsh_reader_errors(tmp_path):
+    import pytest
+    from sucemfem.Utilities.GmshIO import GmshMesh, GmshFormatError
+    p = tmp_path / "bad.msh"
+    p.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n2\n1 0 0 0\n$EndNodes\n")
+    with pytest.raises(GmshFormatError):
+        GmshMesh(str(p))
+    p.write_text("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n$Nodes\n0\n$EndNodes\n$Elements\n0\n"
+                 "$EndElements\n")
+    with pytest.raises(GmshFormatError):
+        GmshMesh(str(p))
+    p.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n4\n1 0 0 0\n2 1 0 0\n3 0 1 0\n"
+                 "4 0 0 1\n$EndNodes\n$Elements\n1\n1 4 2 0 1 1 2 3 9\n$EndElements\n")
+    with pytest.raises(GmshFormatError):
+        GmshMesh(str(p)).coordinates_and_cells()
+
+
+def test_femmesh_malformed_block(tmp_path):
+    import pytest
+    from sucemfem.Utilities import MeshIO
+    p = tmp_path / "bad.femmesh"
+    p.write_text("BLOCK nodes\n2\n1 0 0 0\n3 1 0 0\nENDBLOCK\n")
+    with pytest.raises(AssertionError):
+        MeshIO.FemmeshReader(str(p)).read_meshfile()
+    p.write_text("BLOCK nodes\n2\n1 0 0 0\nENDBLOCK\n")
+    with pytest.raises(AssertionError):
+        MeshIO.FemmeshReader(str(p)).read_meshfile()
