@@ -71,7 +71,8 @@ typedef struct sucem_solve_opts {
 typedef struct sucem_eig_opts {
   int ncv;              /* Lanczos basis size (0: max(2 nev + 1, 20)) */
   int max_restarts;
-  int inner_precision;  /* 64 (32 is rejected: FP32 inner solves of S - sigma T stall) */
+  int inner_precision;  /* 64: FP64 inner solves; 32: FP32 refinement cycles (FP32 COCG
+                           recursions restarted from the FP64 residual, x in FP64) */
   int inner_max_iter;
   double tol;           /* relative Ritz residual of nu = 1/(k^2 - sigma) */
   double inner_tol;     /* relative residual of the inner (S - sigma T) solves */
@@ -83,18 +84,20 @@ typedef struct sucem_eig_stats {
   int nconv;
   int restarts;
   int lanczos_steps;
-  int inner_failures;
-  int pad_;
+  int inner_failures;   /* inner solves that hit inner_max_iter or broke down */
+  int inner_stagnated;  /* inner solves stopped at the floor of their true residual */
   int64_t inner_iterations;
   double max_ritz_residual;
   double elapsed_ms;
 } sucem_eig_stats;
 
 typedef struct sucem_solve_stats {
-  int status;           /* 0 converged, >0 iteration limit, <0 breakdown (scipy 'info') */
+  int status;           /* 0 converged, >0 iteration limit or stagnation, <0 breakdown
+                           (scipy 'info') */
   int iterations;
   int reliable_updates;
-  int pad_;
+  int stagnated;        /* 1: stopped because three residual replacements in a row did not
+                           halve the true residual (its FP64 floor lies above tol) */
   double rel_residual;  /* final FP64 ||b - A x|| / ||b|| */
   double elapsed_ms;
   int64_t spmv_lo;      /* SpMVs in the working precision */
@@ -103,15 +106,29 @@ typedef struct sucem_solve_stats {
                            (its kernels, with the fused dot-product epilogue), measured with
                            HIP events on the solve's stream at one iteration per check interval */
   int64_t op_samples;   /* number of such samples (0: none taken) */
+  int fp64_finish;      /* precision 32: 1 when the FP32 cycles stopped lowering the true
+                           residual (or diverged) and the solve was finished in FP64 from
+                           their accepted solution */
+  int pad_;
 } sucem_solve_stats;
 
 const char* sucem_last_error(void);
 int sucem_version(void);
 
-/* Context: device and the stream all work is issued on (NULL = the null stream). */
+/* Context: device and the stream all work is issued on (NULL = the null stream).  A
+ * context keeps the solvers' work vectors resident (a workspace that grows to the largest
+ * solve run on it), so repeated solves allocate no device memory; calls on one context
+ * must come from one host thread at a time. */
 int sucem_ctx_create(int device, void* hip_stream, sucem_ctx** out);
 int sucem_ctx_destroy(sucem_ctx* ctx);
 int sucem_ctx_synchronize(sucem_ctx* ctx);
+/* Device bytes held by the context's workspace; release frees it (the next solve
+ * allocates again).  (No reference counterpart: memory accounting for serving.) */
+int sucem_ctx_workspace_bytes(sucem_ctx* ctx, int64_t* bytes_h);
+int sucem_ctx_release_workspace(sucem_ctx* ctx);
+/* Free and total device memory of the context's device (hipMemGetInfo).  (No reference
+ * counterpart: admission of requests by device memory.) */
+int sucem_ctx_mem_info(sucem_ctx* ctx, int64_t* free_h, int64_t* total_h);
 
 /* ---- mesh (replaces the DOLFIN Mesh handed to EMProblem.set_mesh, EMProblem.py:49-54) ---- */
 /* Cells are sorted per row on the device (DOLFIN mesh ordering). */
@@ -222,16 +239,27 @@ int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const doubl
                        double* res_hist_h, int64_t hist_cap);
 
 /* nb driven systems on one assembled system, system j at wavenumber k0_h[j]:
- * (S - k0_j^2 T + i k0_j S0) x_j = b_j.  B_d, X_d: nb x N complex128 (system-major).  With
- * FP64 COCG and SUCEM_PRECOND_GRADIENT the assembled matrix is read once per iteration for
- * a batch of up to 16 systems (vectors interleaved internally); any other options (the
- * default, opts = NULL: SUCEM_PRECOND_AUXILIARY) solve the systems one after another with
- * the system's operator.  stats_h (nullable): nb entries.
+ * (S - k0_j^2 T + i k0_j S0) x_j = b_j.  B_d, X_d: nb x N complex128 (system-major).  COCG
+ * with SUCEM_PRECOND_AUXILIARY (the default, opts = NULL) runs the batched sweep below (8
+ * systems at a time, hipGraph); FP64 COCG with SUCEM_PRECOND_GRADIENT reads the assembled
+ * matrix once per iteration for up to 16 interleaved systems; other options solve the
+ * systems one after another.  stats_h (nullable): nb entries.
  * (Frequency sweeps of the reference re-form and re-solve the LHS per frequency,
  * sandbox/patch/patch_driver.py:64-129.) */
 int sucem_solve_driven_batch(sucem_ctx* ctx, sucem_system* sys, int64_t nb, const double* k0_h,
                              const double* B_d, double* X_d, const sucem_solve_opts* opts,
                              sucem_solve_stats* stats_h);
+/* The batched sweep: the nb systems are solved `width` (1, 2, 4 or 8) at a time, each
+ * group as that many COCG recursions (auxiliary-space preconditioner, precision 64 or 32
+ * with FP64 residual replacement per system) over vectors interleaved per DOF, whose
+ * operator reads each patch's maps and cell coefficients once for the whole group; with
+ * use_graph the check interval's iterations are captured once in a hipGraph and replayed.
+ * Results do not depend on width or use_graph beyond rounding of the FP32 recursions.
+ * (Replaces the per-frequency / per-source loops of sandbox/patch/patch_driver.py:106-129
+ * and Sources/point_source.py:49-94.) */
+int sucem_solve_driven_sweep(sucem_ctx* ctx, sucem_system* sys, int64_t nb, const double* k0_h,
+                             const double* B_d, double* X_d, const sucem_solve_opts* opts,
+                             int width, int use_graph, sucem_solve_stats* stats_h);
 /* Solve (cs S + ct T + i c0 S0) x = b (e.g. the shifted S - sigma T of the eigen solver). */
 int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
                        const double* b_d, double* x_d, const sucem_solve_opts* opts,

@@ -18,7 +18,7 @@ OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libsucem_hip.so")
 
 SOURCES = ["refdata.cpp", "capi.cpp", "mesh_space.hip", "assemble.hip", "ops.hip", "krylov.hip", "eigen.hip",
-           "hiptmair.hip", "batch.hip", "postproc.hip", "matfree.hip", "patch.hip"]
+           "hiptmair.hip", "batch.hip", "postproc.hip", "matfree.hip", "patch.hip", "sweep.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SUCEM_OFFLOAD_ARCH", "gfx950")

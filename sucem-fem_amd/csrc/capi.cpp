@@ -102,6 +102,10 @@ int sucem_ctx_destroy(sucem_ctx* ctx) {
   Context& c = ctx->c;
   (void)hipSetDevice(c.device);
   (void)hipStreamSynchronize(c.stream);
+  if (c.aux_stream) {
+    (void)hipStreamSynchronize(c.aux_stream);
+    (void)hipStreamDestroy(c.aux_stream);
+  }
   if (c.pinned) (void)hipHostFree(c.pinned);
   if (c.own_stream) (void)hipStreamDestroy(c.stream);
   delete ctx;
@@ -113,6 +117,33 @@ int sucem_ctx_synchronize(sucem_ctx* ctx) {
   NOT_NULL(ctx);
   ctx->c.use();
   ctx->c.sync();
+  API_END
+}
+
+int sucem_ctx_workspace_bytes(sucem_ctx* ctx, int64_t* bytes_h) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(bytes_h);
+  *bytes_h = (int64_t)ctx->c.ws.bytes();
+  API_END
+}
+
+int sucem_ctx_release_workspace(sucem_ctx* ctx) {
+  API_BEGIN
+  NOT_NULL(ctx);
+  ctx->c.use();
+  ctx->c.sync();
+  ctx->c.ws.release();
+  API_END
+}
+
+int sucem_ctx_mem_info(sucem_ctx* ctx, int64_t* free_h, int64_t* total_h) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(free_h); NOT_NULL(total_h);
+  ctx->c.use();
+  size_t fr = 0, tot = 0;
+  SUCEM_HIP(hipMemGetInfo(&fr, &tot));
+  *free_h = (int64_t)fr;
+  *total_h = (int64_t)tot;
   API_END
 }
 
@@ -637,13 +668,15 @@ static void convert_stats(const SolveStats& s, sucem_solve_stats* out) {
   out->status = s.status;
   out->iterations = s.iterations;
   out->reliable_updates = s.reliable_updates;
-  out->pad_ = 0;
+  out->stagnated = s.stagnated;
   out->rel_residual = s.rel_residual;
   out->elapsed_ms = s.elapsed_ms;
   out->spmv_lo = s.spmv_lo;
   out->spmv_hi = s.spmv_hi;
   out->op_ms = s.op_ms;
   out->op_samples = s.op_samples;
+  out->fp64_finish = s.fp32_failed;
+  out->pad_ = 0;
 }
 
 int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const double* b_d,
@@ -671,7 +704,30 @@ int sucem_solve_driven_batch(sucem_ctx* ctx, sucem_system* sys, int64_t nb, cons
   std::vector<SolveStats> st(nb);
   SolveOpts o = convert(opts);
   if (!opts) o.precond = SUCEM_PRECOND_AUXILIARY;
-  solve_system_batch(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, st.data());
+  if (o.method == SUCEM_METHOD_COCG && o.precond == SUCEM_PRECOND_AUXILIARY &&
+      sys_supports_sweep(c, &sys->s))
+    solve_system_sweep(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, 8, true,
+                       st.data());
+  else
+    solve_system_batch(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, st.data());
+  if (stats_h)
+    for (int64_t j = 0; j < nb; ++j) convert_stats(st[j], stats_h + j);
+  API_END
+}
+
+int sucem_solve_driven_sweep(sucem_ctx* ctx, sucem_system* sys, int64_t nb, const double* k0_h,
+                             const double* B_d, double* X_d, const sucem_solve_opts* opts,
+                             int width, int use_graph, sucem_solve_stats* stats_h) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys); NOT_NULL(k0_h); NOT_NULL(B_d); NOT_NULL(X_d);
+  SUCEM_CHECK(nb > 0, SUCEM_ERR_INVALID, "nb must be positive");
+  Context* c = &ctx->c;
+  c->use();
+  SolveOpts o = convert(opts);
+  if (!opts) o.precond = SUCEM_PRECOND_AUXILIARY;
+  std::vector<SolveStats> st(nb);
+  solve_system_sweep(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, width,
+                     use_graph != 0, st.data());
   if (stats_h)
     for (int64_t j = 0; j < nb; ++j) convert_stats(st[j], stats_h + j);
   API_END
@@ -759,9 +815,6 @@ int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,
   }
   SUCEM_CHECK(o.inner_precision == 32 || o.inner_precision == 64, SUCEM_ERR_INVALID,
               "inner precision must be 32 or 64");
-  SUCEM_CHECK(o.inner_precision == 64, SUCEM_ERR_UNSUPPORTED,
-              "FP32 inner solves of the shifted operator S - sigma T do not converge to the "
-              "accuracy the Lanczos recursion needs; use inner_precision 64");
   SUCEM_CHECK(o.tol > 0 && o.inner_tol > 0 && o.max_restarts >= 0 && o.inner_max_iter > 0,
               SUCEM_ERR_INVALID, "bad eigen options");
   EigStats st;
@@ -772,7 +825,7 @@ int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,
     stats->restarts = st.restarts;
     stats->lanczos_steps = st.lanczos_steps;
     stats->inner_failures = st.inner_failures;
-    stats->pad_ = 0;
+    stats->inner_stagnated = st.inner_stagnated;
     stats->inner_iterations = st.inner_iterations;
     stats->max_ritz_residual = st.max_ritz_residual;
     stats->elapsed_ms = st.elapsed_ms;

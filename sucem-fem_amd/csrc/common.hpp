@@ -64,6 +64,33 @@ struct DBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Device workspace of a context: numbered slots that keep their allocation from call to
+// call and grow when a call needs more, so repeated solves on one context allocate (and
+// free, which synchronises the device) nothing after the first.  A slot is used by one
+// routine at a time: the Krylov solver takes slots [0, 32), the eigen solver [32, 48) (it
+// calls the Krylov solver, so their ranges differ), the batched solver [48, 80).  Calls on
+// one context run one at a time (one host thread per context).
+struct Workspace {
+  std::vector<DBuf<unsigned char>> slot;
+  // device memory of at least `bytes` bytes in slot k (nullptr for 0 bytes); the contents
+  // are whatever the slot's last user left
+  void* get(int k, size_t bytes) {
+    if (bytes == 0) return nullptr;
+    if ((size_t)k >= slot.size()) slot.resize(k + 1);
+    if (slot[k].n < bytes) {
+      slot[k].release();
+      slot[k].alloc(bytes + (bytes >> 3));   // 1/8 headroom for a slightly larger next call
+    }
+    return slot[k].p;
+  }
+  size_t bytes() const {
+    size_t t = 0;
+    for (const auto& b : slot) t += b.n;
+    return t;
+  }
+  void release() { slot.clear(); }
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -72,10 +99,24 @@ struct Context {
   // pinned host staging for scalar readbacks
   double* pinned = nullptr;
   size_t pinned_n = 0;
+  Workspace ws;
+  // library-owned stream (non-blocking, default priority) for work captured in hipGraphs:
+  // the caller's stream may be the null stream, which cannot be captured
+  hipStream_t aux_stream = nullptr;
+  hipStream_t graph_stream() {
+    if (!aux_stream) SUCEM_HIP(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
+    return aux_stream;
+  }
   void sync() { SUCEM_HIP(hipStreamSynchronize(stream)); }
   void use() { SUCEM_HIP(hipSetDevice(device)); }
   double* host_scratch(size_t n);
 };
+
+// typed view of workspace slot k holding `count` elements
+template <typename T>
+inline T* ws_take(Context* c, int k, int64_t count) {
+  return (T*)c->ws.get(k, count > 0 ? sizeof(T) * (size_t)count : 0);
+}
 
 template <typename T>
 inline void h2d(Context* c, T* d, const T* h, size_t n) {

@@ -7,12 +7,17 @@
 // facets removed).  The wanted modes are the largest positive Ritz values
 // nu = 1/(lambda - sigma) > 0, i.e. the lowest k^2 above sigma; the gradient null
 // space of the curl-curl operator maps to nu = -1/sigma < 0 and is never wanted.
-// Inner solves use the device COCG (FP64, or FP32 with FP64 reliable updates);
-// orthogonalisation, the projected matrix and its eigen-decomposition are FP64,
-// and each returned k^2 is the FP64 Rayleigh quotient x^T S x / x^T T x.
+// Inner solves use the device COCG with the auxiliary-space preconditioner: FP64, or FP32
+// refinement cycles (FP32 recursions, each restarted from the residual recomputed with the
+// FP64 operator, x accumulated in FP64, a cycle cut short when its recursion stagnates)
+// that reach the same inner tolerance; orthogonalisation, the projected matrix and its
+// eigen-decomposition are FP64, and each returned k^2 is the FP64 Rayleigh quotient
+// x^T S x / x^T T x of its Ritz vector, with the FP64 residual ||S x - k^2 T x||.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 #include <vector>
 
@@ -48,17 +53,31 @@ __global__ void k_start_vector(int64_t n, unsigned long long seed, const uint8_t
   }
 }
 
-// part[i*G + lb] = sum_r Re(conj(a_i[r]) y[r]) for i < m  (vectors a_i = base + i*n)
+// part[i*G + lb] = sum_r Re(conj(a_i[r]) y[r]) for i < m  (vectors a_i = base + i*n).
+// The basis vectors are taken MB at a time, so y is read once per MB of them.
+constexpr int MB = 8;
 __global__ __launch_bounds__(TB) void k_multidot(int n, int G, int m, const double2* base,
                                                  const double2* y, double2* part) {
   Range rr = rows_of(n, G);
-  for (int i = 0; i < m; ++i) {
-    const double2* a = base + (int64_t)i * n;
-    double2 s = make_double2(0, 0);
-    for (int r = rr.start + threadIdx.x; r < rr.end; r += TB)
-      s.x += a[r].x * y[r].x + a[r].y * y[r].y;
-    s = block_sum<TB>(s);
-    if (threadIdx.x == 0) part[(int64_t)i * G + xcd_block(G)] = s;
+  for (int i0 = 0; i0 < m; i0 += MB) {
+    const int cnt = min(MB, m - i0);
+    double acc[MB];
+#pragma unroll
+    for (int k = 0; k < MB; ++k) acc[k] = 0.0;
+    for (int r = rr.start + threadIdx.x; r < rr.end; r += TB) {
+      const double2 yr = y[r];
+#pragma unroll
+      for (int k = 0; k < MB; ++k) {
+        if (k < cnt) {
+          const double2 a = base[(int64_t)(i0 + k) * n + r];
+          acc[k] += a.x * yr.x + a.y * yr.y;
+        }
+      }
+    }
+    for (int k = 0; k < cnt; ++k) {
+      const double2 t = block_sum<TB>(make_double2(acc[k], 0.0));
+      if (threadIdx.x == 0) part[(int64_t)(i0 + k) * G + xcd_block(G)] = t;
+    }
   }
 }
 
@@ -164,6 +183,12 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
   SUCEM_CHECK(m > nev, SUCEM_ERR_INVALID, "ncv must exceed nev");
   auto t0 = std::chrono::steady_clock::now();
   st = EigStats();
+  // progress lines on stderr: SUCEM_LOG=1 per restart cycle, 2 also per Lanczos step
+  const char* lg = std::getenv("SUCEM_LOG");
+  const int log_level = lg ? std::atoi(lg) : 0;
+  auto elapsed_s = [&] {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
   const int G = [&] {
     int64_t g = std::min<int64_t>((int64_t)c->num_cus * 4, (n + 255) / 256);
     g = (g + 7) / 8 * 8;
@@ -171,11 +196,17 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
   }();
   const uint8_t* mask = s->pec.p;
 
-  DBuf<double2> V((int64_t)(m + 1) * n), TV((int64_t)(m + 1) * n);
-  DBuf<double2> W((int64_t)(m + 1) * n);   // restart scratch
-  DBuf<double2> y(n), ty(n);
-  DBuf<double2> part((int64_t)(m + 1) * G);
-  DBuf<double> hdev(m + 1);
+  // basis, T times basis, restart scratch and work vectors in the context's workspace
+  // (slots 32..40): repeated eigen solves on one context allocate nothing after the first
+  struct Buf { double2* p; };
+  const Buf V{ws_take<double2>(c, 32, (int64_t)(m + 1) * n)};
+  const Buf TV{ws_take<double2>(c, 33, (int64_t)(m + 1) * n)};
+  const Buf W{ws_take<double2>(c, 34, (int64_t)(m + 1) * n)};
+  const Buf y{ws_take<double2>(c, 35, n)}, ty{ws_take<double2>(c, 36, n)};
+  const Buf part{ws_take<double2>(c, 37, (int64_t)(m + 1) * G)};
+  struct { double* p; } hdev{ws_take<double>(c, 38, m + 1)};
+  // small combination matrices of the restarts
+  double* Sd = ws_take<double>(c, 39, (int64_t)(m + 1) * (m + 1));
   std::vector<double> H((m + 1) * (m + 1), 0.0);
   auto h = [&](int i, int j) -> double& { return H[i * (m + 1) + j]; };
 
@@ -186,7 +217,7 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
   Coef Ak;  // S - sigma T
   Ak.s = 1; Ak.t = -sigma; Ak.s0 = 0;
 
-  auto vec = [&](DBuf<double2>& B, int i) { return B.p + (int64_t)i * n; };
+  auto vec = [&](const Buf& B, int i) { return B.p + (int64_t)i * n; };
   auto multidot = [&](const double2* base, int cnt, const double2* v, std::vector<double>& out) {
     SUCEM_LAUNCH(k_multidot, G, TB, 0, c->stream, n, G, cnt, base, v, part.p);
     std::vector<double2> hp((int64_t)cnt * G);
@@ -232,7 +263,12 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
       solve_system(c, sys, Ak, vec(TV, j), y.p, so, ist, nullptr, 0);
       st.inner_iterations += ist.iterations;
       st.lanczos_steps += 1;
-      if (ist.status != 0) st.inner_failures += 1;
+      if (ist.stagnated) st.inner_stagnated += 1;
+      else if (ist.status != 0) st.inner_failures += 1;
+      if (log_level >= 2)
+        std::fprintf(stderr, "[sucem eigs] step %d: inner %d iterations, rel %.2e%s, %.1f s\n",
+                     st.lanczos_steps, ist.iterations, ist.rel_residual,
+                     ist.stagnated ? " (floor)" : "", elapsed_s());
       // full re-orthogonalisation (classical Gram-Schmidt, twice) in the T inner product
       for (int pass = 0; pass < 2; ++pass) {
         std::vector<double> hh;
@@ -279,6 +315,10 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
     }
     st.max_ritz_residual = worst;
     st.restarts = cycle;
+    if (log_level >= 1)
+      std::fprintf(stderr, "[sucem eigs] cycle %d: %d Lanczos steps, %lld inner iterations, "
+                   "worst Ritz residual %.2e, %.1f s\n", cycle, st.lanczos_steps,
+                   (long long)st.inner_iterations, worst, elapsed_s());
     if (conv || mm < m || cycle == o.max_restarts) {
       done = true;
       st.status = conv ? 0 : 1;
@@ -286,9 +326,8 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
       std::vector<double> Sw(mm * nev);
       for (int i = 0; i < mm; ++i)
         for (int q = 0; q < nev; ++q) Sw[i * nev + q] = Sv[i * mm + wanted[q]];
-      DBuf<double> Sd(Sw.size());
-      h2d(c, Sd.p, Sw.data(), Sw.size());
-      SUCEM_LAUNCH(k_combine, G, TB, 0, c->stream, n, G, mm, nev, Sd.p, nev, V.p, W.p);
+      h2d(c, Sd, Sw.data(), Sw.size());
+      SUCEM_LAUNCH(k_combine, G, TB, 0, c->stream, n, G, mm, nev, Sd, nev, V.p, W.p);
       c->sync();
       break;
     }
@@ -299,12 +338,11 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
     std::vector<double> Sk_(mm * kk);
     for (int i = 0; i < mm; ++i)
       for (int q = 0; q < kk; ++q) Sk_[i * kk + q] = Sv[i * mm + keep[q]];
-    DBuf<double> Sd(Sk_.size());
-    h2d(c, Sd.p, Sk_.data(), Sk_.size());
-    SUCEM_LAUNCH(k_combine, G, TB, 0, c->stream, n, G, mm, kk, Sd.p, kk, V.p, W.p);
+    h2d(c, Sd, Sk_.data(), Sk_.size());
+    SUCEM_LAUNCH(k_combine, G, TB, 0, c->stream, n, G, mm, kk, Sd, kk, V.p, W.p);
     SUCEM_HIP(hipMemcpyAsync(vec(V, 0), W.p, sizeof(double2) * (int64_t)kk * n,
                              hipMemcpyDeviceToDevice, c->stream));
-    SUCEM_LAUNCH(k_combine, G, TB, 0, c->stream, n, G, mm, kk, Sd.p, kk, TV.p, W.p);
+    SUCEM_LAUNCH(k_combine, G, TB, 0, c->stream, n, G, mm, kk, Sd, kk, TV.p, W.p);
     SUCEM_HIP(hipMemcpyAsync(vec(TV, 0), W.p, sizeof(double2) * (int64_t)kk * n,
                              hipMemcpyDeviceToDevice, c->stream));
     SUCEM_HIP(hipMemcpyAsync(vec(V, kk), vec(V, mm), sizeof(double2) * n, hipMemcpyDeviceToDevice,
@@ -324,6 +362,7 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
     for (int q = 0; q < kk; ++q) h(q, kk) = 0.0;
   }
   // FP64 Rayleigh quotients and true residuals of the Ritz vectors
+  std::vector<double> tscale(nev, 1.0);
   for (int q = 0; q < nev; ++q) {
     double2* x = vec(W, q);
     apply(Sk, x, y.p);
@@ -341,10 +380,7 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
     multidot(ty.p, 1, ty.p, tt);
     lam_h[q] = lam;
     if (resid_h) resid_h[q] = std::sqrt(rr[0]) / (std::fabs(lam) * std::sqrt(tt[0]));
-    // T-normalise into the output
-    if (vecs_d)
-      SUCEM_LAUNCH(k_scale_copy, G, TB, 0, c->stream, n, G, 1.0 / std::sqrt(b[0]), x,
-                   vecs_d + (int64_t)q * n);
+    tscale[q] = 1.0 / std::sqrt(b[0]);
   }
   // ascending k^2
   std::vector<int> ord(nev);
@@ -359,18 +395,11 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
     lam_h[q] = l2[q];
     if (resid_h) resid_h[q] = r2[q];
   }
-  if (vecs_d) {
-    bool sorted = true;
-    for (int q = 0; q < nev; ++q) sorted &= ord[q] == q;
-    if (!sorted) {
-      DBuf<double2> tmp((int64_t)nev * n);
-      for (int q = 0; q < nev; ++q)
-        SUCEM_HIP(hipMemcpyAsync(tmp.p + (int64_t)q * n, vecs_d + (int64_t)ord[q] * n,
-                                 sizeof(double2) * n, hipMemcpyDeviceToDevice, c->stream));
-      SUCEM_HIP(hipMemcpyAsync(vecs_d, tmp.p, sizeof(double2) * (int64_t)nev * n,
-                               hipMemcpyDeviceToDevice, c->stream));
-    }
-  }
+  // T-normalised modes in ascending k^2 (the Ritz vectors are still in W)
+  if (vecs_d)
+    for (int q = 0; q < nev; ++q)
+      SUCEM_LAUNCH(k_scale_copy, G, TB, 0, c->stream, n, G, tscale[ord[q]], vec(W, ord[q]),
+                   vecs_d + (int64_t)q * n);
   c->sync();
   st.nconv = st.status == 0 ? nev : 0;
   st.elapsed_ms =

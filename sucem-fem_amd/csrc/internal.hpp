@@ -209,6 +209,7 @@ struct SolveOpts {
   double tol = 1e-8;
   double ru_delta = 0.01;
   int eliminate_pec = 0;
+  int warm = 0;          // x holds the initial guess (otherwise x0 = 0)
 };
 struct SolveStats {
   int status = 0, iterations = 0, reliable_updates = 0;
@@ -216,6 +217,8 @@ struct SolveStats {
   int64_t spmv_lo = 0, spmv_hi = 0;
   double op_ms = 0;         // mean device time of a sampled working-precision operator
   int64_t op_samples = 0;
+  int stagnated = 0;        // stopped at the floor of the true residual (status > 0)
+  int fp32_failed = 0;      // FP32 cycles stopped making progress; finished in FP64
 };
 void solve_system(Context* c, const System* sys, Coef k, const double2* b, double2* x,
                   const SolveOpts& o, SolveStats& st, double* hist_h, int64_t hist_cap);
@@ -233,6 +236,15 @@ const double* gradient_diag(Context* c, System* sys, bool masked);
 void solve_system_batch(Context* c, System* sys, int64_t nb, const double* k0,
                         const double2* B, double2* X, const SolveOpts& o, SolveStats* st);
 
+// sweep.hip: nb driven systems (one k0 each) solved together, `width` at a time (1, 2,
+// 4 or 8 interleaved systems) by COCG with the auxiliary-space preconditioner on the
+// patch operator, the iteration replayed from a hipGraph when use_graph
+void solve_system_sweep(Context* c, System* sys, int64_t nb, const double* k0, const double2* B,
+                        double2* X, const SolveOpts& o, int width, bool use_graph,
+                        SolveStats* st);
+// whether the patch operator (which the sweep needs) can be built for this system
+bool sys_supports_sweep(Context* c, System* sys);
+
 // eigen.hip
 struct EigOpts {
   int ncv = 0;               // Lanczos basis size (0: max(2 nev + 1, 20))
@@ -245,6 +257,7 @@ struct EigOpts {
 };
 struct EigStats {
   int status = 0, nconv = 0, restarts = 0, lanczos_steps = 0, inner_failures = 0;
+  int inner_stagnated = 0;
   int64_t inner_iterations = 0;
   double max_ritz_residual = 0, elapsed_ms = 0;
 };

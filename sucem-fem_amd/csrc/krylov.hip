@@ -20,6 +20,8 @@
 // `check_every` iterations.
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "internal.hpp"
 #include "kernels.hpp"
@@ -29,6 +31,12 @@ namespace sucem {
 namespace {
 
 constexpr int TB = 256;
+
+// a workspace slot viewed as an array of T
+template <typename T>
+struct Slot {
+  T* p;
+};
 
 struct Range { int start, end; };
 __device__ __forceinline__ Range block_rows(int n, int G) {
@@ -92,15 +100,16 @@ __global__ __launch_bounds__(TB) void k_residual(int n, int G, const double2* b,
   reduce_finalize<TB>(rs, G, tr, rho);
 }
 
-// x64 += xlo; xlo = 0  (FP32 reliable update)
+// x64 += sign * xlo (FP32 reliable update: the cycle's correction is added, and taken
+// out again when the replacement shows it did not help); with clear, xlo = 0 afterwards
 template <typename C>
-__global__ void k_accum(int n, int G, C* xlo, double2* x64) {
+__global__ void k_accum(int n, int G, C* xlo, double2* x64, double sign, int clear) {
   Range rr = block_rows(n, G);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
     const double2 a = to_d(xlo[i]);
-    x64[i].x += a.x;
-    x64[i].y += a.y;
-    xlo[i] = from_d<C>(make_double2(0, 0));
+    x64[i].x += sign * a.x;
+    x64[i].y += sign * a.y;
+    if (clear) xlo[i] = from_d<C>(make_double2(0, 0));
   }
 }
 
@@ -539,11 +548,15 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   const int64_t nn = aux ? op.nnod() : 0;
   auto t0 = std::chrono::steady_clock::now();
   st = SolveStats();
+  const char* lg = std::getenv("SUCEM_LOG");   // 3: one line per residual replacement
+  const int log_level = lg ? std::atoi(lg) : 0;
 
-  DBuf<double2> part((int64_t)SITE_COUNT * 2 * Gm);
-  DBuf<unsigned> counters(SITE_COUNT);
-  DBuf<double> scal(S_COUNT);
-  DBuf<double> hist(o.max_iter + 1);
+  // work vectors live in the context's workspace (slots 0..20): no device allocation (or
+  // free) inside a solve once the context has run one of this size
+  const Slot<double2> part{ws_take<double2>(c, 0, (int64_t)SITE_COUNT * 2 * Gm)};
+  const Slot<unsigned> counters{ws_take<unsigned>(c, 1, SITE_COUNT)};
+  const Slot<double> scal{ws_take<double>(c, 2, S_COUNT)};
+  const Slot<double> hist{ws_take<double>(c, 3, o.max_iter + 1)};
   SUCEM_LAUNCH(k_zero_u32, 1, 64, 0, c->stream, (int)SITE_COUNT, counters.p);
   SUCEM_LAUNCH(k_zero_f64, 1, 64, 0, c->stream, (int)S_COUNT, scal.p);
   auto site = [&](int s_, int mode, int it = 0) {
@@ -557,13 +570,16 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     return r;
   };
 
-  DBuf<C> dinv(n), r(n), p(n), q(n);
-  DBuf<C> xlo_buf(lo ? n : 0);
-  DBuf<double2> ax(n);
-  DBuf<C> rt(cocg ? 0 : n), v(cocg ? 0 : n), phat(cocg ? 0 : n), s(cocg ? 0 : n),
-      shat(cocg ? 0 : n), t(cocg ? 0 : n);
+  auto vec = [&](int k, int64_t cnt) { return Slot<C>{ws_take<C>(c, k, cnt)}; };
+  const Slot<C> dinv = vec(4, n), r = vec(5, n), p = vec(6, n), q = vec(7, n);
+  const Slot<C> xlo_buf = vec(8, lo ? n : 0);
+  const Slot<double2> ax{ws_take<double2>(c, 9, n)};
+  const Slot<C> rt = vec(10, cocg ? 0 : n), v = vec(11, cocg ? 0 : n),
+                phat = vec(12, cocg ? 0 : n), s = vec(13, cocg ? 0 : n),
+                shat = vec(14, cocg ? 0 : n), t = vec(15, cocg ? 0 : n);
   C* xlo = lo ? xlo_buf.p : (C*)x;
-  DBuf<C> z(grad ? n : 0), w(nl), winv(nl), u(nn), vinv(nn);
+  const Slot<C> z = vec(16, grad ? n : 0), w = vec(17, nl), winv = vec(18, nl),
+                u = vec(19, nn), vinv = vec(20, nn);
   // z = M^-1 r, reducing r.z / |r|^2 into the site's scalars
   auto precondition = [&](const RedSite& rs) {
     op.precond_lo(winv.p, dinv.p, r.p, w.p, z.p, G, rs, aux ? vinv.p : nullptr, u.p);
@@ -573,7 +589,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   op.aux = aux;
   if (grad) op.winv_lo(winv.p);
   if (aux) op.vinv_lo(vinv.p);
-  SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, x);
+  if (!o.warm) SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, x);
   if (lo) SUCEM_LAUNCH((k_zero_c<C>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, xlo);
   SUCEM_LAUNCH(k_norm2, G, TB, 0, c->stream, n, G, b, op.mask, site(SITE_NORM, FIN_NORM));
   finish_reduction(c, site(SITE_NORM, FIN_NORM), G);
@@ -584,11 +600,24 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     c->sync();
     return;
   }
-  // r = b - A x0 with x0 = 0
-  SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, ax.p);
+  // r = b - A x0 (x0 = 0, or the caller's x with o.warm)
+  if (o.warm) {
+    op.apply_hi(x, ax.p);
+    st.spmv_hi += 1;
+  } else {
+    SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, ax.p);
+  }
   SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p,
                cocg ? 1 : 0, site(SITE_RESID, FIN_RESID));
   finish_reduction(c, site(SITE_RESID, FIN_RESID), G);
+  const double start_rel =
+      o.warm ? std::sqrt(read_scalar(c, scal.p + S_TRUE)) / bnorm : 1.0;
+  if (start_rel <= o.tol) {
+    st.status = 0;
+    st.rel_residual = start_rel;
+    c->sync();
+    return;
+  }
   if (grad) {
     precondition(site(SITE_UPDATE, FIN_SET_RHO));
     SUCEM_LAUNCH((k_copy<C>), G, TB, 0, c->stream, n, G, z.p, p.p);
@@ -597,9 +626,14 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   } else {
     SUCEM_LAUNCH((k_copy<C>), G, TB, 0, c->stream, n, G, r.p, rt.p);
   }
-  double last_true = 1.0;
+  double last_true = start_rel;
+  // stagnation: the best true residual so far and the replacements since it last halved
+  double best_true = start_rel;
+  int flat = 0;
   int it = 0;
   int restart = 1;
+  // FP32 refinement cycles: a cycle runs from one residual replacement to the next
+  int cycle_start = 0, longest_cycle = 0;
   st.status = o.max_iter;
   double rel = 1.0;
   // Operator timing: at the first iteration of every check interval (from the second
@@ -667,10 +701,32 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
       st.status = -1;   // breakdown
       break;
     }
-    const bool ru = rel <= o.tol || (lo && rel <= o.ru_delta * last_true);
+    // An FP32 cycle whose recursion has not dropped by ru_delta within 3x the longest
+    // completed cycle (at least 64 iterations) is cut short: its correction is folded in
+    // by a replacement anyway and a fresh recursion starts from the true residual.  (Near
+    // the FP64 floor of b - A x, or on the indefinite shifted cavity operator, an FP32
+    // recursion can stagnate above its target for the rest of the iteration budget.)
+    const int cyc = it + 1 - cycle_start;
+    // the first cycle has no completed cycle to compare with: a fixed cap
+    const int cap = longest_cycle > 0 ? std::max(64, 3 * longest_cycle)
+                                      : std::max(1000, o.max_iter / 8);
+    const bool cut = lo && cyc >= cap;
+    // an FP32 recursion whose residual grew a thousandfold over the cycle's start has
+    // diverged (COCG on an indefinite operator with an indefinite preconditioner has no
+    // monotone residual; in FP32 it can run away): its correction is dropped and the
+    // solve is finished in FP64 from the last accepted x
+    if (lo && rel > 1e3 * last_true) {
+      SUCEM_LAUNCH((k_zero_c<C>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, xlo);
+      st.fp32_failed = 1;
+      ++it;
+      break;
+    }
+    const bool ru = rel <= o.tol || (lo && rel <= o.ru_delta * last_true) || cut;
     if (!ru) continue;
+    if (!cut) longest_cycle = std::max(longest_cycle, cyc);
+    cycle_start = it + 1;
     // reliable update / residual replacement with the FP64 operator
-    if (lo) SUCEM_LAUNCH((k_accum<C>), G, TB, 0, c->stream, n, G, xlo, x);
+    if (lo) SUCEM_LAUNCH((k_accum<C>), G, TB, 0, c->stream, n, G, xlo, x, 1.0, 0);
     op.apply_hi(x, ax.p);
     st.spmv_hi += 1;
     st.reliable_updates += 1;
@@ -678,10 +734,39 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
                  cocg ? 1 : 0, site(SITE_RESID, FIN_RESID));
     finish_reduction(c, site(SITE_RESID, FIN_RESID), G);
     const double true_rel = std::sqrt(read_scalar(c, scal.p + S_TRUE)) / bnorm;
+    if (log_level >= 3)
+      std::fprintf(stderr, "[sucem krylov] it %d: recursion %.3e, true %.3e%s (cycle %d)\n",
+                   it + 1, rel, true_rel, cut ? " cut" : "", cyc);
+    if (lo && !(true_rel < last_true)) {
+      // an FP32 cycle that did not lower the true residual: take its correction out again
+      // and finish in FP64 from the last accepted x
+      SUCEM_LAUNCH((k_accum<C>), G, TB, 0, c->stream, n, G, xlo, x, -1.0, 1);
+      st.fp32_failed = 1;
+      ++it;
+      break;
+    }
+    if (lo) SUCEM_LAUNCH((k_zero_c<C>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, xlo);
     rel = true_rel;
     last_true = true_rel;
     if (true_rel <= o.tol) {
       st.status = 0;
+      ++it;
+      break;
+    }
+    // The true residual has a floor set by the FP64 arithmetic of b - A x (it rises with
+    // the operator's condition number).  A recursion that keeps reaching its target while
+    // three replacements in a row fail to halve the best true residual is at that floor:
+    // stop (status > 0, the iterations done) instead of running to max_iter.
+    if (true_rel <= 0.5 * best_true) {
+      best_true = true_rel;
+      flat = 0;
+    } else if (++flat >= 3) {
+      if (lo) {
+        st.fp32_failed = 1;   // FP32 cycles stopped paying: FP64 takes over
+      } else {
+        st.status = it + 1;
+        st.stagnated = 1;
+      }
       ++it;
       break;
     }
@@ -697,9 +782,18 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
       restart = 1;
     }
   }
+  if (st.fp32_failed) {
+    // the caller finishes the solve in FP64 from x (the accepted corrections)
+    st.iterations = it;
+    c->sync();
+    timer.harvest();
+    st.op_samples = timer.samples;
+    st.op_ms = timer.samples ? timer.total_ms / (double)timer.samples : 0.0;
+    return;
+  }
   if (st.status != 0) {
     // final true residual for the report
-    if (lo) SUCEM_LAUNCH((k_accum<C>), G, TB, 0, c->stream, n, G, xlo, x);
+    if (lo) SUCEM_LAUNCH((k_accum<C>), G, TB, 0, c->stream, n, G, xlo, x, 1.0, 1);
     op.apply_hi(x, ax.p);
     st.spmv_hi += 1;
     SUCEM_LAUNCH((k_residual<C>), G, TB, 0, c->stream, n, G, b, ax.p, op.mask, dinv.p, r.p, 1,
@@ -759,6 +853,32 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
     op.k = k;
     op.identity_prec = o.precond == SUCEM_PRECOND_NONE;
     krylov<float>(op, b, x, o, st, hist_h, hist_cap);
+    if (st.fp32_failed) {
+      // finish in FP64, warm-started from the FP32 cycles' accepted solution
+      SolveOpts o64 = o;
+      o64.precision = 64;
+      o64.warm = 1;
+      o64.max_iter = std::max(1, o.max_iter - st.iterations);
+      SysOp<double> op64;
+      fill(op64);
+      op64.flo = op64.fhi;
+      op64.sys = sys;
+      op64.k = k;
+      op64.identity_prec = op.identity_prec;
+      SolveStats s64;
+      krylov<double>(op64, b, x, o64, s64, nullptr, 0);
+      const SolveStats s32 = st;
+      st = s64;
+      st.iterations += s32.iterations;
+      st.reliable_updates += s32.reliable_updates;
+      st.spmv_lo += s32.spmv_lo;
+      st.spmv_hi += s32.spmv_hi;
+      st.elapsed_ms += s32.elapsed_ms;
+      st.fp32_failed = 1;
+      st.op_ms = s32.op_ms;
+      st.op_samples = s32.op_samples;
+      if (st.status > 0) st.status = st.iterations;
+    }
   } else {
     SysOp<double> op;
     fill(op);
@@ -786,6 +906,27 @@ void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const Sol
     op.A = A;
     op.identity_prec = o.precond == SUCEM_PRECOND_NONE;
     krylov<float>(op, b, x, o, st, hist_h, hist_cap);
+    if (st.fp32_failed) {
+      SolveOpts o64 = o;
+      o64.precision = 64;
+      o64.warm = 1;
+      o64.max_iter = std::max(1, o.max_iter - st.iterations);
+      CsrOp<double> op64;
+      fill(op64);
+      op64.A = A;
+      op64.identity_prec = op.identity_prec;
+      SolveStats s64;
+      krylov<double>(op64, b, x, o64, s64, nullptr, 0);
+      const SolveStats s32 = st;
+      st = s64;
+      st.iterations += s32.iterations;
+      st.reliable_updates += s32.reliable_updates;
+      st.spmv_lo += s32.spmv_lo;
+      st.spmv_hi += s32.spmv_hi;
+      st.elapsed_ms += s32.elapsed_ms;
+      st.fp32_failed = 1;
+      if (st.status > 0) st.status = st.iterations;
+    }
   } else {
     CsrOp<double> op;
     fill(op);

@@ -431,6 +431,20 @@ template <typename R>
 SpmvCfg sys_cfg(Context* c, const System* sys) {
   const SpmvCfg& t = sys->cfg[sizeof(R) == 4 ? 0 : 1];
   if (t.L > 0 || t.mf) return t;
+  // Nothing tuned or set: the patch operator (what the autotuner picks on the driven
+  // cubes, 8-10x faster than the assembled CSR operator), with the patch size it picks
+  // (FP64 order 2: 192 cells, else 256), when the space suits it; the choice is kept on
+  // the system (its maps and coefficients are lazily built caches of the system).
+  System* ms = const_cast<System*>(sys);
+  const int cb = (sys->space->order == 2 && sizeof(R) == 8) ? 192 : 256;
+  if (sys->space->nloc == 6 || sys->space->nloc == 20) {
+    if (pt_prepare(c, ms, cb) && pt_fits<R>(sys->space, cb)) {
+      const int64_t npd = sys->space->pt[cb == 192 ? 1 : 2].npd;
+      SpmvCfg f = pt_cfg(sys->space, cb, op_grid(c, std::max<int64_t>(npd, 1), 16));
+      ms->cfg[sizeof(R) == 4 ? 0 : 1] = f;
+      return f;
+    }
+  }
   return default_cfg(c, sys->space->ndofs, sys->space->nnz);
 }
 template SpmvCfg sys_cfg<float>(Context*, const System*);

@@ -31,7 +31,7 @@ class DeviceProblem(object):
     """Mesh + space + assembled system living on the device."""
 
     def __init__(self, n, dims, order, which, eps_values=(1.0,), seed=0, shift=(0.0, 0.0, 0.0),
-                 ctx=None):
+                 ctx=None, region=None):
         self.ctx = ctx or get_context()
         self.n = (int(n),) * 3 if np.isscalar(n) else tuple(int(v) for v in n)
         self.dims = np.asarray(dims, np.float64) * np.ones(3)
@@ -40,7 +40,10 @@ class DeviceProblem(object):
                                        export=False)
         mh = self.mesh._device[1]
         eps_values = np.asarray(eps_values, np.float64)
-        self.region = box_regions(self.n, eps_values, seed) if len(eps_values) > 1 else None
+        if region is not None:
+            self.region = np.ascontiguousarray(region, np.int64)
+        else:
+            self.region = box_regions(self.n, eps_values, seed) if len(eps_values) > 1 else None
         ids = np.arange(len(eps_values), dtype=np.int64)
         mu = np.ones(len(eps_values))
         _hip.call("sucem_mesh_set_materials", mh.h, _hip.ptr(self.region), len(ids), _hip.ptr(ids),
@@ -122,6 +125,26 @@ class DeviceProblem(object):
         k0 = np.ascontiguousarray(k0s, np.float64)
         _hip.call("sucem_solve_driven_batch", self.ctx.handle, self.system.handle.h, nb,
                   _hip.ptr(k0), _hip.ptr(B), _hip.ptr(X), ctypes.byref(o), st)
+        return [s.as_dict() for s in st]
+
+    def solve_sweep(self, k0s, B, X, precision=32, tol=1e-8, max_iter=50000, check_every=16,
+                    width=8, use_graph=True, ru_delta=0.01):
+        """Solve len(k0s) driven systems together (``sucem_solve_driven_sweep``): ``width``
+        interleaved COCG recursions at a time sharing each operator pass, the iteration
+        replayed from a hipGraph.  B, X: (nb, N) complex128 device tensors."""
+        import torch
+        nb = len(k0s)
+        self._check_vec(B, torch.complex128, nb)
+        self._check_vec(X, torch.complex128, nb)
+        o = _hip.SolveOpts()
+        _hip.load().sucem_solve_opts_default(ctypes.byref(o))
+        o.method, o.precision, o.tol, o.max_iter = _hip.METHOD_COCG, precision, tol, max_iter
+        o.check_every, o.precond, o.ru_delta = check_every, _hip.PRECOND_AUXILIARY, ru_delta
+        st = (_hip.SolveStats * nb)()
+        k0 = np.ascontiguousarray(k0s, np.float64)
+        _hip.call("sucem_solve_driven_sweep", self.ctx.handle, self.system.handle.h, nb,
+                  _hip.ptr(k0), _hip.ptr(B), _hip.ptr(X), ctypes.byref(o), int(width),
+                  int(bool(use_graph)), st)
         return [s.as_dict() for s in st]
 
     def eigs(self, sigma, nev, ncv=0, tol=1e-10, inner_tol=1e-11, vectors=False,
@@ -245,6 +268,68 @@ class DeviceProblem(object):
         sx = 8 if precision == 32 else 16          # complex vector element
         n, nnz, nnz0 = self.ndofs, self.nnz, self.nnz0
         return (nnz * (4 + sv) + (n + 1) * 4 + nnz0 * (4 + s0v) + (n + 1) * 4 + 2 * n * sx)
+
+
+def dielectric_block(n, seed, frac=(0.1, 0.3)):
+    """A box-aligned block of whole boxes occupying a volume fraction in ``frac`` of an
+    (nx, ny, nz) box grid, drawn with ``numpy.random.default_rng(seed)``: per axis an
+    extent of round(a n_i) boxes (at least 1), a ~ U[0.3, 0.9], redrawn until the fraction
+    fits, at a uniform offset.  Returns (region per cell: 1 inside, 0 outside; the block's
+    (lo, hi) box indices; its volume fraction).  Cells of box (ix, iy, iz) are cells
+    6 b .. 6 b + 5 with b = ix + nx (iy + ny iz)."""
+    n = np.asarray(n, np.int64)
+    rng = np.random.default_rng(seed)
+    for _ in range(10000):
+        ext = np.maximum(1, np.rint(rng.uniform(0.3, 0.9, size=3) * n)).astype(np.int64)
+        f = float(np.prod(ext / n))
+        if frac[0] <= f <= frac[1]:
+            break
+    else:
+        raise ValueError("no block with a volume fraction in %r on a %r grid" % (frac, tuple(n)))
+    lo = np.array([rng.integers(0, n[i] - ext[i] + 1) for i in range(3)], np.int64)
+    hi = lo + ext
+    ix, iy, iz = np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+    inside = ((ix >= lo[0]) & (ix < hi[0]) & (iy >= lo[1]) & (iy < hi[1])
+              & (iz >= lo[2]) & (iz < hi[2]))
+    box = inside.transpose(2, 1, 0).reshape(-1)           # box index ix + nx (iy + ny iz)
+    return np.repeat(box.astype(np.int64), 6), (lo, hi), f
+
+
+def dielectric_cavity(n=(120, 60, 90), dims=(1.0, 0.5, 0.75), order=2, eps_block=6.0, seed=0,
+                      ctx=None):
+    """The large eigenproblem: a PEC box cavity of n boxes with a seeded box-aligned
+    dielectric block (eps_r = eps_block, 10-30 % of the volume), S and T assembled."""
+    region, block, f = dielectric_block(n, seed)
+    prob = DeviceProblem(n, dims, order, _hip.MAT_S | _hip.MAT_T, (1.0, float(eps_block)),
+                         seed, ctx=ctx, region=region)
+    prob.block, prob.block_fraction = block, f
+    return prob
+
+
+def sweep_sources(nfreq=64, ndip=8, L=1.0, f_lo=0.3e9, f_hi=1.5e9, seed=0):
+    """The frequency sweep's inputs: nfreq frequencies uniform in [f_lo, f_hi] and, per
+    frequency, ndip unit dipoles with directions uniform on the sphere and positions uniform
+    in the inner 60 % of the cube (drawn with ``numpy.random.default_rng(seed)``).  Returns
+    (freqs (nfreq,), positions (nfreq, ndip, 3), moments (nfreq, ndip, 3))."""
+    rng = np.random.default_rng(seed)
+    freqs = np.linspace(f_lo, f_hi, nfreq)
+    pos = rng.uniform(0.2 * L, 0.8 * L, size=(nfreq, ndip, 3))
+    d = rng.standard_normal(size=(nfreq, ndip, 3))
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    return freqs, pos, d
+
+
+def sweep_rhs(prob, freqs, pos, moments):
+    """Right-hand sides of the sweep on the device: (nfreq * ndip, N) complex128, system
+    f * ndip + s = dipole s at frequency f, and the k0 of each system."""
+    nfreq, ndip = pos.shape[:2]
+    k0s = np.repeat(2 * np.pi * np.asarray(freqs) / c0, ndip)
+    blocks = []
+    for f in range(nfreq):
+        blocks.append(prob.rhs_dipoles(pos[f], moments[f].astype(np.complex128), k0s[f * ndip],
+                                       rhs_index=np.arange(ndip), nrhs=ndip))
+    import torch
+    return torch.cat(blocks).contiguous(), k0s
 
 
 def driven_cube(n=40, L=1.0, order=2, freq=1e9, eps_values=(1.0, 2.2, 4.4), seed=0, ctx=None):

@@ -28,10 +28,11 @@ class SolveOpts(ctypes.Structure):
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int),
-                ("reliable_updates", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("reliable_updates", ctypes.c_int), ("stagnated", ctypes.c_int),
                 ("rel_residual", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
                 ("spmv_lo", ctypes.c_int64), ("spmv_hi", ctypes.c_int64),
-                ("op_ms", ctypes.c_double), ("op_samples", ctypes.c_int64)]
+                ("op_ms", ctypes.c_double), ("op_samples", ctypes.c_int64),
+                ("fp64_finish", ctypes.c_int), ("pad_", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
@@ -47,7 +48,7 @@ class EigOpts(ctypes.Structure):
 class EigStats(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("nconv", ctypes.c_int), ("restarts", ctypes.c_int),
                 ("lanczos_steps", ctypes.c_int), ("inner_failures", ctypes.c_int),
-                ("pad_", ctypes.c_int), ("inner_iterations", ctypes.c_int64),
+                ("inner_stagnated", ctypes.c_int), ("inner_iterations", ctypes.c_int64),
                 ("max_ritz_residual", ctypes.c_double), ("elapsed_ms", ctypes.c_double)]
 
     def as_dict(self):
@@ -70,6 +71,9 @@ SIGNATURES = {
     "sucem_ctx_create": (ctypes.c_int, [ctypes.c_int, _P, _PP]),
     "sucem_ctx_destroy": (ctypes.c_int, [_P]),
     "sucem_ctx_synchronize": (ctypes.c_int, [_P]),
+    "sucem_ctx_workspace_bytes": (ctypes.c_int, [_P, _P]),
+    "sucem_ctx_release_workspace": (ctypes.c_int, [_P]),
+    "sucem_ctx_mem_info": (ctypes.c_int, [_P, _P, _P]),
     "sucem_mesh_create": (ctypes.c_int, [_P, _P, _I64, _P, _I64, _PP]),
     "sucem_mesh_create_box": (ctypes.c_int, [_P, _I64, _I64, _I64, _P, _P, _PP]),
     "sucem_mesh_info": (ctypes.c_int, [_P, _P, _P]),
@@ -102,6 +106,8 @@ SIGNATURES = {
     "sucem_solve_opts_default": (None, [_P]),
     "sucem_solve_driven": (ctypes.c_int, [_P, _P, _D, _P, _P, _P, _P, _P, _I64]),
     "sucem_solve_driven_batch": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _P, _P]),
+    "sucem_solve_driven_sweep": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _P, ctypes.c_int,
+                                                ctypes.c_int, _P]),
     "sucem_solve_system": (ctypes.c_int, [_P, _P, _D, _D, _D, _P, _P, _P, _P, _P, _I64]),
     "sucem_eig_opts_default": (None, [_P]),
     "sucem_eigs": (ctypes.c_int, [_P, _P, _D, ctypes.c_int, _P, _P, _P, _P, _P]),

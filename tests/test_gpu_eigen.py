@@ -63,13 +63,66 @@ def test_reference_return_convention():
     assert np.allclose(k2, ref, rtol=1e-8)
 
 
-def test_fp32_inner_solves_are_rejected():
-    """FP32 inner solves of S - sigma T stall (measured); the solver refuses them."""
+@pytest.mark.parametrize("order", [1, 2])
+def test_fp32_inner_solves_match_oracle(order):
+    """FP32 inner solves (refinement cycles of FP32 auxiliary-space COCG restarted from the
+    FP64 residual) give the same 10 lowest k^2 as the oracle's pencil to 1e-8."""
     from sucemfem.ProblemConfigurations.EMVectorWaveEigenproblem import DefaultEigenSolver
-    ep, _ = _cavity(1)
+    print("cavity order %d, FP32 inner solves" % order, flush=True)
+    ep, (co, ce) = _cavity(order)
     es = DefaultEigenSolver()
     es.set_eigenproblem(ep)
     es.set_sigma(1.1)
     es.set_options(inner_precision=32)
-    with pytest.raises(NotImplementedError, match="inner_precision 64"):
-        es.solve_problem_k2(4)
+    k2, modes, resid = es.solve_problem_k2(10)
+    assert es.stats["status"] == 0 and es.stats["inner_failures"] == 0, es.stats
+    ref, _, _ = O.cavity_eigs(O.Space(co, ce, order), 10, 1.1)
+    assert np.all(np.abs(k2 - ref) / ref <= 1e-8), (k2, ref)
+    assert np.all(resid < 1e-7)
+
+
+def _block_cavity(n, order, seed, inner_precision):
+    import torch
+    torch.cuda.set_device(0)
+    from sucemfem.Utilities.Workloads import dielectric_cavity
+    prob = dielectric_cavity(n=n, dims=DIMS, order=order, eps_block=6.0, seed=seed)
+    k2, resid, st = prob.eigs(1.1, 20, inner_precision=inner_precision)
+    return prob, k2, resid, st
+
+
+@pytest.mark.parametrize("inner_precision", [64, 32])
+def test_dielectric_block_cavity_20_modes(inner_precision):
+    """The large eigenproblem's workload at oracle size: the 1.0 x 0.5 x 0.75 PEC cavity with
+    a seeded eps_r = 6 block of whole boxes (10-30 % of the volume), order 2, the 20 lowest
+    k^2 above sigma = 1.1, against the oracle's shift-invert Lanczos on the same pencil."""
+    n = (8, 4, 6)
+    print("dielectric block cavity %s, inner FP%d" % (n, inner_precision), flush=True)
+    prob, k2, resid, st = _block_cavity(n, 2, 0, inner_precision)
+    assert st["status"] == 0 and st["inner_failures"] == 0, st
+    co, ce = O.box_mesh(DIMS, n)
+    eps = np.where(prob.region == 1, 6.0, 1.0)
+    assert 0.1 <= prob.block_fraction <= 0.3 and eps.mean() > 1.0
+    osp = O.Space(co, ce, 2)
+    assert osp.ndofs == prob.ndofs and osp.ndofs > 4000   # the oracle's eigsh branch
+    ref, _, _ = O.cavity_eigs(osp, 20, 1.1, eps_r=eps)
+    assert np.all(np.abs(k2 - ref) / ref <= 1e-8), (k2, ref)
+    assert np.all(resid < 1e-7), resid
+    # the dielectric lowers the spectrum below the empty cavity's
+    empty = O.analytic_cavity_k2(DIMS, 1)[0]
+    assert k2[0] < empty
+
+
+def test_homogeneous_eps_scales_device_k2():
+    """eps_r = 2.5 in every cell divides every k^2 by 2.5 (closed form: T scales, S not)."""
+    import torch
+    torch.cuda.set_device(0)
+    from sucemfem import _hip
+    from sucemfem.Utilities.Workloads import DeviceProblem
+    print("homogeneous eps cavity", flush=True)
+    which = _hip.MAT_S | _hip.MAT_T
+    p1 = DeviceProblem(3, DIMS, 2, which)
+    pe = DeviceProblem(3, DIMS, 2, which, eps_values=(2.5, 2.5), seed=1)
+    k1, _, s1 = p1.eigs(1.1, 8)
+    ke, _, se = pe.eigs(1.1 / 2.5, 8)
+    assert s1["status"] == 0 and se["status"] == 0
+    assert np.allclose(ke * 2.5, k1, rtol=1e-9, atol=0), (ke * 2.5, k1)

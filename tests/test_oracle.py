@@ -395,3 +395,30 @@ def test_rectangular_prism_cavity_modes(golden):
     k2, _, _ = O.cavity_eigs(sp_, 4, 1.1, facet_flags=d["facet_region"] == 1)
     ana = O.analytic_cavity_k2([1.0, 0.5, 0.25], 4)
     assert np.all(np.abs(k2 - ana) / ana < 0.05), (k2, ana)
+
+
+def test_cavity_homogeneous_eps_scales_k2():
+    """A cavity filled with eps_r = e everywhere has k^2 = k^2(vacuum) / e (T scales by e,
+    S does not), and 1/mu_r = m everywhere gives m k^2: the per-cell material weights enter
+    T and S, and nothing else."""
+    co, ce = O.box_mesh([1.0, 0.5, 0.75], 3)
+    sp_ = O.Space(co, ce, 2)
+    nc = len(ce)
+    w1, _, _ = O.cavity_eigs(sp_, 6, 0.5)
+    we, _, _ = O.cavity_eigs(sp_, 6, 0.5 / 2.5, eps_r=np.full(nc, 2.5))
+    assert np.allclose(we * 2.5, w1, rtol=1e-11, atol=0)
+    wm, _, _ = O.cavity_eigs(sp_, 6, 0.5 * 0.4, inv_mu_r=np.full(nc, 0.4))
+    assert np.allclose(wm / 0.4, w1, rtol=1e-11, atol=0)
+    # a planted mistake (eps applied to S instead of T) would give k^2 e, not k^2 / e
+    assert not np.allclose(w1 * 2.5, w1 / 2.5)
+
+
+def test_dielectric_block_workload_is_seeded():
+    """The large-eigenproblem block: box-aligned, 10-30 % of the volume, reproducible."""
+    from sucemfem.Utilities.Workloads import dielectric_block
+    r1, (lo, hi), f = dielectric_block((120, 60, 90), 0)
+    r2, _, _ = dielectric_block((120, 60, 90), 0)
+    assert np.array_equal(r1, r2) and 0.1 <= f <= 0.3
+    assert r1.mean() == f and len(r1) == 6 * 120 * 60 * 90
+    nb = (hi - lo).prod()
+    assert r1.sum() == 6 * nb
