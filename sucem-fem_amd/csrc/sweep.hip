@@ -79,9 +79,39 @@ __device__ __forceinline__ double2 block_sum_nb(double2 v) {
 }
 
 // ---- operator -------------------------------------------------------------------
-// One workgroup per patch; systems in turn.  kco = {ct_0..ct_NB-1, c0_0..c0_NB-1} (cs = 1).
-// part: partial sums [slot][NB]; dp: with EPI, block partials of p_j.q_j at dp[j Gp + lb].
-template <typename R, int ORDER, int NFD, int NB, int CB, bool EPI>
+// Block sums of NBL values at once (two barriers for all of them); every thread gets them.
+template <int CB, int NBL>
+__device__ __forceinline__ void block_sum_multi(double2 (&v)[NBL]) {
+  __shared__ double2 red[CB / 64][NBL];
+#pragma unroll
+  for (int jj = 0; jj < NBL; ++jj)
+    for (int o = 32; o > 0; o >>= 1) {
+      v[jj].x += __shfl_xor(v[jj].x, o);
+      v[jj].y += __shfl_xor(v[jj].y, o);
+    }
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int jj = 0; jj < NBL; ++jj) red[w][jj] = v[jj];
+  __syncthreads();
+#pragma unroll
+  for (int jj = 0; jj < NBL; ++jj) {
+    double2 t = red[0][jj];
+    for (int i = 1; i < CB / 64; ++i) {
+      t.x += red[i][jj].x;
+      t.y += red[i][jj].y;
+    }
+    v[jj] = t;
+  }
+}
+
+// One workgroup per patch; the systems in groups of NBL: each phase (gather of x, element
+// products, segment sums) handles a whole group, with the patch's LDS copy laid out
+// [slot][NBL].  The cell's coefficients and LDS places are loaded once for all systems.
+// kco = {ct_0..ct_NB-1, c0_0..c0_NB-1} (cs = 1).  part: partial sums [slot][NB]; dp: with
+// EPI, block partials of p_j.q_j at dp[j Gp + lb].
+template <typename R, int ORDER, int NFD, int NB, int NBL, int CB, bool EPI>
 __global__ __launch_bounds__(CB) void k_sw_patch(
     int64_t np, int Gp, int64_t nc, int64_t nbf, int lds_main, const R* __restrict__ coef,
     const uint32_t* __restrict__ lp, const int32_t* __restrict__ uptr,
@@ -94,6 +124,7 @@ __global__ __launch_bounds__(CB) void k_sw_patch(
     typename C2<R>::T* __restrict__ y, typename C2<R>::T* __restrict__ part,
     double2* __restrict__ dp) {
   using C = typename C2<R>::T;
+  static_assert(NB % NBL == 0, "groups of NBL systems");
   constexpr int NLOC = MfTab<ORDER>::NLOC;
   constexpr int KK = NFD * NFD;
   extern __shared__ double2 smem_[];
@@ -111,7 +142,6 @@ __global__ __launch_bounds__(CB) void k_sw_patch(
     nfr = (fptr[p + 1] - f0) * NFD;
     has = r < nc;
   }
-  // the cell's coefficients and LDS places: loaded once, used for every system
   R pa[12];
   uint32_t pli[NLOC];
   if (has) {
@@ -120,10 +150,11 @@ __global__ __launch_bounds__(CB) void k_sw_patch(
 #pragma unroll
     for (int i = 0; i < NLOC; ++i) pli[i] = lp[(int64_t)i * nc + r];
   }
-  C* stage = sm + lds_main;
-  for (int j = 0; j < NB; ++j) {
-    const R ct = (R)kco[j], c0 = (R)kco[NB + j];
-    double2 d1 = make_double2(0.0, 0.0);
+  C* stage = sm + (int64_t)lds_main * NBL;
+  for (int jb = 0; jb < NB; jb += NBL) {
+    double2 d1[NBL];
+#pragma unroll
+    for (int jj = 0; jj < NBL; ++jj) d1[jj] = make_double2(0.0, 0.0);
     if (live) {
       for (int j0 = 0; j0 < nu; j0 += UBS * CB) {
         int dv[UBS];
@@ -132,29 +163,37 @@ __global__ __launch_bounds__(CB) void k_sw_patch(
           const int jj = j0 + q * CB + threadIdx.x;
           dv[q] = jj < nu ? pdof[u0 + jj] : -1;
         }
-        C xv[UBS];
-#pragma unroll
-        for (int q = 0; q < UBS; ++q)
-          xv[q] = dv[q] >= 0 ? x[(int64_t)dv[q] * NB + j] : mk(R(0), R(0));
 #pragma unroll
         for (int q = 0; q < UBS; ++q) {
-          const int jj = j0 + q * CB + threadIdx.x;
-          if (jj < nu) sm[jj] = xv[q];
+          const int u = j0 + q * CB + threadIdx.x;
+          if (dv[q] >= 0) {
+            C xv[NBL];
+#pragma unroll
+            for (int jj = 0; jj < NBL; ++jj) xv[jj] = x[(int64_t)dv[q] * NB + jb + jj];
+#pragma unroll
+            for (int jj = 0; jj < NBL; ++jj) sm[u * NBL + jj] = xv[jj];
+          }
         }
       }
     }
     __syncthreads();
-    C acc[NLOC];
+    C acc[NBL][NLOC];
     if (has) {
-      R a[12];
 #pragma unroll
-      for (int k = 0; k < 12; ++k) a[k] = pa[k] * (k < 6 ? ct : R(1)) * R(1.0 / MfTab<ORDER>::den);
-      C xl[NLOC];
+      for (int jj = 0; jj < NBL; ++jj) {
+        const R ct = (R)kco[jb + jj];
+        R a[12];
 #pragma unroll
-      for (int i = 0; i < NLOC; ++i) xl[i] = sm[pli[i] & 0xffffu];
+        for (int k = 0; k < 12; ++k)
+          a[k] = pa[k] * (k < 6 ? ct : R(1)) * R(1.0 / MfTab<ORDER>::den);
+        C xl[NLOC];
 #pragma unroll
-      for (int i = 0; i < NLOC; ++i) acc[i] = mk(R(0), R(0));
-      element_product<R, ORDER>(a, xl, acc, std::make_integer_sequence<int, MfTab<ORDER>::NP>{});
+        for (int i = 0; i < NLOC; ++i) xl[i] = sm[(pli[i] & 0xffffu) * NBL + jj];
+#pragma unroll
+        for (int i = 0; i < NLOC; ++i) acc[jj][i] = mk(R(0), R(0));
+        element_product<R, ORDER>(a, xl, acc[jj],
+                                  std::make_integer_sequence<int, MfTab<ORDER>::NP>{});
+      }
     }
     for (int id = threadIdx.x; id < nfr; id += CB) {
       const int64_t fp = f0 + id / NFD;
@@ -166,25 +205,40 @@ __global__ __launch_bounds__(CB) void k_sw_patch(
 #pragma unroll
       for (int k = 0; k < 6; ++k) hk[k] = hf[k * nbf + fp];
       const double* rf = ref_face + lf * 6 * KK + aa * NFD;
-      C v = mk(R(0), R(0));
+      C v[NBL];
+#pragma unroll
+      for (int jj = 0; jj < NBL; ++jj) v[jj] = mk(R(0), R(0));
 #pragma unroll
       for (int b = 0; b < NFD; ++b) {
         R m = R(0);
 #pragma unroll
         for (int k = 0; k < 6; ++k) m += hk[k] * (R)rf[k * KK + b];
         const uint32_t li = lp[(int64_t)face_dofs[lf * NFD + b] * nc + rk] & 0xffffu;
-        const C xb = sm[li];
-        v.x += m * xb.x;
-        v.y += m * xb.y;
+#pragma unroll
+        for (int jj = 0; jj < NBL; ++jj) {
+          const C xb = sm[li * NBL + jj];
+          v[jj].x += m * xb.x;
+          v[jj].y += m * xb.y;
+        }
       }
-      stage[id] = mk(-c0 * v.y, c0 * v.x);
+#pragma unroll
+      for (int jj = 0; jj < NBL; ++jj) {
+        const R c0 = (R)kco[NB + jb + jj];
+        stage[id * NBL + jj] = mk(-c0 * v[jj].y, c0 * v[jj].x);
+      }
     }
     __syncthreads();   // the x copy is no longer read: the products overwrite it
     if (has) {
 #pragma unroll
-      for (int i = 0; i < NLOC; ++i) sm[pli[i] >> 16] = acc[i];
+      for (int i = 0; i < NLOC; ++i)
+#pragma unroll
+        for (int jj = 0; jj < NBL; ++jj) sm[(pli[i] >> 16) * NBL + jj] = acc[jj][i];
     }
-    for (int id = threadIdx.x; id < nfr; id += CB) sm[fpos[(int64_t)f0 * NFD + id]] = stage[id];
+    for (int id = threadIdx.x; id < nfr; id += CB) {
+      const int t = fpos[(int64_t)f0 * NFD + id];
+#pragma unroll
+      for (int jj = 0; jj < NBL; ++jj) sm[t * NBL + jj] = stage[id * NBL + jj];
+    }
     __syncthreads();
     if (live) {
       for (int j0 = 0; j0 < nu; j0 += UBS * CB) {
@@ -200,33 +254,44 @@ __global__ __launch_bounds__(CB) void k_sw_patch(
         }
 #pragma unroll
         for (int q = 0; q < UBS; ++q) {
-          C v = mk(R(0), R(0));
+          if (hi[q] == 0) continue;   // not an entry of this patch (segments are never empty)
+          C v[NBL];
+#pragma unroll
+          for (int jj = 0; jj < NBL; ++jj) v[jj] = mk(R(0), R(0));
           for (int t = lo[q]; t < hi[q]; ++t) {
-            const C s = sm[t];
-            v.x += s.x;
-            v.y += s.y;
+#pragma unroll
+            for (int jj = 0; jj < NBL; ++jj) {
+              const C s = sm[t * NBL + jj];
+              v[jj].x += s.x;
+              v[jj].y += s.y;
+            }
           }
-          if (hi[q] > 0) {
+          const bool masked = out[q] < 0 && mask && mask[dv[q]];
+#pragma unroll
+          for (int jj = 0; jj < NBL; ++jj) {
+            C w = v[jj];
             if (out[q] >= 0) {
-              part[(int64_t)out[q] * NB + j] = v;
+              part[(int64_t)out[q] * NB + jb + jj] = w;
             } else {
-              if (mask && mask[dv[q]]) v = mk(R(0), R(0));
-              y[(int64_t)dv[q] * NB + j] = v;
+              if (masked) w = mk(R(0), R(0));
+              y[(int64_t)dv[q] * NB + jb + jj] = w;
             }
             if constexpr (EPI) {
               // p.Ap over the finished rows and the partial sums (linear in q; p is zero
               // on masked rows)
-              const double2 xd = to_d(x[(int64_t)dv[q] * NB + j]), yd = to_d(v);
-              d1.x += xd.x * yd.x - xd.y * yd.y;
-              d1.y += xd.x * yd.y + xd.y * yd.x;
+              const double2 xd = to_d(x[(int64_t)dv[q] * NB + jb + jj]), yd = to_d(w);
+              d1[jj].x += xd.x * yd.x - xd.y * yd.y;
+              d1[jj].y += xd.x * yd.y + xd.y * yd.x;
             }
           }
         }
       }
     }
     if constexpr (EPI) {
-      d1 = block_sum<CB>(d1);   // its barriers also separate this system's reads of the
-      if (threadIdx.x == 0) dp[(int64_t)j * Gp + lb] = d1;   // products from the next gather
+      block_sum_multi<CB, NBL>(d1);   // its barriers also separate this group's reads of the
+      if (threadIdx.x == 0)            // products from the next group's gather
+#pragma unroll
+        for (int jj = 0; jj < NBL; ++jj) dp[(int64_t)(jb + jj) * Gp + lb] = d1[jj];
     } else {
       __syncthreads();
     }
@@ -257,24 +322,48 @@ __global__ __launch_bounds__(TB) void k_sw_psum(int npd, int G, const int32_t* _
   }
 }
 
-// ---- scalars (one block) ---------------------------------------------------------
+// ---- scalars: one block, one wavefront per system ------------------------------------
+// Lane l of wavefront j sums system j's block partials l, l + 64, ... (four independent
+// running sums, combined in a fixed order), then the wavefront reduces: the order of every
+// sum is fixed, and the systems are summed in parallel.
+__device__ __forceinline__ double2 wave_sum_partials(const double2* __restrict__ a, int G) {
+  const int lane = threadIdx.x & 63;
+  double2 s[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) s[k] = make_double2(0.0, 0.0);
+  int b = lane;
+  for (; b + 192 < G; b += 256) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double2 t = a[b + 64 * k];
+      s[k].x += t.x;
+      s[k].y += t.y;
+    }
+  }
+  for (int k = 0; b < G; b += 64, ++k) {
+    const double2 t = a[b];
+    s[k & 3].x += t.x;
+    s[k & 3].y += t.y;
+  }
+  double2 v = make_double2((s[0].x + s[1].x) + (s[2].x + s[3].x),
+                           (s[0].y + s[1].y) + (s[2].y + s[3].y));
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o);
+    v.y += __shfl_xor(v.y, o);
+  }
+  return v;
+}
+
 template <int NB>
-__global__ __launch_bounds__(TB) void k_sw_alpha(int Gp, const double2* __restrict__ dp,
-                                                 const int* __restrict__ act,
-                                                 double* __restrict__ sc) {
-  for (int j = 0; j < NB; ++j) {
-    double2 s = make_double2(0.0, 0.0);
-    for (int b = threadIdx.x; b < Gp; b += TB) {
-      const double2 t = dp[(int64_t)j * Gp + b];
-      s.x += t.x;
-      s.y += t.y;
-    }
-    s = block_sum<TB>(s);
-    if (threadIdx.x == 0) {
-      const double2 al = act[j] ? cdiv0(scal2(sc + j * SW, SW_RHO), s) : make_double2(0, 0);
-      sc[j * SW + SW_ALPHA] = al.x;
-      sc[j * SW + SW_ALPHA + 1] = al.y;
-    }
+__global__ __launch_bounds__(64 * NB) void k_sw_alpha(int Gp, const double2* __restrict__ dp,
+                                                      const int* __restrict__ act,
+                                                      double* __restrict__ sc) {
+  const int j = threadIdx.x >> 6;
+  const double2 s = wave_sum_partials(dp + (int64_t)j * Gp, Gp);
+  if ((threadIdx.x & 63) == 0) {
+    const double2 al = act[j] ? cdiv0(scal2(sc + j * SW, SW_RHO), s) : make_double2(0, 0);
+    sc[j * SW + SW_ALPHA] = al.x;
+    sc[j * SW + SW_ALPHA + 1] = al.y;
   }
 }
 
@@ -282,50 +371,39 @@ __global__ __launch_bounds__(TB) void k_sw_alpha(int Gp, const double2* __restri
 // the running systems (beta 0 for the others); mode 1 (restart): rho_j = (r.z)_j for the
 // systems with sel[j].
 template <int NB>
-__global__ __launch_bounds__(TB) void k_sw_beta(int G, const double2* __restrict__ dp,
-                                                const int* __restrict__ act,
-                                                const int* __restrict__ sel, double* sc,
-                                                double* __restrict__ ring, int k, int mode) {
-  for (int j = 0; j < NB; ++j) {
-    double2 s0 = make_double2(0.0, 0.0), s1 = make_double2(0.0, 0.0);
-    for (int b = threadIdx.x; b < G; b += TB) {
-      const double2 t = dp[(int64_t)j * G + b], u = dp[(int64_t)(NB + j) * G + b];
-      s0.x += t.x;
-      s0.y += t.y;
-      s1.x += u.x;
-    }
-    s0 = block_sum<TB>(s0);
-    s1 = block_sum<TB>(s1);
-    if (threadIdx.x == 0) {
-      double* s = sc + j * SW;
-      if (mode == 0) {
-        const bool on = act[j] != 0;
-        const double2 be = on ? cdiv0(s0, scal2(s, SW_RHO)) : make_double2(0, 0);
-        s[SW_BETA] = be.x;
-        s[SW_BETA + 1] = be.y;
-        if (on) {
-          s[SW_RHO] = s0.x;
-          s[SW_RHO + 1] = s0.y;
-        }
-        ring[k * NB + j] = sqrt(s1.x);
-      } else if (sel[j]) {
+__global__ __launch_bounds__(64 * NB) void k_sw_beta(int G, const double2* __restrict__ dp,
+                                                     const int* __restrict__ act,
+                                                     const int* __restrict__ sel, double* sc,
+                                                     double* __restrict__ ring, int k, int mode) {
+  const int j = threadIdx.x >> 6;
+  const double2 s0 = wave_sum_partials(dp + (int64_t)j * G, G);
+  const double2 s1 = wave_sum_partials(dp + (int64_t)(NB + j) * G, G);
+  if ((threadIdx.x & 63) == 0) {
+    double* s = sc + j * SW;
+    if (mode == 0) {
+      const bool on = act[j] != 0;
+      const double2 be = on ? cdiv0(s0, scal2(s, SW_RHO)) : make_double2(0, 0);
+      s[SW_BETA] = be.x;
+      s[SW_BETA + 1] = be.y;
+      if (on) {
         s[SW_RHO] = s0.x;
         s[SW_RHO + 1] = s0.y;
       }
+      ring[k * NB + j] = sqrt(s1.x);
+    } else if (sel[j]) {
+      s[SW_RHO] = s0.x;
+      s[SW_RHO + 1] = s0.y;
     }
   }
 }
 
 // sums[j] = sum of the block partials dp[j G + b] (.x), for norms
 template <int NB>
-__global__ __launch_bounds__(TB) void k_sw_sum(int G, const double2* __restrict__ dp,
-                                               double* __restrict__ sums) {
-  for (int j = 0; j < NB; ++j) {
-    double2 s = make_double2(0.0, 0.0);
-    for (int b = threadIdx.x; b < G; b += TB) s.x += dp[(int64_t)j * G + b].x;
-    s = block_sum<TB>(s);
-    if (threadIdx.x == 0) sums[j] = s.x;
-  }
+__global__ __launch_bounds__(64 * NB) void k_sw_sum(int G, const double2* __restrict__ dp,
+                                                    double* __restrict__ sums) {
+  const int j = threadIdx.x >> 6;
+  const double2 s = wave_sum_partials(dp + (int64_t)j * G, G);
+  if ((threadIdx.x & 63) == 0) sums[j] = s.x;
 }
 
 // ---- vector kernels (element e = row NB + j; TB is a multiple of NB, so a thread keeps
@@ -743,6 +821,9 @@ template <typename R, int NB, int ORDER>
 struct Chunk {
   using C = typename C2<R>::T;
   static constexpr int NFD = ORDER == 2 ? 8 : 3;
+  // systems per phase of the working-precision patch kernel: order 1 takes four at a time
+  // (its patch copy is small: ~60 KB of LDS for four FP32 systems); order 2 one
+  static constexpr int NBL = (ORDER == 1 && NB >= 4) ? 4 : 1;
   static constexpr int LV = NB >= 8 ? 2 : (NB >= 4 ? 4 : 8 / NB);  // lanes = NB * LV <= 16
   static constexpr int LE = NB >= 2 ? 1 : 2;
   Context* c;
@@ -770,17 +851,17 @@ struct Chunk {
     } else {
       coef = sys->pt_coef64.p; hf = sys->pt_face64.p;
     }
-    const size_t lds = (size_t)(pm->lds + pm->maxfr) * sizeof(C);
+    const size_t lds = (size_t)(pm->lds + pm->maxfr) * NBL * sizeof(C);
 #define SW_PT pm->np, Gp, s->mesh->nc, s->nbf, pm->lds, coef, pm->lp.p, pm->uptr.p, pm->dof.p, \
               pm->seg.p, pm->out.p, pm->fptr.p, s->pt_fmeta.p, pm->fpos.p, hf, \
               s->ref_face_dofs.p, s->ref_face.p, kco, mask, xin, yout, part, dp
     const int cb = pm->cb;
     if (cb == 256) {
-      if (epi) SUCEM_LAUNCH((k_sw_patch<R, ORDER, NFD, NB, 256, true>), Gp, 256, lds, st, SW_PT);
-      else SUCEM_LAUNCH((k_sw_patch<R, ORDER, NFD, NB, 256, false>), Gp, 256, lds, st, SW_PT);
+      if (epi) SUCEM_LAUNCH((k_sw_patch<R, ORDER, NFD, NB, NBL, 256, true>), Gp, 256, lds, st, SW_PT);
+      else SUCEM_LAUNCH((k_sw_patch<R, ORDER, NFD, NB, NBL, 256, false>), Gp, 256, lds, st, SW_PT);
     } else if constexpr (ORDER == 2) {
-      if (epi) SUCEM_LAUNCH((k_sw_patch<R, ORDER, NFD, NB, 192, true>), Gp, 192, lds, st, SW_PT);
-      else SUCEM_LAUNCH((k_sw_patch<R, ORDER, NFD, NB, 192, false>), Gp, 192, lds, st, SW_PT);
+      if (epi) SUCEM_LAUNCH((k_sw_patch<R, ORDER, NFD, NB, NBL, 192, true>), Gp, 192, lds, st, SW_PT);
+      else SUCEM_LAUNCH((k_sw_patch<R, ORDER, NFD, NB, NBL, 192, false>), Gp, 192, lds, st, SW_PT);
     }
 #undef SW_PT
     SUCEM_LAUNCH((k_sw_psum<R, NB>), Gq, TB, 0, st, (int)pm->npd, Gq, pm->qptr.p, pm->qdof.p,
@@ -796,9 +877,9 @@ struct Chunk {
                 s->pt_fmeta.p, pm64->fpos.p, sys->pt_face64.p, s->ref_face_dofs.p, \
                 s->ref_face.p, kco, mask, (const double2*)x64, ax, part64, dp
     if (pm64->cb == 256)
-      SUCEM_LAUNCH((k_sw_patch<double, ORDER, NFD, NB, 256, false>), Gp64, 256, lds, st, SW_PT64);
+      SUCEM_LAUNCH((k_sw_patch<double, ORDER, NFD, NB, 1, 256, false>), Gp64, 256, lds, st, SW_PT64);
     else if constexpr (ORDER == 2)
-      SUCEM_LAUNCH((k_sw_patch<double, ORDER, NFD, NB, 192, false>), Gp64, 192, lds, st, SW_PT64);
+      SUCEM_LAUNCH((k_sw_patch<double, ORDER, NFD, NB, 1, 192, false>), Gp64, 192, lds, st, SW_PT64);
 #undef SW_PT64
     SUCEM_LAUNCH((k_sw_psum<double, NB>), Gq, TB, 0, st, (int)pm64->npd, Gq, pm64->qptr.p,
                  pm64->qdof.p, (const double2*)part64, mask, ax);
@@ -823,11 +904,11 @@ struct Chunk {
   // one COCG iteration of every running system; k = slot of |r_j| in the ring
   void iteration(int k) {
     launch_patch(p, q, true);
-    SUCEM_LAUNCH((k_sw_alpha<NB>), 1, TB, 0, st, Gp, (const double2*)dp, (const int*)act, sc);
+    SUCEM_LAUNCH((k_sw_alpha<NB>), 1, 64 * NB, 0, st, Gp, (const double2*)dp, (const int*)act, sc);
     SUCEM_LAUNCH((k_sw_xr<R, NB>), G, TB, 0, st, n, G, (const double*)sc, x, r, (const C*)p,
                  (const C*)q);
     precondition();
-    SUCEM_LAUNCH((k_sw_beta<NB>), 1, TB, 0, st, Gz, (const double2*)dp, (const int*)act,
+    SUCEM_LAUNCH((k_sw_beta<NB>), 1, 64 * NB, 0, st, Gz, (const double2*)dp, (const int*)act,
                  (const int*)sel, sc, ring, k, 0);
     SUCEM_LAUNCH((k_sw_pz<R, NB>), G, TB, 0, st, n, G, (const double*)sc, (const C*)z, p);
   }
@@ -835,7 +916,7 @@ struct Chunk {
   // restart the selected systems from their (replaced) residual: z = M^-1 r, rho, p = z
   void restart() {
     precondition();
-    SUCEM_LAUNCH((k_sw_beta<NB>), 1, TB, 0, st, Gz, (const double2*)dp, (const int*)act,
+    SUCEM_LAUNCH((k_sw_beta<NB>), 1, 64 * NB, 0, st, Gz, (const double2*)dp, (const int*)act,
                  (const int*)sel, sc, ring, 0, 1);
     SUCEM_LAUNCH((k_sw_restart_p<R, NB>), G, TB, 0, st, n, G, (const int*)sel, (const C*)z, p);
   }
@@ -870,6 +951,9 @@ void sweep_chunk(Context* c, System* sys, int nb, const double* k0, const double
               SUCEM_ERR_UNSUPPORTED, "the batched sweep needs the patch operator on this space");
   k.pm = &s->pt[cb == 192 ? 1 : 2];
   k.pm64 = &s->pt[cb64 == 192 ? 1 : 2];
+  SUCEM_CHECK((int64_t)(k.pm->lds + k.pm->maxfr) * K::NBL * (int64_t)sizeof(C) <=
+                  (int64_t)s->pt_max_lds,
+              SUCEM_ERR_UNSUPPORTED, "the batched patch kernel needs more LDS than the device has");
   k.Gp = (int)((k.pm->np + 7) / 8 * 8);
   k.Gp64 = (int)((k.pm64->np + 7) / 8 * 8);
   k.Gq = op_grid(c, std::max<int64_t>(std::max(k.pm->npd, k.pm64->npd), 1), 16);
@@ -944,7 +1028,7 @@ void sweep_chunk(Context* c, System* sys, int nb, const double* k0, const double
   hipStream_t st = k.st;
 
   SUCEM_LAUNCH((k_sw_norm<NB>), k.G, TB, 0, st, n, k.G, (const double2*)k.b64, k.mask, k.dp);
-  SUCEM_LAUNCH((k_sw_sum<NB>), 1, TB, 0, st, k.G, (const double2*)k.dp, k.sums);
+  SUCEM_LAUNCH((k_sw_sum<NB>), 1, 64 * NB, 0, st, k.G, (const double2*)k.dp, k.sums);
   std::vector<double> bn = d2h_vec(c, st, k.sums, NB);
   for (auto& v : bn) v = std::sqrt(v);
   std::vector<int> act(NB, 0), sel(NB, 0);
@@ -1041,7 +1125,7 @@ void sweep_chunk(Context* c, System* sys, int nb, const double* k0, const double
       k.apply_hi();
       SUCEM_LAUNCH((k_sw_resid<R, NB>), k.G, TB, 0, st, n, k.G, (const int*)k.sel,
                    (const double2*)k.b64, (const double2*)k.ax, k.mask, k.r, k.dp);
-      SUCEM_LAUNCH((k_sw_sum<NB>), 1, TB, 0, st, k.G, (const double2*)k.dp, k.sums);
+      SUCEM_LAUNCH((k_sw_sum<NB>), 1, 64 * NB, 0, st, k.G, (const double2*)k.dp, k.sums);
       std::vector<double> tr = d2h_vec(c, st, k.sums, NB);
       bool restart_any = false, any_revert = false;
       std::vector<int> revert(NB, 0), keep(NB, 0);
@@ -1119,7 +1203,7 @@ void sweep_chunk(Context* c, System* sys, int nb, const double* k0, const double
     k.apply_hi();
     SUCEM_LAUNCH((k_sw_resid<R, NB>), k.G, TB, 0, st, n, k.G, (const int*)k.sel,
                  (const double2*)k.b64, (const double2*)k.ax, k.mask, k.r, k.dp);
-    SUCEM_LAUNCH((k_sw_sum<NB>), 1, TB, 0, st, k.G, (const double2*)k.dp, k.sums);
+    SUCEM_LAUNCH((k_sw_sum<NB>), 1, 64 * NB, 0, st, k.G, (const double2*)k.dp, k.sums);
     std::vector<double> tr = d2h_vec(c, st, k.sums, NB);
     for (int j = 0; j < nb; ++j)
       if (open[j]) rel[j] = std::sqrt(tr[j]) / bn[j];
