@@ -231,6 +231,18 @@ int sucem_rhs_point_sources(sucem_ctx* ctx, sucem_space* s, int64_t nsrc, const 
                             const double* val_h, const int64_t* rhs_h, int64_t nrhs,
                             double scale_re, double scale_im, double* b_d);
 
+/* Allocation-free variant of sucem_rhs_point_sources (same arguments, same result).  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory
+ * (serving loops, code that forbids hipMalloc).  At most 4096 sources per call
+ * (SUCEM_ERR_INVALID beyond); their positions, moments, indices and cells are staged in a
+ * fixed array of device memory that the library holds on each GPU until the process ends
+ * (~352 KB), so the call makes no hipMalloc, hipFree or pinned-host allocation.  Calls
+ * that overlap on one GPU (from several contexts or threads) race on that array. */
+int sucem_rhs_point_sources_noalloc(sucem_ctx* ctx, sucem_space* s, int64_t nsrc,
+                                    const double* pos_h, const double* val_h,
+                                    const int64_t* rhs_h, int64_t nrhs, double scale_re,
+                                    double scale_im, double* b_d);
+
 /* ---- solves (replaces LinalgSolvers.BiCGStabSolver/solve_sparse_system,
  *      Utilities/LinalgSolvers.py:239-248,308-322, on the driven system) ---- */
 void sucem_solve_opts_default(sucem_solve_opts* o);

@@ -84,6 +84,7 @@ int sucem_ctx_create(int device, void* hip_stream, sucem_ctx** out) {
     hipDeviceProp_t prop;
     SUCEM_HIP(hipGetDeviceProperties(&prop, device));
     c.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    noalloc_init(device);   // device address of the allocation-free staging array
     // Work is issued on exactly the caller's stream (NULL = the device's null
     // stream), so it is ordered with the caller's own copies and kernels.
     c.stream = (hipStream_t)hip_stream;
@@ -627,6 +628,25 @@ int sucem_rhs_point_sources(sucem_ctx* ctx, sucem_space* s, int64_t nsrc, const 
   rhs_point_sources(c, &s->s, nsrc, pos.p, val.p, rhs_d.p, cell.p, nrhs,
                     make_double2(scale_re, scale_im), (double2*)b_d);
   c->sync();
+  API_END
+}
+
+int sucem_rhs_point_sources_noalloc(sucem_ctx* ctx, sucem_space* s, int64_t nsrc,
+                                    const double* pos_h, const double* val_h,
+                                    const int64_t* rhs_h, int64_t nrhs, double scale_re,
+                                    double scale_im, double* b_d) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(s); NOT_NULL(b_d);
+  SUCEM_CHECK(nsrc == 0 || (pos_h && val_h), SUCEM_ERR_INVALID, "missing source arrays");
+  SUCEM_CHECK(nsrc >= 0 && nsrc <= kNoAllocSources, SUCEM_ERR_INVALID,
+              "the allocation-free point-source call takes at most 4096 sources");
+  for (int64_t i = 0; rhs_h && i < nsrc; ++i)
+    SUCEM_CHECK(rhs_h[i] >= 0 && rhs_h[i] < nrhs, SUCEM_ERR_INVALID,
+                "source rhs index out of range");
+  Context* c = &ctx->c;
+  c->use();
+  rhs_point_sources_noalloc(c, &s->s, nsrc, pos_h, val_h, rhs_h, nrhs,
+                            make_double2(scale_re, scale_im), (double2*)b_d);
   API_END
 }
 

@@ -198,6 +198,13 @@ void rhs_point_sources(Context* c, const Space* s, int64_t nsrc, const double* p
                        int64_t nrhs, double2 scale, double2* b_d);
 void basis_at_points(Context* c, const Space* s, int64_t n, const double* pts_d,
                      const int32_t* cell_d, double* vals_d);
+// allocation-free variant of the point-source RHS: staging in a fixed device array
+// (noalloc_init once per device with that device current, at context creation)
+constexpr int64_t kNoAllocSources = 4096;
+void noalloc_init(int device);
+void rhs_point_sources_noalloc(Context* c, const Space* s, int64_t nsrc, const double* pos_h,
+                               const double* val_h, const int64_t* rhs_h, int64_t nrhs,
+                               double2 scale, double2* b_d);
 
 // krylov.hip
 struct SolveOpts {

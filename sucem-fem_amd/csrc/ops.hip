@@ -5,6 +5,7 @@
 // the compact ABC rows, replacing the three CSR conversions and two sparse adds
 // of DrivenProblemABC.get_LHS_matrix (ProblemConfigurations/EMDrivenProblem.py:67-72).
 #include <cmath>
+#include <mutex>
 
 #include "internal.hpp"
 #include "kernels.hpp"
@@ -616,6 +617,105 @@ void basis_at_points(Context* c, const Space* s, int64_t n, const double* pts_d,
   if (n == 0) return;
   SUCEM_LAUNCH(k_basis_at, grid_for(n, 64), 64, 0, c->stream, n, pts_d, cell_d, m->xyz.p,
                m->cells.p, s->ref_basis.p, s->nloc, vals_d);
+}
+
+// ---- allocation-free point-source right-hand sides -----------------------------------
+// Staging of sucem_rhs_point_sources_noalloc lives in a fixed array of device memory (one
+// copy per GPU, held until the process ends) instead of per-call allocations.  In host
+// code the array's name only stands for the symbol: its device address on each GPU comes
+// from hipGetSymbolAddress (noalloc_init, called when a context is created on that GPU),
+// and only that address is handed to kernels and copies.  Every call on a GPU shares the
+// array: calls that overlap on one GPU race.
+namespace {
+
+constexpr int64_t kNA = kNoAllocSources;
+constexpr size_t kNaPos = 0;                                   // double[3 kNA]
+constexpr size_t kNaVal = kNaPos + 24 * kNA;                   // double2[3 kNA]
+constexpr size_t kNaRhs64 = kNaVal + 48 * kNA;                 // int64[kNA]
+constexpr size_t kNaRhs = kNaRhs64 + 8 * kNA;                  // int32[kNA]
+constexpr size_t kNaCell = kNaRhs + 4 * kNA;                   // int32[kNA]
+constexpr size_t kNaFlag = kNaCell + 4 * kNA;                  // int32: a point is outside
+constexpr size_t kNaBytes = kNaFlag + 256;
+static_assert(kNaVal % 256 == 0 && kNaRhs64 % 256 == 0 && kNaRhs % 256 == 0 &&
+                  kNaCell % 256 == 0 && kNaFlag % 256 == 0,
+              "staging sections are 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+std::mutex g_na_mu;
+constexpr int kNaMaxDevices = 64;
+unsigned char* g_na_ptr[kNaMaxDevices] = {};
+
+// rhs32 = rhs64 (0 without indices), cell = "not found", the outside flag cleared
+__global__ void k_na_prep(int64_t n, const int64_t* __restrict__ rhs64, int32_t* __restrict__ rhs,
+                          int32_t* __restrict__ cell, int32_t* __restrict__ outside) {
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t0 == 0) *outside = 0;
+  for (int64_t i = t0; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    rhs[i] = rhs64 ? (int32_t)rhs64[i] : 0;
+    cell[i] = 0x7fffffff;
+  }
+}
+
+__global__ void k_na_outside(int64_t n, const int32_t* __restrict__ cell,
+                             int32_t* __restrict__ outside) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (cell[i] == 0x7fffffff) atomicOr(outside, 1);
+}
+
+}  // namespace
+
+void noalloc_init(int device) {
+  std::lock_guard<std::mutex> hold(g_na_mu);
+  SUCEM_CHECK(device >= 0 && device < kNaMaxDevices, SUCEM_ERR_UNSUPPORTED,
+              "device index beyond the allocation-free staging table");
+  if (g_na_ptr[device]) return;
+  void* p = nullptr;
+  SUCEM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  SUCEM_CHECK(p != nullptr, SUCEM_ERR_HIP, "no device address for the staging array");
+  g_na_ptr[device] = (unsigned char*)p;
+}
+
+void rhs_point_sources_noalloc(Context* c, const Space* s, int64_t nsrc, const double* pos_h,
+                               const double* val_h, const int64_t* rhs_h, int64_t nrhs,
+                               double2 scale, double2* b_d) {
+  SUCEM_CHECK(nsrc >= 0 && nsrc <= kNA, SUCEM_ERR_INVALID,
+              "the allocation-free point-source call takes at most 4096 sources");
+  unsigned char* base;
+  {
+    std::lock_guard<std::mutex> hold(g_na_mu);
+    base = (c->device >= 0 && c->device < kNaMaxDevices) ? g_na_ptr[c->device] : nullptr;
+  }
+  SUCEM_CHECK(base != nullptr, SUCEM_ERR_INTERNAL,
+              "allocation-free staging not initialised on this device");
+  if (nsrc == 0 || nrhs == 0) return;
+  double* pos = (double*)(base + kNaPos);
+  double2* val = (double2*)(base + kNaVal);
+  int64_t* rhs64 = (int64_t*)(base + kNaRhs64);
+  int32_t* rhs = (int32_t*)(base + kNaRhs);
+  int32_t* cell = (int32_t*)(base + kNaCell);
+  int32_t* outside = (int32_t*)(base + kNaFlag);
+  SUCEM_HIP(hipMemcpyAsync(pos, pos_h, sizeof(double) * 3 * nsrc, hipMemcpyHostToDevice,
+                           c->stream));
+  SUCEM_HIP(hipMemcpyAsync(val, val_h, sizeof(double2) * 3 * nsrc, hipMemcpyHostToDevice,
+                           c->stream));
+  if (rhs_h)
+    SUCEM_HIP(hipMemcpyAsync(rhs64, rhs_h, sizeof(int64_t) * nsrc, hipMemcpyHostToDevice,
+                             c->stream));
+  SUCEM_LAUNCH(k_na_prep, grid_for(nsrc, 64), 64, 0, c->stream, nsrc,
+               rhs_h ? (const int64_t*)rhs64 : nullptr, rhs, cell, outside);
+  const Mesh* m = s->mesh;
+  SUCEM_LAUNCH(k_locate, grid_for(m->nc, TB), TB, 0, c->stream, m->nc, m->xyz.p, m->cells.p,
+               nsrc, (const double*)pos, 1e-12, cell);
+  SUCEM_LAUNCH(k_na_outside, grid_for(nsrc, 64), 64, 0, c->stream, nsrc, (const int32_t*)cell,
+               outside);
+  int32_t out_h = 0;
+  SUCEM_HIP(hipMemcpyAsync(&out_h, outside, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  c->sync();
+  SUCEM_CHECK(out_h == 0, SUCEM_ERR_OUTSIDE, "point source is outside the mesh");
+  rhs_point_sources(c, s, nsrc, pos, val, rhs, cell, nrhs, scale, b_d);
+  c->sync();
 }
 
 }  // namespace sucem

@@ -103,6 +103,8 @@ SIGNATURES = {
     "sucem_locate_points": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
     "sucem_basis_at_points": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P]),
     "sucem_rhs_point_sources": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _I64, _D, _D, _P]),
+    "sucem_rhs_point_sources_noalloc": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _I64, _D, _D,
+                                                       _P]),
     "sucem_solve_opts_default": (None, [_P]),
     "sucem_solve_driven": (ctypes.c_int, [_P, _P, _D, _P, _P, _P, _P, _P, _I64]),
     "sucem_solve_driven_batch": (ctypes.c_int, [_P, _P, _I64, _P, _P, _P, _P, _P]),
