@@ -101,6 +101,144 @@ def read_traffic(matrix_free, precision, cb):
     return None
 
 
+METRIC = "solves/s + p50/p99 latency at rel. residual 1e-8, FP32+IR vs plain FP64 GPU"
+
+
+def sweep_measure(n=40, order=1, nfreq=64, ndip=8, seed=0, precisions=(32, 64), width=8,
+                  use_graph=True, sequential_sample=0, tol=1e-8):
+    """The batched frequency sweep: an n^3-box cube (1 m, ABC, eps_r = 1), nfreq frequencies
+    in 0.3-1.5 GHz x ndip seeded unit dipoles, all systems solved by the batched sweep
+    (``width`` systems per group, hipGraph replay); per precision: seconds, solves/s,
+    iterations.  ``sequential_sample``: also time that many of the systems (spread over the
+    frequencies) with the single-system solver, for the per-solve comparison."""
+    import numpy as np
+    import torch
+    from sucemfem import _hip
+    from sucemfem.Utilities.Workloads import DeviceProblem, sweep_rhs, sweep_sources
+    prob = DeviceProblem(n, 1.0, order, _hip.MAT_S | _hip.MAT_T | _hip.MAT_S0)
+    freqs, pos, mom = sweep_sources(nfreq=nfreq, ndip=ndip, seed=seed)
+    B, k0s = sweep_rhs(prob, freqs, pos, mom)
+    nb, N = len(k0s), prob.ndofs
+    X = torch.zeros_like(B)
+    out = {"workload": "sweep %d^3 boxes, order %d, %d frequencies 0.3-1.5 GHz x %d seeded "
+                       "unit dipoles (seed %d)" % (n, order, nfreq, ndip, seed),
+           "ndofs": N, "systems": nb, "width": width, "graph": bool(use_graph)}
+    for prec in precisions:
+        prob.solve_sweep(k0s[:width], B[:width * N], X[:width * N], precision=prec, tol=tol,
+                         width=width, use_graph=use_graph)          # warm-up (first use)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = prob.solve_sweep(k0s, B, X, precision=prec, tol=tol, width=width,
+                              use_graph=use_graph)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        its = [q["iterations"] for q in st]
+        ok = all(q["status"] == 0 and q["rel_residual"] <= tol for q in st)
+        key = "fp32ir" if prec == 32 else "fp64"
+        out[key] = {"s": dt, "solves_per_s": nb / dt if ok else 0.0, "converged": ok,
+                    "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its)),
+                    "fp64_finish": int(sum(q["fp64_finish"] for q in st))}
+        log("sweep FP%d: %.2f s, %.1f solves/s, iterations mean %.0f max %d" % (
+            prec, dt, nb / dt, np.mean(its), max(its)))
+        if sequential_sample:
+            idx = np.linspace(0, nb - 1, sequential_sample).astype(int)
+            x = torch.zeros(N, dtype=torch.complex128, device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sts = [prob.solve(k0s[j], B.view(nb, N)[j].contiguous(), x, precision=prec, tol=tol)
+                   for j in idx]
+            torch.cuda.synchronize()
+            ds = time.perf_counter() - t0
+            out[key]["sequential_sample"] = {
+                "systems": len(idx), "solves_per_s": len(idx) / ds,
+                "iterations_mean": float(np.mean([q["iterations"] for q in sts]))}
+            log("  sequential sample of %d: %.1f solves/s" % (len(idx), len(idx) / ds))
+    if "fp32ir" in out and "fp64" in out and out["fp64"]["solves_per_s"]:
+        out["fp32ir_vs_fp64"] = out["fp32ir"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    return out
+
+
+def eigen_measure(n=(120, 60, 90), nev=20, sigma=1.1, seed=0, precisions=(32, 64), tol=1e-8,
+                  inner_tol=1e-9):
+    """The dielectric-block cavity: 1.0 x 0.5 x 0.75 m PEC box of n boxes, a seeded eps_r = 6
+    block (10-30 % of the volume), order 2, the nev lowest k^2 above sigma; per inner
+    precision: seconds, Lanczos steps, inner iterations, FP64 residuals."""
+    import numpy as np
+    import torch
+    from sucemfem.Utilities.Workloads import dielectric_cavity
+    t0 = time.perf_counter()
+    prob = dielectric_cavity(n=n, seed=seed)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    out = {"workload": "dielectric-block cavity %dx%dx%d boxes, order 2, eps_r 6 block "
+                       "(seed %d, %.3f of the volume), %d lowest k^2 above sigma %g"
+                       % (tuple(n) + (seed, prob.block_fraction, nev, sigma)),
+           "ndofs": prob.ndofs, "nnz": prob.nnz, "setup_s": setup, "tol": tol,
+           "inner_tol": inner_tol}
+    ks = {}
+    for prec in precisions:
+        t0 = time.perf_counter()
+        k2, resid, st = prob.eigs(sigma, nev, tol=tol, inner_tol=inner_tol, inner_precision=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        key = "fp32ir" if prec == 32 else "fp64"
+        ks[key] = k2
+        out[key] = {"s": dt, "solves_per_s": 1.0 / dt if st["status"] == 0 else 0.0,
+                    "status": st["status"], "lanczos_steps": st["lanczos_steps"],
+                    "inner_iterations": st["inner_iterations"],
+                    "inner_stagnated": st["inner_stagnated"],
+                    "inner_failures": st["inner_failures"],
+                    "max_ritz_residual": st["max_ritz_residual"],
+                    "max_mode_residual": float(np.max(resid)), "k2": [float(v) for v in k2]}
+        log("eigen FP%d inner: %.2f s, %d Lanczos steps, %d inner iterations, status %d" % (
+            prec, dt, st["lanczos_steps"], st["inner_iterations"], st["status"]))
+    if len(ks) == 2:
+        a, b = ks["fp32ir"], ks["fp64"]
+        out["max_rel_k2_diff_fp32ir_vs_fp64"] = float(np.max(np.abs(a - b) / np.abs(b)))
+        if out["fp64"]["s"]:
+            out["fp32ir_vs_fp64"] = out["fp64"]["s"] / out["fp32ir"]["s"]
+    return out
+
+
+def serve_measure(count=16, seed=0, n_range=(6, 48), slots_list=(4, 1), precisions=(32, 64)):
+    """The serving stream: a seeded Poisson trace (200 requests/s) of ``count`` mixed
+    requests (box sides U[0.2, 1.5] m, n ~ U{n_range} boxes per axis, order 1 or 2, 70 %
+    eigen with 8 modes / 30 % driven), replayed by the request loop (device-memory
+    admission, per-slot streams); per (precision, slots): requests/s and p50/p99 latency."""
+    from sucemfem.Serving import Server, device_executor, make_trace, summarize
+    trace = make_trace(count, seed=seed, n_range=n_range)
+    out = {"workload": "serving trace of %d requests (seed %d, Poisson 200/s, n in [%d, %d], "
+                       "70%% eigen 8 modes / 30%% driven, orders 1/2)"
+                       % (count, seed, n_range[0], n_range[1]),
+           "requests": [{"kind": r.kind, "n": r.n, "order": r.order} for r in trace]}
+    for prec in precisions:
+        ex = device_executor(precision=prec)
+        # warm the library on one small request (first-use set-up of the kernels)
+        Server(ex, slots=1).run(make_trace(1, seed=seed + 1, n_range=(4, 4)), realtime=False)
+        for slots in slots_list:
+            res, span = Server(ex, slots=slots).run(trace)
+            s = summarize(res, span)
+            key = "%s_slots%d" % ("fp32ir" if prec == 32 else "fp64", slots)
+            out[key] = s
+            log("serve FP%d, %d slots: %.2f req/s, p50 %.0f ms, p99 %.0f ms, done %d/%d" % (
+                prec, slots, s["solves_per_s"], s["latency_ms_p50"], s["latency_ms_p99"],
+                s["done"], s["requests"]))
+    for slots in slots_list:
+        a, b = out.get("fp32ir_slots%d" % slots), out.get("fp64_slots%d" % slots)
+        if a and b and b["solves_per_s"]:
+            out["fp32ir_vs_fp64_slots%d" % slots] = a["solves_per_s"] / b["solves_per_s"]
+    return out
+
+
+def workload_line(kind, a, result, value, unit, ms, dtype):
+    """One JSON line of a non-default workload (same top-level keys as the headline)."""
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": 1,
+            "warmup": 1, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+            "config": {"workload": result["workload"], "parallelism": "single GPU"},
+            kind: result}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +257,16 @@ def main():
                     help="auto (autotuned), assembled, cells (matrix-free per cell) or "
                          "patch128 / patch192 / patch256 (matrix-free per patch of cells), "
                          "for both precisions")
+    ap.add_argument("--workload", default="driven",
+                    choices=["driven", "sweep", "eigen", "serve"],
+                    help="driven (the headline line), sweep (64 frequencies x 8 dipoles, "
+                         "order 1, 40^3), eigen (dielectric-block cavity, 120x60x90 boxes, "
+                         "order 2, 20 modes), serve (mixed request trace)")
+    ap.add_argument("--eig-n", default="120,60,90", help="eigen workload boxes per axis")
+    ap.add_argument("--serve-count", type=int, default=16)
+    ap.add_argument("--serve-seed", type=int, default=0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="driven workload: skip the sweep / eigen / serving samples")
     ap.add_argument("--copy-probe-mb", type=int, default=256,
                     help="one plain device copy of this size after the SpMV timing: the "
                          "known-size kernel that scales HBM byte counters")
@@ -132,6 +280,27 @@ def main():
     import torch
     torch.cuda.set_device(0)
     from sucemfem.Utilities.Workloads import driven_cube
+
+    if a.workload == "sweep":
+        r = sweep_measure(sequential_sample=32)
+        v = r["fp32ir"]["solves_per_s"]
+        print(json.dumps(workload_line("sweep", a, r, v, "solves/s", 1e3 * r["fp32ir"]["s"],
+                                       "f32+f64-ir")), flush=True)
+        return
+    if a.workload == "eigen":
+        nn = tuple(int(v) for v in a.eig_n.split(","))
+        r = eigen_measure(n=nn)
+        v = r["fp32ir"]["solves_per_s"]
+        print(json.dumps(workload_line("eigen", a, r, v, "solves/s", 1e3 * r["fp32ir"]["s"],
+                                       "f32+f64-ir")), flush=True)
+        return
+    if a.workload == "serve":
+        r = serve_measure(count=a.serve_count, seed=a.serve_seed)
+        v = r["fp32ir_slots4"]["solves_per_s"]
+        print(json.dumps(workload_line("serving", a, r, v, "requests/s",
+                                       1e3 * r["fp32ir_slots4"]["makespan_s"], "f32+f64-ir")),
+              flush=True)
+        return
 
     t0 = time.perf_counter()
     prob = driven_cube(n=a.n, order=a.order)

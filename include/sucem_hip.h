@@ -41,6 +41,8 @@ enum {
 /* solver methods / preconditioners */
 #define SUCEM_METHOD_COCG 0
 #define SUCEM_METHOD_BICGSTAB 1
+/* restarted GMRES, FP64, right-preconditioned (LinalgSolvers.GMRESSolver, LinalgSolvers.py:251-260) */
+#define SUCEM_METHOD_GMRES 2
 #define SUCEM_PRECOND_NONE 0
 #define SUCEM_PRECOND_JACOBI 1
 /* Jacobi plus the gradient-space correction G diag(G^T A G)^-1 G^T (driven / shifted
@@ -66,6 +68,8 @@ typedef struct sucem_solve_opts {
   int eliminate_pec;  /* 1: restrict to non-PEC DOFs (rows/cols of PEC DOFs removed) */
   double tol;         /* relative residual ||b - A x|| / ||b|| (FP64) */
   double ru_delta;    /* FP32: reliable update when the residual fell by this factor */
+  int gmres_restart;  /* GMRES: Arnoldi steps per cycle (SciPy's default 20) */
+  int pad_;
 } sucem_solve_opts;
 
 typedef struct sucem_eig_opts {

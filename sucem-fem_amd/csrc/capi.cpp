@@ -661,6 +661,8 @@ void sucem_solve_opts_default(sucem_solve_opts* o) {
   o->eliminate_pec = 0;
   o->tol = 1e-8;
   o->ru_delta = 0.01;
+  o->gmres_restart = 20;
+  o->pad_ = 0;
 }
 
 static SolveOpts convert(const sucem_solve_opts* o) {
@@ -674,6 +676,7 @@ static SolveOpts convert(const sucem_solve_opts* o) {
     s.eliminate_pec = o->eliminate_pec;
     s.tol = o->tol;
     s.ru_delta = o->ru_delta;
+    s.restart = o->gmres_restart;
   }
   SUCEM_CHECK(s.tol > 0 && s.ru_delta > 0 && s.ru_delta < 1, SUCEM_ERR_INVALID,
               "tol must be > 0 and ru_delta in (0, 1)");

@@ -217,6 +217,7 @@ struct SolveOpts {
   double ru_delta = 0.01;
   int eliminate_pec = 0;
   int warm = 0;          // x holds the initial guess (otherwise x0 = 0)
+  int restart = 20;      // GMRES restart length
 };
 struct SolveStats {
   int status = 0, iterations = 0, reliable_updates = 0;

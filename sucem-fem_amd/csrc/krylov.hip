@@ -20,6 +20,7 @@
 // `check_every` iterations.
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 
@@ -819,9 +820,272 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// ---- GMRES (LinalgSolvers.GMRESSolver, LinalgSolvers.py:251-260) ----------------------
+// Restarted GMRES(m) in FP64 with right preconditioning, so the quantity it minimises and
+// monitors is the true residual ||b - A x||: Arnoldi with classical Gram-Schmidt applied
+// twice (Hermitian inner products on the device, block partials summed on the host in
+// block order), Givens rotations and the small triangular solve on the host, x updated by
+// M^-1 (V y) at the end of each cycle and the residual recomputed from scratch before the
+// next one.
+
+// part[i G + lb] = sum_r conj(V_i[r]) w[r], i < m; basis vectors taken 8 at a time so w is
+// read once per 8 of them
+__global__ __launch_bounds__(TB) void k_zmultidot(int n, int G, int m, const double2* V,
+                                                  const double2* w, double2* part) {
+  Range rr = block_rows(n, G);
+  for (int i0 = 0; i0 < m; i0 += 8) {
+    const int cnt = min(8, m - i0);
+    double2 acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = make_double2(0.0, 0.0);
+    for (int r = rr.start + threadIdx.x; r < rr.end; r += TB) {
+      const double2 wr = w[r];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k < cnt) {
+          const double2 v = V[(int64_t)(i0 + k) * n + r];
+          acc[k].x += v.x * wr.x + v.y * wr.y;
+          acc[k].y += v.x * wr.y - v.y * wr.x;
+        }
+      }
+    }
+    for (int k = 0; k < cnt; ++k) {
+      const double2 t = block_sum<TB>(acc[k]);
+      if (threadIdx.x == 0) part[(int64_t)(i0 + k) * G + xcd_block(G)] = t;
+    }
+  }
+}
+
+// w -= sum_i h_i V_i (i < m)
+__global__ void k_zsubtract(int n, int G, int m, const double2* h, const double2* V, double2* w) {
+  Range rr = block_rows(n, G);
+  for (int r = rr.start + threadIdx.x; r < rr.end; r += TB) {
+    double2 a = w[r];
+    for (int i = 0; i < m; ++i) {
+      const double2 v = V[(int64_t)i * n + r], hi = h[i];
+      a.x -= hi.x * v.x - hi.y * v.y;
+      a.y -= hi.x * v.y + hi.y * v.x;
+    }
+    w[r] = a;
+  }
+}
+
+// u = sum_i y_i V_i (i < m)
+__global__ void k_zcombine(int n, int G, int m, const double2* y, const double2* V, double2* u) {
+  Range rr = block_rows(n, G);
+  for (int r = rr.start + threadIdx.x; r < rr.end; r += TB) {
+    double2 a = make_double2(0.0, 0.0);
+    for (int i = 0; i < m; ++i) {
+      const double2 v = V[(int64_t)i * n + r], yi = y[i];
+      a.x += yi.x * v.x - yi.y * v.y;
+      a.y += yi.x * v.y + yi.y * v.x;
+    }
+    u[r] = a;
+  }
+}
+
+__global__ void k_zscale(int n, int G, double s, const double2* in, double2* out) {
+  Range rr = block_rows(n, G);
+  for (int r = rr.start + threadIdx.x; r < rr.end; r += TB)
+    out[r] = make_double2(s * in[r].x, s * in[r].y);
+}
+
+// r = b - ax (masked rows 0)
+__global__ void k_zresid(int n, int G, const double2* b, const double2* ax, const uint8_t* mask,
+                         double2* r) {
+  Range rr = block_rows(n, G);
+  for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
+    double2 v = make_double2(b[i].x - ax[i].x, b[i].y - ax[i].y);
+    if (mask && mask[i]) v = make_double2(0.0, 0.0);
+    r[i] = v;
+  }
+}
+
+// out = dinv .* in (Jacobi), in place allowed
+__global__ void k_zjacobi(int n, int G, const double2* dinv, const double2* in, double2* out) {
+  Range rr = block_rows(n, G);
+  for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) out[i] = cmul(dinv[i], in[i]);
+}
+
+__global__ void k_zadd(int n, int G, const double2* u, double2* x) {
+  Range rr = block_rows(n, G);
+  for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
+    x[i].x += u[i].x;
+    x[i].y += u[i].y;
+  }
+}
+
+void gmres(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats& st,
+           double* hist_h, int64_t hist_cap) {
+  using cd = std::complex<double>;
+  Context* c = op.c;
+  const int n = (int)op.n;
+  const int G = op_grid(c, n, 8);
+  const int m = std::max(1, o.restart);
+  const bool aux = o.precond == SUCEM_PRECOND_AUXILIARY;
+  const bool grad = o.precond == SUCEM_PRECOND_GRADIENT || aux;
+  const bool jac = o.precond == SUCEM_PRECOND_JACOBI;
+  auto t0 = std::chrono::steady_clock::now();
+  st = SolveStats();
+  // workspace slots 21..31 (the Krylov solver's are 0..20)
+  double2* V = ws_take<double2>(c, 21, (int64_t)(m + 1) * n);
+  double2* w = ws_take<double2>(c, 22, n);
+  double2* z = ws_take<double2>(c, 23, n);
+  double2* ax = ws_take<double2>(c, 24, n);
+  double2* dinv = ws_take<double2>(c, 25, (jac || grad) ? n : 0);
+  double2* part = ws_take<double2>(c, 26, (int64_t)(m + 1) * G);
+  double2* hd = ws_take<double2>(c, 27, m + 1);
+  const int64_t nl = grad ? op.nlag() : 0;
+  const int64_t nn = aux ? op.nnod() : 0;
+  double2* wg = ws_take<double2>(c, 28, nl);
+  double2* winv = ws_take<double2>(c, 29, nl);
+  double2* ug = ws_take<double2>(c, 30, nn);
+  double2* vinv = ws_take<double2>(c, 31, nn);
+  SUCEM_CHECK(!grad || nl > 0, SUCEM_ERR_UNSUPPORTED,
+              "the gradient and auxiliary-space preconditioners need an assembled Nedelec "
+              "system");
+  if (jac || grad) op.dinv_lo(dinv);
+  op.aux = aux;
+  if (grad) op.winv_lo(winv);
+  if (aux) op.vinv_lo(vinv);
+  // z = M^-1 v
+  auto precondition = [&](const double2* v, double2* out) {
+    if (grad) {
+      RedSite none{};
+      op.precond_lo(winv, dinv, v, wg, out, G, none, aux ? vinv : nullptr, ug);
+    } else if (jac) {
+      SUCEM_LAUNCH(k_zjacobi, G, TB, 0, c->stream, n, G, (const double2*)dinv, v, out);
+    } else {
+      SUCEM_HIP(hipMemcpyAsync(out, v, sizeof(double2) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+  };
+  std::vector<double2> hp;
+  auto dots = [&](const double2* base, int cnt, const double2* v) {
+    SUCEM_LAUNCH(k_zmultidot, G, TB, 0, c->stream, n, G, cnt, base, v, part);
+    hp.resize((size_t)cnt * G);
+    SUCEM_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double2) * cnt * G, hipMemcpyDeviceToHost,
+                             c->stream));
+    c->sync();
+    std::vector<cd> out(cnt, cd(0, 0));
+    for (int i = 0; i < cnt; ++i)
+      for (int g = 0; g < G; ++g) out[i] += cd(hp[(size_t)i * G + g].x, hp[(size_t)i * G + g].y);
+    return out;
+  };
+  auto upload = [&](const std::vector<cd>& v) {
+    std::vector<double2> t(v.size());
+    for (size_t i = 0; i < v.size(); ++i) t[i] = make_double2(v[i].real(), v[i].imag());
+    SUCEM_HIP(hipMemcpyAsync(hd, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice,
+                             c->stream));
+  };
+  // ||b|| over the unmasked rows
+  SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, ax);
+  SUCEM_LAUNCH(k_zresid, G, TB, 0, c->stream, n, G, b, (const double2*)ax, op.mask, w);
+  const double bnorm = std::sqrt(std::abs(dots(w, 1, w)[0]));
+  if (!o.warm || bnorm == 0.0)
+    SUCEM_LAUNCH((k_zero_c<double2>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, x);
+  if (bnorm == 0.0) {
+    st.status = 0;
+    c->sync();
+    return;
+  }
+  int it = 0;
+  double rel = 1.0;
+  st.status = o.max_iter;
+  std::vector<cd> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m);
+  auto h = [&](int i, int j) -> cd& { return H[(size_t)i * m + j]; };
+  while (true) {
+    // true residual of the current x
+    op.apply_hi(x, ax);
+    st.spmv_hi += 1;
+    SUCEM_LAUNCH(k_zresid, G, TB, 0, c->stream, n, G, b, (const double2*)ax, op.mask, w);
+    const double beta = std::sqrt(std::abs(dots(w, 1, w)[0]));
+    rel = beta / bnorm;
+    if (!std::isfinite(rel)) { st.status = -1; break; }
+    if (rel <= o.tol) { st.status = 0; break; }
+    if (it >= o.max_iter) break;
+    SUCEM_LAUNCH(k_zscale, G, TB, 0, c->stream, n, G, 1.0 / beta, (const double2*)w, V);
+    std::fill(H.begin(), H.end(), cd(0, 0));
+    std::fill(g.begin(), g.end(), cd(0, 0));
+    g[0] = beta;
+    int jend = 0;
+    bool breakdown = false;
+    for (int j = 0; j < m && it < o.max_iter; ++j) {
+      double2* vj = V + (int64_t)j * n;
+      precondition(vj, z);
+      op.apply_hi(z, w);
+      st.spmv_hi += 1;
+      // classical Gram-Schmidt, twice
+      for (int pass = 0; pass < 2; ++pass) {
+        std::vector<cd> hh = dots(V, j + 1, w);
+        for (int i = 0; i <= j; ++i) h(i, j) += hh[i];
+        upload(hh);
+        SUCEM_LAUNCH(k_zsubtract, G, TB, 0, c->stream, n, G, j + 1, (const double2*)hd,
+                     (const double2*)V, w);
+      }
+      const double hn = std::sqrt(std::abs(dots(w, 1, w)[0]));
+      h(j + 1, j) = hn;
+      // rotations so far, then a new one zeroing h(j+1, j)
+      for (int i = 0; i < j; ++i) {
+        const cd a = h(i, j), bb = h(i + 1, j);
+        h(i, j) = cs[i] * a + sn[i] * bb;
+        h(i + 1, j) = -std::conj(sn[i]) * a + cs[i] * bb;
+      }
+      const cd a = h(j, j), bb = h(j + 1, j);
+      const double r = std::sqrt(std::norm(a) + std::norm(bb));
+      if (r == 0.0) {
+        cs[j] = 1.0;
+        sn[j] = 0.0;
+      } else if (std::abs(a) == 0.0) {
+        cs[j] = 0.0;
+        sn[j] = std::conj(bb) / std::abs(bb);
+      } else {
+        cs[j] = std::abs(a) / r;
+        sn[j] = (a / std::abs(a)) * std::conj(bb) / r;
+      }
+      h(j, j) = cs[j] * a + sn[j] * bb;
+      h(j + 1, j) = 0.0;
+      g[j + 1] = -std::conj(sn[j]) * g[j];
+      g[j] = cs[j] * g[j];
+      ++it;
+      jend = j + 1;
+      const double est = std::abs(g[j + 1]) / bnorm;
+      if (hist_h && it - 1 < hist_cap) hist_h[it - 1] = est;
+      if (hn == 0.0 || !std::isfinite(hn)) { breakdown = true; break; }
+      if (est <= o.tol) break;
+      SUCEM_LAUNCH(k_zscale, G, TB, 0, c->stream, n, G, 1.0 / hn, (const double2*)w,
+                   V + (int64_t)(j + 1) * n);
+    }
+    // y = H^-1 g (upper triangular), x += M^-1 (V y)
+    std::vector<cd> y(jend);
+    for (int i = jend - 1; i >= 0; --i) {
+      cd s = g[i];
+      for (int k = i + 1; k < jend; ++k) s -= h(i, k) * y[k];
+      y[i] = h(i, i) == cd(0, 0) ? cd(0, 0) : s / h(i, i);
+    }
+    upload(y);
+    SUCEM_LAUNCH(k_zcombine, G, TB, 0, c->stream, n, G, jend, (const double2*)hd,
+                 (const double2*)V, w);
+    precondition(w, z);
+    SUCEM_LAUNCH(k_zadd, G, TB, 0, c->stream, n, G, (const double2*)z, x);
+    st.reliable_updates += 1;   // cycles
+    (void)breakdown;
+  }
+  c->sync();
+  st.iterations = it;
+  st.rel_residual = rel;
+  if (st.status > 0) st.status = it;
+  st.elapsed_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 void check_opts(const SolveOpts& o) {
-  SUCEM_CHECK(o.method == SUCEM_METHOD_COCG || o.method == SUCEM_METHOD_BICGSTAB,
-              SUCEM_ERR_UNSUPPORTED, "method must be COCG or BiCGStab");
+  SUCEM_CHECK(o.method == SUCEM_METHOD_COCG || o.method == SUCEM_METHOD_BICGSTAB ||
+                  o.method == SUCEM_METHOD_GMRES,
+              SUCEM_ERR_UNSUPPORTED, "method must be COCG, BiCGStab or GMRES");
+  SUCEM_CHECK(o.method != SUCEM_METHOD_GMRES || o.precision == 64, SUCEM_ERR_UNSUPPORTED,
+              "GMRES runs in FP64 (precision 64)");
+  SUCEM_CHECK(o.method != SUCEM_METHOD_GMRES || (o.restart >= 1 && o.restart <= 1000),
+              SUCEM_ERR_INVALID, "GMRES restart length must be in [1, 1000]");
   SUCEM_CHECK(o.precision == 32 || o.precision == 64, SUCEM_ERR_INVALID,
               "precision must be 32 or 64");
   SUCEM_CHECK(o.max_iter > 0 && o.check_every > 0, SUCEM_ERR_INVALID, "bad iteration limits");
@@ -845,6 +1109,16 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
     op.mask = o.eliminate_pec ? s->pec.p : nullptr;
     op.fhi = sys_cfg<double>(c, sys);
   };
+  if (o.method == SUCEM_METHOD_GMRES) {
+    SysOp<double> op;
+    fill(op);
+    op.flo = op.fhi;
+    op.sys = sys;
+    op.k = k;
+    op.identity_prec = o.precond == SUCEM_PRECOND_NONE;
+    gmres(op, b, x, o, st, hist_h, hist_cap);
+    return;
+  }
   if (o.precision == 32) {
     SysOp<float> op;
     fill(op);
@@ -899,6 +1173,16 @@ void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const Sol
     op.mask = nullptr;
     op.flo = op.fhi = csr_cfg(c, A);
   };
+  if (o.method == SUCEM_METHOD_GMRES) {
+    SUCEM_CHECK(o.precond == SUCEM_PRECOND_NONE || o.precond == SUCEM_PRECOND_JACOBI,
+                SUCEM_ERR_UNSUPPORTED, "GMRES on a generic matrix: no or Jacobi preconditioning");
+    CsrOp<double> op;
+    fill(op);
+    op.A = A;
+    op.identity_prec = o.precond == SUCEM_PRECOND_NONE;
+    gmres(op, b, x, o, st, hist_h, hist_cap);
+    return;
+  }
   if (o.precision == 32) {
     SUCEM_CHECK(A->val32.p, SUCEM_ERR_INVALID, "matrix has no FP32 copy");
     CsrOp<float> op;

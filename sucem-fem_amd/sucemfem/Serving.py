@@ -105,13 +105,36 @@ def footprint_bytes(req):
     return int(max(build, resident) + solver + (64 << 20))
 
 
-class Server(object):
-    """Admission + slots.  ``executor(request) -> (value, stats)`` runs one request."""
+def device_free_bytes(device=None):
+    """Free device memory (hipMemGetInfo through the library), or None without a device."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+    except ImportError:
+        return None
+    import ctypes
+    from . import _hip
+    from .runtime import get_context
+    ctx = get_context(device)
+    free, total = ctypes.c_int64(), ctypes.c_int64()
+    _hip.call("sucem_ctx_mem_info", ctx.handle, ctypes.byref(free), ctypes.byref(total))
+    return int(free.value)
 
-    def __init__(self, executor, slots=4, hbm_budget=None, footprint=footprint_bytes):
+
+class Server(object):
+    """Admission + slots.  ``executor(request) -> (value, stats)`` runs one request.
+
+    ``hbm_budget``: bytes the admitted requests' footprints may sum to; ``"device"`` (the
+    default) takes ``reserve`` times the device's free memory when ``run`` starts (no
+    budget when there is no device), None admits by slots only."""
+
+    def __init__(self, executor, slots=4, hbm_budget="device", footprint=footprint_bytes,
+                 reserve=0.9):
         self.executor = executor
         self.slots = int(slots)
         self.hbm_budget = hbm_budget
+        self.reserve = float(reserve)
         self.footprint = footprint
         self._lock = threading.Condition()
         self._in_use = 0
@@ -122,8 +145,16 @@ class Server(object):
         with self._lock:
             self._cancelled.add(request_id)
 
+    def budget(self):
+        """The admission budget in bytes (None: none)."""
+        if self.hbm_budget == "device":
+            free = device_free_bytes()
+            return None if free is None else int(self.reserve * free)
+        return self.hbm_budget
+
     def run(self, trace, realtime=True, thread_init=None):
         """Replay ``trace``; returns the results in request-id order and the makespan."""
+        budget = self.budget()
         results = {}
         pending = []                                   # (arrival, id, request) heap
         for r in trace:
@@ -169,13 +200,13 @@ class Server(object):
                     heapq.heappop(pending)
                     results[rid] = Result(rid, req.kind, 'cancelled', arrival)
                     continue
-                if self.hbm_budget is not None and need > self.hbm_budget:
+                if budget is not None and need > budget:
                     heapq.heappop(pending)
                     results[rid] = Result(rid, req.kind, 'rejected', arrival,
-                                          error="needs %d bytes, budget %d" % (need, self.hbm_budget))
+                                          error="needs %d bytes, budget %d" % (need, budget))
                     continue
                 while self._running >= self.slots or (
-                        self.hbm_budget is not None and self._in_use + need > self.hbm_budget):
+                        budget is not None and self._in_use + need > budget):
                     self._lock.wait(0.05)
                 heapq.heappop(pending)
                 self._in_use += need
@@ -205,13 +236,18 @@ def summarize(results, makespan):
 # ---------------------------------------------------------------------------------------
 # device executor
 # ---------------------------------------------------------------------------------------
-def device_executor(tol=1e-8, max_iter=50000, precision=64):
+def device_executor(tol=1e-8, max_iter=50000, precision=64, eig_tol=1e-8, eig_inner_tol=1e-9,
+                    eig_precision=None):
     """Executor running each request on the device on the calling thread's own stream.
     ``precision``: 64 (FP64 throughout) or 32 (FP32 operator and Krylov vectors with FP64
-    residual replacement) for the driven solves; the eigen solver's inner solves stay FP64
-    (the Lanczos recursion needs inner solves accurate to 1e-11)."""
+    residual replacement) for the driven solves and, unless ``eig_precision`` says
+    otherwise, for the eigen solver's inner solves.  The eigen requests stop at a relative
+    Ritz residual of ``eig_tol`` with inner solves to ``eig_inner_tol``: the returned k^2
+    are FP64 Rayleigh quotients, whose error is of the order of the square of the modes'
+    residual."""
     if precision not in (32, 64):
         raise ValueError("precision must be 32 or 64")
+    eig_precision = precision if eig_precision is None else int(eig_precision)
     import torch
     from .Utilities.Workloads import DeviceProblem
     from . import _hip
@@ -230,7 +266,9 @@ def device_executor(tol=1e-8, max_iter=50000, precision=64):
             context = get_context()
             if req.kind == 'eigen':
                 prob = DeviceProblem(req.n, req.dims, req.order, _hip.MAT_S | _hip.MAT_T, ctx=context)
-                k2, resid, st = prob.eigs(req.sigma, req.nev)
+                k2, resid, st = prob.eigs(req.sigma, req.nev, tol=eig_tol,
+                                          inner_tol=eig_inner_tol,
+                                          inner_precision=eig_precision)
                 stream.synchronize()
                 return k2, dict(st, ndofs=prob.ndofs)
             prob = DeviceProblem(req.n, req.dims, req.order,

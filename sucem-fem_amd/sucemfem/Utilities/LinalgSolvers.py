@@ -35,6 +35,11 @@ def _device_opts(method, precond, tol, maxiter, precision, check_every):
     return o
 
 
+def _with_restart(o, restart):
+    o.gmres_restart = int(restart)
+    return o
+
+
 class DeviceCSR(object):
     """A complex CSR matrix uploaded to the device."""
 
@@ -98,7 +103,7 @@ class SystemSolverBase(object):
             self._precond = _hip.PRECOND_AUXILIARY
         elif M_type.lower() == 'ilu':
             driven = getattr(self._A, "device_operator", None) is not None
-            if driven and self.method == _hip.METHOD_COCG:
+            if driven and self.method in (_hip.METHOD_COCG, _hip.METHOD_GMRES):
                 warnings.warn("incomplete LU is not available on the device; using the "
                               "auxiliary-space preconditioner")
                 self._precond = _hip.PRECOND_AUXILIARY
@@ -120,6 +125,8 @@ class SystemSolverBase(object):
         maxiter = self.maxiter or 10 * n
         opts = _device_opts(self.method, self._precond, self.tol, maxiter, self.precision,
                             self.check_every)
+        if self.method == _hip.METHOD_GMRES:
+            opts = _with_restart(opts, self.restart)
         stats = _hip.SolveStats()
         hist = np.zeros(maxiter, np.float64)
         op = getattr(A, "device_operator", None)
@@ -211,11 +218,14 @@ class COCGSolver(SystemSolverBase):
 
 
 class GMRESSolver(SystemSolverBase):
-    """GMRES (LinalgSolvers.py:251-260) has no device implementation yet."""
+    """Restarted GMRES (LinalgSolvers.py:251-260; SciPy's restart length 20), FP64 with
+    right preconditioning, so the residual it monitors is the true ||b - A x||."""
+    method = _hip.METHOD_GMRES
+    restart = 20
 
-    def _call_solver(self):
-        raise NotImplementedError("GMRES is not implemented on the device; use BiCGStabSolver "
-                                  "or COCGSolver")
+    def __init__(self, A, preconditioner_type=None, restart=20, **kw):
+        self.restart = int(restart)
+        SystemSolverBase.__init__(self, A, preconditioner_type, **kw)
 
 
 class UMFPACKSolver(SystemSolverBase):

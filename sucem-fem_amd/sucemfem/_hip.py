@@ -15,7 +15,7 @@ SUCEM_OK = 0
 ERR_INVALID, ERR_HIP, ERR_OOM, ERR_OUTSIDE, ERR_MATERIAL, ERR_UNSUPPORTED, ERR_INTERNAL = (
     1, 2, 3, 4, 5, 6, 7)
 MAT_S, MAT_T, MAT_S0 = 1, 2, 4
-METHOD_COCG, METHOD_BICGSTAB = 0, 1
+METHOD_COCG, METHOD_BICGSTAB, METHOD_GMRES = 0, 1, 2
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_GRADIENT, PRECOND_AUXILIARY = 0, 1, 2, 3
 
 
@@ -23,7 +23,8 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("precision", ctypes.c_int), ("precond", ctypes.c_int),
                 ("max_iter", ctypes.c_int), ("check_every", ctypes.c_int),
                 ("eliminate_pec", ctypes.c_int), ("tol", ctypes.c_double),
-                ("ru_delta", ctypes.c_double)]
+                ("ru_delta", ctypes.c_double), ("gmres_restart", ctypes.c_int),
+                ("pad_", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):

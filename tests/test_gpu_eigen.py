@@ -126,3 +126,22 @@ def test_homogeneous_eps_scales_device_k2():
     ke, _, se = pe.eigs(1.1 / 2.5, 8)
     assert s1["status"] == 0 and se["status"] == 0
     assert np.allclose(ke * 2.5, k1, rtol=1e-9, atol=0), (ke * 2.5, k1)
+
+
+@pytest.mark.parametrize("inner_precision", [64, 32])
+def test_relaxed_tolerances_keep_eigenvalues_within_1e8(inner_precision):
+    """Lanczos stopped at a Ritz residual of 1e-8 with inner solves to 1e-9 (the serving
+    settings): the returned k^2, FP64 Rayleigh quotients of the Ritz vectors, still match
+    the oracle's pencil to 1e-8."""
+    import torch
+    torch.cuda.set_device(0)
+    from sucemfem.Utilities.Workloads import dielectric_cavity
+    n = (8, 4, 6)
+    print("relaxed tolerances, inner FP%d" % inner_precision, flush=True)
+    prob = dielectric_cavity(n=n, dims=DIMS, order=2, eps_block=6.0, seed=0)
+    k2, resid, st = prob.eigs(1.1, 20, tol=1e-8, inner_tol=1e-9, inner_precision=inner_precision)
+    assert st["status"] == 0 and st["inner_failures"] == 0, st
+    co, ce = O.box_mesh(DIMS, n)
+    eps = np.where(prob.region == 1, 6.0, 1.0)
+    ref, _, _ = O.cavity_eigs(O.Space(co, ce, 2), 20, 1.1, eps_r=eps)
+    assert np.all(np.abs(k2 - ref) / ref <= 1e-8), (np.abs(k2 - ref) / ref).max()

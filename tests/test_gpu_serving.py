@@ -23,12 +23,18 @@ def test_requests_in_the_mix_match_alone():
             trace.append(Request(i, kind, d, n, order, 0.0, nev=3, sigma=0.5 * lo))
         else:
             trace.append(Request(i, kind, d, n, order, 0.0, freq=3e8 / (10 * max(d) / n)))
-    ex = device_executor(tol=1e-8, max_iter=20000)
-    results, makespan = Server(ex, slots=3).run(trace, realtime=False)
-    s = summarize(results, makespan)
-    assert s["done"] == len(trace), [r.error for r in results if r.status != "done"]
-    alone = Server(ex, slots=1)
-    for r, req in zip(results, trace):
-        (ra,), _ = alone.run([req], realtime=False)
-        assert ra.status == "done", ra.error
-        assert np.array_equal(np.asarray(r.value), np.asarray(ra.value)), req
+    # one large request: the 40^3-box order-2 cube (2,480,240 DOFs), driven
+    trace.append(Request(len(trace), "driven", (1.0, 1.0, 1.0), 40, 2, 0.0, freq=1e9))
+    for precision in (64, 32):
+        # FP32: driven solves and the eigen requests' inner solves in FP32 with FP64
+        # refinement (the large request stays in the FP64 mix only)
+        reqs = trace if precision == 64 else trace[:-1]
+        ex = device_executor(tol=1e-8, max_iter=20000, precision=precision)
+        results, makespan = Server(ex, slots=3).run(reqs, realtime=False)
+        s = summarize(results, makespan)
+        assert s["done"] == len(reqs), [r.error for r in results if r.status != "done"]
+        alone = Server(ex, slots=1)
+        for r, req in zip(results, reqs):
+            (ra,), _ = alone.run([req], realtime=False)
+            assert ra.status == "done", ra.error
+            assert np.array_equal(np.asarray(r.value), np.asarray(ra.value)), (precision, req)
