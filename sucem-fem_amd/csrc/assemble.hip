@@ -11,6 +11,7 @@
 // The gather sums contributions in cell order, so the result is deterministic.
 #include "internal.hpp"
 #include "geom.hpp"
+#include "kernels.hpp"
 
 namespace sucem {
 namespace {
@@ -155,12 +156,8 @@ __global__ void k_export_s0(int64_t n, const int32_t* rp, const int32_t* col, co
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
        r += (int64_t)gridDim.x * blockDim.x) {
     for (int j = s0p[r]; j < s0p[r + 1]; ++j) {
-      int target = s0c[j], lo = rp[r], hi = rp[r + 1] - 1;
-      while (lo <= hi) {
-        int mid = (lo + hi) >> 1;
-        if (col[mid] == target) { out[mid] = s0v[j]; break; }
-        if (col[mid] < target) lo = mid + 1; else hi = mid - 1;
-      }
+      const int pos = find_col(col, rp[r], rp[r + 1], s0c[j]);
+      if (pos >= 0) out[pos] = s0v[j];
     }
   }
 }
@@ -175,12 +172,8 @@ __global__ void k_export_lhs(int64_t n, const int32_t* rp, const int32_t* col, c
       out[j] = make_double2(st[j].x - k0 * k0 * st[j].y, 0.0);
     if (!s0p) continue;
     for (int j = s0p[r]; j < s0p[r + 1]; ++j) {
-      int target = s0c[j], lo = rp[r], hi = rp[r + 1] - 1;
-      while (lo <= hi) {
-        int mid = (lo + hi) >> 1;
-        if (col[mid] == target) { out[mid].y = k0 * s0v[j]; break; }
-        if (col[mid] < target) lo = mid + 1; else hi = mid - 1;
-      }
+      const int pos = find_col(col, rp[r], rp[r + 1], s0c[j]);
+      if (pos >= 0) out[pos].y = k0 * s0v[j];
     }
   }
 }

@@ -69,17 +69,6 @@ __global__ void k_grad_vals(int64_t nnz, const int32_t* seg, const uint32_t* per
 __device__ __forceinline__ int pk_col(uint32_t p) { return (int)(p >> 4); }
 __device__ __forceinline__ int pk_num(uint32_t p) { return ((int)(p << 28)) >> 28; }  // signed
 
-__device__ __forceinline__ int find_col(const int32_t* col, int lo, int hi, int target) {
-  hi -= 1;
-  while (lo <= hi) {
-    const int mid = (lo + hi) >> 1;
-    const int cm = col[mid];
-    if (cm == target) return mid;
-    if (cm < target) lo = mid + 1; else hi = mid - 1;
-  }
-  return -1;
-}
-
 // One wavefront per Lagrange node j: out[3j..3j+2] = g_j^T {S, T, S0} g_j over the
 // (unmasked) support of column j of G.
 __global__ __launch_bounds__(TB) void k_grad_diag(int64_t nlag, const int32_t* gt_ptr,

@@ -62,6 +62,20 @@ __device__ __forceinline__ int xcd_block(int G) {
   return (b & 7) * (G >> 3) + (b >> 3);
 }
 
+// Position of `target` in the ascending col[lo, hi) (one CSR row), or -1.  The midpoint
+// is lo + (hi - lo) / 2: the row pointers of a pattern with more than 2^30 nonzeros (the
+// 120 x 60 x 90-box order-2 cavity has 1,078,395,480) make lo + hi overflow 32 bits.
+__device__ __forceinline__ int find_col(const int32_t* col, int lo, int hi, int target) {
+  hi -= 1;
+  while (lo <= hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    const int cm = col[mid];
+    if (cm == target) return mid;
+    if (cm < target) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
 // ---- device scalars of the Krylov recursions -------------------------------------------
 enum ScalarSlot {
   S_RHO = 0,      // complex: current rho

@@ -13,6 +13,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "internal.hpp"
+#include "kernels.hpp"
 #include "refdata.hpp"
 
 namespace sucem {
@@ -350,13 +351,7 @@ __global__ void k_row_count(int64_t nnz, const uint64_t* ukeys, int B, int32_t* 
 __global__ void k_diag(int64_t n, const int32_t* rp, const int32_t* col, int32_t* diag) {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
        r += (int64_t)gridDim.x * blockDim.x) {
-    int lo = rp[r], hi = rp[r + 1] - 1, pos = -1;
-    while (lo <= hi) {
-      int mid = (lo + hi) >> 1;
-      if (col[mid] == r) { pos = mid; break; }
-      if (col[mid] < r) lo = mid + 1; else hi = mid - 1;
-    }
-    diag[r] = pos;
+    diag[r] = find_col(col, rp[r], rp[r + 1], (int)r);
   }
 }
 

@@ -255,15 +255,18 @@ def device_executor(tol=1e-8, max_iter=50000, precision=64, eig_tol=1e-8, eig_in
     local = threading.local()
 
     def ctx():
+        # a stream and a library context per slot thread; torch hands out streams from a
+        # fixed pool, so two threads can be given the same stream: the context (and the
+        # solver workspace it holds) is still the thread's own, never shared
         if not hasattr(local, "stream"):
+            from .runtime import Context
             local.stream = torch.cuda.Stream()             # default priority
-        return local.stream
+            local.context = Context(stream=local.stream)
+        return local.stream, local.context
 
     def run(req):
-        stream = ctx()
+        stream, context = ctx()
         with torch.cuda.stream(stream):
-            from .runtime import get_context
-            context = get_context()
             if req.kind == 'eigen':
                 prob = DeviceProblem(req.n, req.dims, req.order, _hip.MAT_S | _hip.MAT_T, ctx=context)
                 k2, resid, st = prob.eigs(req.sigma, req.nev, tol=eig_tol,

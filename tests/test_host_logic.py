@@ -141,3 +141,27 @@ def test_device_api_fails_loudly_without_gpu():
     from sucemfem.runtime import get_context
     with pytest.raises(RuntimeError):
         get_context()
+
+
+def test_device_binary_searches_use_the_overflow_free_midpoint():
+    """Row pointers of the 120 x 60 x 90-box order-2 pattern pass 2^30 (1,078,395,480
+    nonzeros), so a 32-bit midpoint (lo + hi) / 2 wraps negative there.  Every CSR row
+    search in the kernels goes through find_col, whose midpoint is lo + (hi - lo) / 2."""
+    import glob
+    import re
+    root = os.path.join(os.path.dirname(__file__), "..", "sucem-fem_amd", "csrc")
+    summed = re.compile(r"\(\s*(lo|low|a)\s*\+\s*(hi|high|b)\s*\)\s*(>>\s*1|/\s*2)")
+    offenders = []
+    for path in sorted(glob.glob(os.path.join(root, "*.hip")) + glob.glob(os.path.join(root, "*.hpp"))):
+        with open(path) as f:
+            for k, line in enumerate(f, 1):
+                if summed.search(line):
+                    offenders.append("%s:%d" % (os.path.basename(path), k))
+    assert not offenders, offenders
+    with open(os.path.join(root, "kernels.hpp")) as f:
+        assert "lo + ((hi - lo) >> 1)" in f.read()
+    # the midpoint stays inside [lo, hi] for row pointers up to the 32-bit limit
+    for lo, hi in ((2 ** 30, 2 ** 30 + 40), (2 ** 31 - 50, 2 ** 31 - 2), (0, 7)):
+        mid = lo + ((hi - lo) >> 1)
+        assert lo <= mid <= hi and mid < 2 ** 31
+        assert lo + hi >= 2 ** 31 or hi < 2 ** 30   # the summed form would have wrapped
