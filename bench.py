@@ -188,6 +188,7 @@ def eigen_measure(n=(120, 60, 90), nev=20, sigma=1.1, seed=0, precisions=(32, 64
                     "inner_iterations": st["inner_iterations"],
                     "inner_stagnated": st["inner_stagnated"],
                     "inner_failures": st["inner_failures"],
+                    "inner_fp64_from": st["inner_fp64_from"],
                     "max_ritz_residual": st["max_ritz_residual"],
                     "max_mode_residual": float(np.max(resid)), "k2": [float(v) for v in k2]}
         log("eigen FP%d inner: %.2f s, %d Lanczos steps, %d inner iterations, status %d" % (
@@ -197,6 +198,42 @@ def eigen_measure(n=(120, 60, 90), nev=20, sigma=1.1, seed=0, precisions=(32, 64
         out["max_rel_k2_diff_fp32ir_vs_fp64"] = float(np.max(np.abs(a - b) / np.abs(b)))
         if out["fp64"]["s"]:
             out["fp32ir_vs_fp64"] = out["fp64"]["s"] / out["fp32ir"]["s"]
+    return out
+
+
+def inner_measure(n=(120, 60, 90), sigma=1.1, seed=0, precisions=(64, 32), tol=1e-9,
+                  max_iter=40000):
+    """One inner solve of the eigen workload: (S - sigma T) x = b on the dielectric-block
+    cavity with the PEC DOFs eliminated, auxiliary-space COCG to ``tol``, b a seeded
+    normal vector; per precision: seconds, iterations, ms per iteration and per operator
+    application (HIP events inside the solve)."""
+    import torch
+    from sucemfem.Utilities.Workloads import dielectric_cavity
+    t0 = time.perf_counter()
+    prob = dielectric_cavity(n=n, seed=seed)
+    torch.cuda.synchronize()
+    out = {"workload": "inner solve of the dielectric-block cavity %dx%dx%d boxes, order 2: "
+                       "(S - %g T) x = b, PEC eliminated, b seeded normal (seed %d), tol %g"
+                       % (tuple(n) + (sigma, seed, tol)),
+           "ndofs": prob.ndofs, "nnz": prob.nnz, "setup_s": time.perf_counter() - t0}
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    b = torch.randn(prob.ndofs, dtype=torch.complex128, device="cuda", generator=g)
+    for prec in precisions:
+        x = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = prob.solve_coef(1.0, -sigma, 0.0, b, x, precision=prec, tol=tol, max_iter=max_iter,
+                             eliminate_pec=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        key = "fp32ir" if prec == 32 else "fp64"
+        out[key] = {"s": dt, "iterations": st["iterations"], "status": st["status"],
+                    "rel_residual": st["rel_residual"], "fp64_finish": st["fp64_finish"],
+                    "ms_per_iteration": 1e3 * dt / max(st["iterations"], 1),
+                    "op_ms": st["op_ms"]}
+        log("inner FP%d: %.2f s, %d iterations, %.3f ms/iteration, operator %.3f ms, rel %.2e"
+            % (prec, dt, st["iterations"], 1e3 * dt / max(st["iterations"], 1), st["op_ms"],
+               st["rel_residual"]))
     return out
 
 
@@ -258,11 +295,14 @@ def main():
                          "patch128 / patch192 / patch256 (matrix-free per patch of cells), "
                          "for both precisions")
     ap.add_argument("--workload", default="driven",
-                    choices=["driven", "sweep", "eigen", "serve"],
+                    choices=["driven", "sweep", "eigen", "inner", "serve"],
                     help="driven (the headline line), sweep (64 frequencies x 8 dipoles, "
                          "order 1, 40^3), eigen (dielectric-block cavity, 120x60x90 boxes, "
-                         "order 2, 20 modes), serve (mixed request trace)")
+                         "order 2, 20 modes), inner (one inner solve of that cavity's "
+                         "shifted operator), serve (mixed request trace)")
     ap.add_argument("--eig-n", default="120,60,90", help="eigen workload boxes per axis")
+    ap.add_argument("--eig-prec", default="32,64",
+                    help="eigen workload inner precisions, comma-separated (32 = FP32+IR)")
     ap.add_argument("--serve-count", type=int, default=16)
     ap.add_argument("--serve-seed", type=int, default=0)
     ap.add_argument("--no-extras", action="store_true",
@@ -289,10 +329,21 @@ def main():
         return
     if a.workload == "eigen":
         nn = tuple(int(v) for v in a.eig_n.split(","))
-        r = eigen_measure(n=nn)
-        v = r["fp32ir"]["solves_per_s"]
-        print(json.dumps(workload_line("eigen", a, r, v, "solves/s", 1e3 * r["fp32ir"]["s"],
-                                       "f32+f64-ir")), flush=True)
+        precs = tuple(int(v) for v in a.eig_prec.split(","))
+        r = eigen_measure(n=nn, precisions=precs)
+        key = "fp32ir" if 32 in precs else "fp64"
+        v = r[key]["solves_per_s"]
+        print(json.dumps(workload_line("eigen", a, r, v, "solves/s", 1e3 * r[key]["s"],
+                                       "f32+f64-ir" if key == "fp32ir" else "f64")), flush=True)
+        return
+    if a.workload == "inner":
+        nn = tuple(int(v) for v in a.eig_n.split(","))
+        precs = tuple(int(v) for v in a.eig_prec.split(","))
+        r = inner_measure(n=nn, precisions=precs)
+        key = "fp32ir" if 32 in precs else "fp64"
+        v = 1.0 / r[key]["s"]
+        print(json.dumps(workload_line("inner", a, r, v, "solves/s", 1e3 * r[key]["s"],
+                                       "f32+f64-ir" if key == "fp32ir" else "f64")), flush=True)
         return
     if a.workload == "serve":
         r = serve_measure(count=a.serve_count, seed=a.serve_seed)

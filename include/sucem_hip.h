@@ -90,6 +90,10 @@ typedef struct sucem_eig_stats {
   int lanczos_steps;
   int inner_failures;   /* inner solves that hit inner_max_iter or broke down */
   int inner_stagnated;  /* inner solves stopped at the floor of their true residual */
+  int inner_fp64_from;  /* inner_precision 32: the Lanczos step from which the inner solves
+                           ran in FP64 because two in a row had their FP32 cycles fail and
+                           were finished in FP64 (0: all steps ran FP32 cycles) */
+  int pad_;
   int64_t inner_iterations;
   double max_ritz_residual;
   double elapsed_ms;

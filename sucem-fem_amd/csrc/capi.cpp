@@ -849,6 +849,8 @@ int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,
     stats->lanczos_steps = st.lanczos_steps;
     stats->inner_failures = st.inner_failures;
     stats->inner_stagnated = st.inner_stagnated;
+    stats->inner_fp64_from = st.inner_fp64_from;
+    stats->pad_ = 0;
     stats->inner_iterations = st.inner_iterations;
     stats->max_ritz_residual = st.max_ritz_residual;
     stats->elapsed_ms = st.elapsed_ms;

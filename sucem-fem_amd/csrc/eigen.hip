@@ -239,6 +239,7 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
   so.check_every = 8;
   so.tol = o.inner_tol;
   so.eliminate_pec = 1;
+  int fp32_failed_run = 0;
 
   // v0: random, T-normalised
   SUCEM_LAUNCH(k_start_vector, grid_for(n, TB), TB, 0, c->stream, (int64_t)n,
@@ -265,6 +266,18 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
       st.lanczos_steps += 1;
       if (ist.stagnated) st.inner_stagnated += 1;
       else if (ist.status != 0) st.inner_failures += 1;
+      // FP32 inner cycles that fail (and are finished in FP64) on two solves in a row will
+      // fail on the rest: the remaining inner solves run in FP64 from the start
+      if (so.precision == 32) {
+        fp32_failed_run = ist.fp32_failed ? fp32_failed_run + 1 : 0;
+        if (fp32_failed_run >= 2) {
+          so.precision = 64;
+          st.inner_fp64_from = st.lanczos_steps + 1;
+          if (log_level >= 1)
+            std::fprintf(stderr, "[sucem eigs] FP32 inner cycles failed on two solves in a "
+                                 "row: FP64 inner solves from step %d\n", st.inner_fp64_from);
+        }
+      }
       if (log_level >= 2)
         std::fprintf(stderr, "[sucem eigs] step %d: inner %d iterations, rel %.2e%s, %.1f s\n",
                      st.lanczos_steps, ist.iterations, ist.rel_residual,

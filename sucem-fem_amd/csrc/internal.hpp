@@ -266,6 +266,7 @@ struct EigOpts {
 struct EigStats {
   int status = 0, nconv = 0, restarts = 0, lanczos_steps = 0, inner_failures = 0;
   int inner_stagnated = 0;
+  int inner_fp64_from = 0;   // precision 32: Lanczos step from which inner solves ran in FP64
   int64_t inner_iterations = 0;
   double max_ritz_residual = 0, elapsed_ms = 0;
 };

@@ -635,6 +635,9 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   int restart = 1;
   // FP32 refinement cycles: a cycle runs from one residual replacement to the next
   int cycle_start = 0, longest_cycle = 0;
+  // best recursion residual of the current cycle and the iteration it was reached
+  double cycle_best = 1e300;
+  int cycle_best_it = 0;
   st.status = o.max_iter;
   double rel = 1.0;
   // Operator timing: at the first iteration of every check interval (from the second
@@ -711,7 +714,18 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     // the first cycle has no completed cycle to compare with: a fixed cap
     const int cap = longest_cycle > 0 ? std::max(64, 3 * longest_cycle)
                                       : std::max(1000, o.max_iter / 8);
-    const bool cut = lo && cyc >= cap;
+    // An FP32 cycle whose recursion has not set a new low (by 10 %) for 256 iterations (or
+    // twice the longest completed cycle) has stalled.  On the shifted cavity operator
+    // S - sigma T of a fine mesh the preconditioned directions carry gradient components
+    // about 1/(sigma h^2) larger than the rest, which S annihilates only to FP32 rounding:
+    // the recursion then wanders instead of converging, and waiting for the fixed cap
+    // would spend up to max_iter / 8 iterations on it.
+    if (rel < 0.9 * cycle_best) {
+      cycle_best = rel;
+      cycle_best_it = it + 1;
+    }
+    const int stall = std::max(256, 2 * longest_cycle);
+    const bool cut = lo && (cyc >= cap || it + 1 - cycle_best_it >= stall);
     // an FP32 recursion whose residual grew a thousandfold over the cycle's start has
     // diverged (COCG on an indefinite operator with an indefinite preconditioner has no
     // monotone residual; in FP32 it can run away): its correction is dropped and the
@@ -726,6 +740,8 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     if (!ru) continue;
     if (!cut) longest_cycle = std::max(longest_cycle, cyc);
     cycle_start = it + 1;
+    cycle_best = 1e300;
+    cycle_best_it = it + 1;
     // reliable update / residual replacement with the FP64 operator
     if (lo) SUCEM_LAUNCH((k_accum<C>), G, TB, 0, c->stream, n, G, xlo, x, 1.0, 0);
     op.apply_hi(x, ax.p);

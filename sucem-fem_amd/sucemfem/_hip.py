@@ -49,7 +49,8 @@ class EigOpts(ctypes.Structure):
 class EigStats(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("nconv", ctypes.c_int), ("restarts", ctypes.c_int),
                 ("lanczos_steps", ctypes.c_int), ("inner_failures", ctypes.c_int),
-                ("inner_stagnated", ctypes.c_int), ("inner_iterations", ctypes.c_int64),
+                ("inner_stagnated", ctypes.c_int), ("inner_fp64_from", ctypes.c_int),
+                ("pad_", ctypes.c_int), ("inner_iterations", ctypes.c_int64),
                 ("max_ritz_residual", ctypes.c_double), ("elapsed_ms", ctypes.c_double)]
 
     def as_dict(self):

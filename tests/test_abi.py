@@ -54,3 +54,36 @@ def test_null_handles_are_rejected_without_gpu():
     o = _hip.SolveOpts()
     lib.sucem_solve_opts_default(ctypes.byref(o))
     assert o.tol == 1e-8 and o.precision == 64 and o.method == _hip.METHOD_COCG
+
+
+def test_struct_layouts_match_the_header():
+    """The ctypes mirrors of the option and statistics structs have the header's field
+    offsets and sizes (gcc on include/sucem_hip.h)."""
+    import ctypes
+    import shutil
+    import subprocess
+    import tempfile
+    from sucemfem import _hip
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    structs = {"sucem_solve_opts": _hip.SolveOpts, "sucem_solve_stats": _hip.SolveStats,
+               "sucem_eig_opts": _hip.EigOpts, "sucem_eig_stats": _hip.EigStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sucem_hip.h"',
+             'int main(void) {']
+    for cname, py in structs.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append('return 0; }')
+    inc = os.path.join(os.path.dirname(__file__), "..", "include")
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "layout.c"), os.path.join(d, "layout")
+        with open(src, "w") as fh:
+            fh.write("\n".join(lines))
+        subprocess.check_call(["gcc", "-I", inc, src, "-o", exe])
+        out = subprocess.check_output([exe]).decode().split("\n")
+    got = dict(l.rsplit(" ", 1) for l in out if l)
+    for cname, py in structs.items():
+        assert int(got["%s size" % cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)
