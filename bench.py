@@ -555,6 +555,16 @@ def main():
                              "algorithmic bytes"},
     }
     out.update(extra)
+    if not a.no_extras:
+        # the other configured workloads, measured in this run and nested in the same line:
+        # the full 512-solve sweep, the dielectric-block cavity at half its 120x60x90
+        # resolution per axis (the full size runs ~8 min per precision: --workload eigen),
+        # and the serving trace
+        wl = {}
+        wl["sweep"] = sweep_measure()
+        wl["eigen"] = eigen_measure(n=(60, 30, 45))
+        wl["serving"] = serve_measure(count=a.serve_count, seed=a.serve_seed)
+        out["workloads"] = wl
     print(json.dumps(out), flush=True)
 
 
