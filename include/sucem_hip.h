@@ -288,7 +288,11 @@ int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, 
 /* ---- cavity eigenproblem (replaces DefaultEigenSolver.solve_problem,
  *      EMVectorWaveEigenproblem.py:61-105): the nev lowest k^2 > sigma of the PEC-eliminated
  *      pencil (S, T), ascending, returned as k^2 directly (the reference returns
- *      nu = 1/(k^2 - sigma)).  resid_h (nullable): ||S x - k^2 T x|| / (k^2 ||T x||);
+ *      nu = 1/(k^2 - sigma)).  Only modes above sigma: the reference's ARPACK shift-invert
+ *      (which='LM') returns the nev modes nearest sigma on either side, which differs when
+ *      sigma lies between modes; place sigma below the wanted band (the reference drivers
+ *      use 1.1, below the lowest cavity mode).  resid_h (nullable): ||S x - k^2 T x|| /
+ *      (k^2 ||T x||);
  *      vecs_d (nullable): nev x N complex128 T-normalised modes (zero on PEC DOFs). ---- */
 void sucem_eig_opts_default(sucem_eig_opts* o);
 int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,

@@ -48,6 +48,23 @@ def test_cavity_k2_matches_oracle_pencil(order):
     assert np.abs(modes[:, pec]).max() == 0.0
 
 
+def test_sigma_between_modes_returns_the_modes_above_sigma():
+    """sigma between the 3rd and 4th cavity modes: the device returns the 5 lowest k^2
+    above sigma, as the oracle's pencil has them (the modes below sigma are not returned:
+    the above-sigma contract of sucem_eigs)."""
+    from sucemfem.ProblemConfigurations.EMVectorWaveEigenproblem import DefaultEigenSolver
+    print("sigma between modes", flush=True)
+    ep, (co, ce) = _cavity(2)
+    ref_all, _, _ = O.cavity_eigs(O.Space(co, ce, 2), 10, 1.1)
+    sigma = 0.5 * (ref_all[2] + ref_all[3])
+    es = DefaultEigenSolver()
+    es.set_eigenproblem(ep)
+    es.set_sigma(sigma)
+    k2, _, _ = es.solve_problem_k2(5)
+    assert es.stats["status"] == 0, es.stats
+    assert np.all(np.abs(k2 - ref_all[3:8]) / ref_all[3:8] <= 1e-8), (k2, ref_all[3:8])
+
+
 def test_reference_return_convention():
     """solve_problem returns nu = 1/(k^2 - sigma); the examples recover k^2 = 1/nu + sigma
     (examples/resonant_cavity/driver.py:66)."""

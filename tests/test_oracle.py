@@ -182,6 +182,23 @@ def test_cavity_order1_null_space_and_modes():
     assert np.all(np.abs(k2 - ana) / ana < 0.08)
 
 
+def test_cavity_sigma_between_modes_selects_modes_above_sigma():
+    """With sigma between two cavity modes the shift-invert branch (largest positive
+    nu = 1/(k^2 - sigma)) returns the nev lowest k^2 above sigma, the same values dense
+    eigh of the pencil gives there; the modes below sigma are not returned."""
+    co, ce = O.box_mesh([1.0, 0.5, 0.75], 4)
+    sp_ = O.Space(co, ce, 1)
+    m = O.assemble_fast(sp_, abc=False)
+    free = np.flatnonzero(~sp_.pec_mask())
+    w = scipy.linalg.eigh(m["S"][free][:, free].toarray(), m["M"][free][:, free].toarray(),
+                          eigvals_only=True)
+    phys = np.sort(w[w > 1e-6])
+    sigma = 0.5 * (phys[2] + phys[3])                     # between the 3rd and 4th modes
+    k2, _, _ = O.cavity_eigs(sp_, 5, sigma, dense_limit=0)   # force the eigsh branch
+    assert np.allclose(k2, phys[3:8], rtol=1e-10, atol=0), (k2, phys[3:8])
+    assert np.all(k2 > sigma)
+
+
 def test_cavity_order2_close_to_analytic():
     co, ce = O.box_mesh([1.0, 0.5, 0.75], 4)
     sp_ = O.Space(co, ce, 2)
