@@ -286,6 +286,9 @@ def main():
     ap.add_argument("--precision", type=int, default=32, help="headline path: 32 (+IR) or 64")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=60000)
+    ap.add_argument("--precond-shift", type=float, default=-1.0,
+                    help="complex shift of the preconditioner's operator (< 0: the library's "
+                         "default for driven solves, 0.25)")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the comparison solve in the other precision")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -376,7 +379,8 @@ def main():
     stream = torch.cuda.current_stream()
 
     def step(precision):
-        return prob.solve(prob.k0, b, x, precision=precision, tol=a.tol, max_iter=a.max_iter)
+        return prob.solve(prob.k0, b, x, precision=precision, tol=a.tol, max_iter=a.max_iter,
+                          precond_shift=a.precond_shift)
 
     for i in range(a.warmup):
         st = step(a.precision)

@@ -70,6 +70,12 @@ typedef struct sucem_solve_opts {
   double ru_delta;    /* FP32: reliable update when the residual fell by this factor */
   int gmres_restart;  /* GMRES: Arnoldi steps per cycle (SciPy's default 20) */
   int pad_;
+  double precond_shift; /* complex shift beta of the operator the Jacobi / gradient /
+                           auxiliary diagonals are built from: cs S + ct (1 - i beta) T +
+                           i c0 S0 (the solved operator is unchanged).  < 0 (the default):
+                           0.25 for the driven entry points (fewer COCG iterations on driven
+                           Helmholtz systems), 0 for sucem_solve_system and generic matrices
+                           (the shifted cavity operator S - sigma T converges slower with it) */
 } sucem_solve_opts;
 
 typedef struct sucem_eig_opts {

@@ -769,19 +769,22 @@ def gradient_diag(A, G):
     return np.asarray(G.multiply(AG).sum(axis=0)).ravel()
 
 
-def cocg_gradient(A, b, G, tol=1e-8, maxiter=10000, gdiag=None, theta_g=2.0):
+def cocg_gradient(A, b, G, tol=1e-8, maxiter=10000, gdiag=None, theta_g=2.0, A_prec=None):
     """COCG with the additive gradient-space preconditioner, FP64 (scipy CSR).
 
     M^-1 r = D^-1 r + theta_g G diag(G^T A G)^-1 G^T r, D = diag(A); Lagrange nodes whose
     diagonal is zero (none on a mesh without PEC walls) are dropped. Same
-    recurrences as ``cocg_jacobi`` with z = M^-1 r. Returns ``(x, iterations, history)``.
+    recurrences as ``cocg_jacobi`` with z = M^-1 r.  ``A_prec``: the matrix the diagonals
+    are taken from (default A; the device builds them from a complex-shifted driven
+    operator).  Returns ``(x, iterations, history)``.
     """
     A = A.tocsr()
+    P = A if A_prec is None else A_prec.tocsr()
     if gdiag is None:
-        gdiag = gradient_diag(A, G)
+        gdiag = gradient_diag(P, G)
     G = G.tocsr().astype(np.complex128)        # complex once, not per product
     Gt = G.T.tocsr()
-    D = 1.0 / A.diagonal()
+    D = 1.0 / P.diagonal()
     W = np.zeros(len(gdiag), np.complex128)
     nz = gdiag != 0
     W[nz] = theta_g / gdiag[nz]
@@ -844,21 +847,23 @@ def nodal_interpolation(space):
 
 
 def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000, gdiag=None, vdiag=None,
-                   theta_g=3.0):
+                   theta_g=3.0, A_prec=None):
     """COCG with the auxiliary-space preconditioner, FP64 (scipy CSR):
     M^-1 r = D^-1 r + theta_g G diag(G^T A G)^-1 G^T r + Pi diag(Pi^T A Pi)^-1 Pi^T r
     (zero diagonals dropped; ``gdiag`` / ``vdiag`` precomputed diagonals; the device uses
-    theta_g = 3 at order 2 and 2 at order 1).  Returns ``(x, iterations, history)``."""
+    theta_g = 3 at order 2 and 2 at order 1).  ``A_prec``: the matrix D and the two
+    auxiliary diagonals are taken from (default A).  Returns ``(x, iterations, history)``."""
     A = A.tocsr()
-    D = 1.0 / A.diagonal()
+    P = A if A_prec is None else A_prec.tocsr()
+    D = 1.0 / P.diagonal()
 
     def inv(d):
         w = np.zeros(len(d), np.complex128)
         nz = d != 0
         w[nz] = 1.0 / d[nz]
         return w
-    W = theta_g * inv(gradient_diag(A, G) if gdiag is None else gdiag)
-    V = inv(gradient_diag(A, Pi) if vdiag is None else vdiag)
+    W = theta_g * inv(gradient_diag(P, G) if gdiag is None else gdiag)
+    V = inv(gradient_diag(P, Pi) if vdiag is None else vdiag)
     G = G.tocsr().astype(np.complex128)
     Gt = G.T.tocsr()
     Pi = Pi.tocsr().astype(np.complex128)

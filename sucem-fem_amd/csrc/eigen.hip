@@ -239,6 +239,7 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
   so.check_every = 8;
   so.tol = o.inner_tol;
   so.eliminate_pec = 1;
+  so.precond_shift = 0.0;   // a shifted preconditioner slows the cavity operator S - sigma T
   int fp32_failed_run = 0;
 
   // v0: random, T-normalised

@@ -111,14 +111,14 @@ __global__ __launch_bounds__(TB) void k_grad_diag(int64_t nlag, const int32_t* g
   }
 }
 
-// winv[j] = theta / (cs dS + ct dT + i c0 dS0), 0 where the diagonal vanishes
+// winv[j] = theta / (cs dS + (ct + i cti) dT + i c0 dS0), 0 where the diagonal vanishes
 template <typename C>
-__global__ void k_grad_winv(int64_t nlag, const double* d3, double cs, double ct, double c0,
-                            double theta, C* winv) {
+__global__ void k_grad_winv(int64_t nlag, const double* d3, double cs, double ct, double cti,
+                            double c0, double theta, C* winv) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nlag;
        j += (int64_t)gridDim.x * blockDim.x) {
     const double re = cs * d3[3 * j] + ct * d3[3 * j + 1];
-    const double im = c0 * d3[3 * j + 2];
+    const double im = cti * d3[3 * j + 1] + c0 * d3[3 * j + 2];
     const double den = re * re + im * im;
     winv[j] = den > 0.0 ? from_d<C>(make_double2(theta * re / den, -theta * im / den))
                         : from_d<C>(make_double2(0.0, 0.0));
@@ -770,7 +770,7 @@ void nodal_winv(Context* c, System* sys, Coef k, bool masked, void* vinv) {
   const double* d9 = nodal_diag(c, sys, masked);
   using C = typename C2<R>::T;
   const int64_t nn = 3 * sys->space->mesh->nv;
-  SUCEM_LAUNCH((k_grad_winv<C>), grid_for(nn, TB), TB, 0, c->stream, nn, d9, k.s, k.t, k.s0, 1.0,
+  SUCEM_LAUNCH((k_grad_winv<C>), grid_for(nn, TB), TB, 0, c->stream, nn, d9, k.s, k.t, k.ti, k.s0, 1.0,
                (C*)vinv);
 }
 template void nodal_winv<float>(Context*, System*, Coef, bool, void*);
@@ -869,7 +869,7 @@ void grad_winv(Context* c, System* sys, Coef k, bool masked, bool aux, void* win
   const double theta = gradient_weight(sys->space->order, aux);
   using C = typename C2<R>::T;
   SUCEM_LAUNCH((k_grad_winv<C>), grid_for(sys->space->nlag, TB), TB, 0, c->stream,
-               sys->space->nlag, d3, k.s, k.t, k.s0, theta, (C*)winv);
+               sys->space->nlag, d3, k.s, k.t, k.ti, k.s0, theta, (C*)winv);
 }
 template void grad_winv<float>(Context*, System*, Coef, bool, bool, void*);
 template void grad_winv<double>(Context*, System*, Coef, bool, bool, void*);

@@ -184,6 +184,9 @@ void export_values(Context* c, const System* sys, unsigned which_one, double* ou
 // operator S - k0^2 T + i k0 S0 is {1, -k0^2, k0}.
 struct Coef {
   double s = 1.0, t = 0.0, s0 = 0.0;
+  // preconditioner diagonals only: imaginary part of the T coefficient (a complex shift
+  // of the operator the diagonals are built from; the operator itself never uses it)
+  double ti = 0.0;
 };
 inline Coef driven_coef(double k0) { Coef k; k.s = 1.0; k.t = -k0 * k0; k.s0 = k0; return k; }
 
@@ -218,6 +221,10 @@ struct SolveOpts {
   int eliminate_pec = 0;
   int warm = 0;          // x holds the initial guess (otherwise x0 = 0)
   int restart = 20;      // GMRES restart length
+  // complex shift beta of the preconditioner's operator: its diagonals are built from
+  // cs S + ct (1 - i beta) T + i c0 S0 (driven Helmholtz systems converge faster);
+  // < 0: not resolved yet (the entry points set 0.5 for driven systems, 0 otherwise)
+  double precond_shift = -1.0;
 };
 struct SolveStats {
   int status = 0, iterations = 0, reliable_updates = 0;

@@ -430,6 +430,7 @@ template <typename R>
 struct SysOp : Op {
   const System* sys = nullptr;
   Coef k;
+  Coef kp;   // the preconditioner's coefficients (k with the complex shift)
   bool identity_prec = false;
   void apply_lo(const void* x, void* y, const void* w, int epi, const RedSite& rs) override {
     sys_apply<R>(c, sys, k, mask, x, y, w, epi, rs, flo);
@@ -440,7 +441,7 @@ struct SysOp : Op {
   }
   void dinv_lo(void* d) override {
     if (identity_prec) csr_dinv<R>(c, nullptr, mask, true, d, n);
-    else sys_dinv<R>(c, sys, k, mask, d);
+    else sys_dinv<R>(c, sys, kp, mask, d);
   }
   int64_t nlag() override {
     ensure_gradient(c, sys->space);
@@ -448,7 +449,7 @@ struct SysOp : Op {
     return sys->space->nlag;
   }
   void winv_lo(void* winv) override {
-    grad_winv<R>(c, const_cast<System*>(sys), k, mask != nullptr, aux, winv);
+    grad_winv<R>(c, const_cast<System*>(sys), kp, mask != nullptr, aux, winv);
   }
   void precond_lo(const void* winv, const void* dinv, const void* r, void* w, void* z, int G,
                   const RedSite& rs, const void* vinv, void* u) override {
@@ -459,7 +460,7 @@ struct SysOp : Op {
     return 3 * sys->space->mesh->nv;
   }
   void vinv_lo(void* vinv) override {
-    nodal_winv<R>(c, const_cast<System*>(sys), k, mask != nullptr, vinv);
+    nodal_winv<R>(c, const_cast<System*>(sys), kp, mask != nullptr, vinv);
   }
   int deferred_parts() override { return pt_deferred_parts(sys, flo); }
   void apply_deferred(const void* x, void* y, const void* w, const RedSite& rs) override {
@@ -1118,6 +1119,9 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
                   const SolveOpts& o, SolveStats& st, double* hist_h, int64_t hist_cap) {
   SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S and T must be assembled");
   check_opts(o);
+  // the preconditioner's diagonals come from cs S + ct (1 - i beta) T + i c0 S0
+  Coef kp = k;
+  kp.ti = -std::max(0.0, o.precond_shift) * k.t;
   const Space* s = sys->space;
   auto fill = [&](Op& op) {
     op.c = c;
@@ -1131,6 +1135,7 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
     op.flo = op.fhi;
     op.sys = sys;
     op.k = k;
+    op.kp = kp;
     op.identity_prec = o.precond == SUCEM_PRECOND_NONE;
     gmres(op, b, x, o, st, hist_h, hist_cap);
     return;
@@ -1141,6 +1146,7 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
     op.flo = sys_cfg<float>(c, sys);
     op.sys = sys;
     op.k = k;
+    op.kp = kp;
     op.identity_prec = o.precond == SUCEM_PRECOND_NONE;
     krylov<float>(op, b, x, o, st, hist_h, hist_cap);
     if (st.fp32_failed) {
@@ -1154,6 +1160,7 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
       op64.flo = op64.fhi;
       op64.sys = sys;
       op64.k = k;
+      op64.kp = kp;
       op64.identity_prec = op.identity_prec;
       SolveStats s64;
       krylov<double>(op64, b, x, o64, s64, nullptr, 0);
@@ -1175,6 +1182,7 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
     op.flo = op.fhi;
     op.sys = sys;
     op.k = k;
+    op.kp = kp;
     op.identity_prec = o.precond == SUCEM_PRECOND_NONE;
     krylov<double>(op, b, x, o, st, hist_h, hist_cap);
   }

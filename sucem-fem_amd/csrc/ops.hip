@@ -163,12 +163,14 @@ __global__ __launch_bounds__(TB) void k_csr_spmv(int n, int G, const int32_t* __
 template <typename R>
 __global__ void k_sys_dinv(int64_t n, const int32_t* diag, const typename ST2<R>::T* st,
                            const int32_t* s0diag, const R* s0v, double cs, double ct,
-                           double c0, const uint8_t* mask, typename C2<R>::T* dinv) {
+                           double cti, double c0, const uint8_t* mask,
+                           typename C2<R>::T* dinv) {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
        r += (int64_t)gridDim.x * blockDim.x) {
     const auto v = st[diag[r]];
     double re = cs * (double)v.x + ct * (double)v.y;
-    double im = (s0diag && s0diag[r] >= 0) ? c0 * (double)s0v[s0diag[r]] : 0.0;
+    double im = cti * (double)v.y;
+    if (s0diag && s0diag[r] >= 0) im += c0 * (double)s0v[s0diag[r]];
     double den = re * re + im * im;
     bool off = (mask && mask[r]) || den == 0.0;
     dinv[r] = off ? mk(R(0), R(0)) : mk(R(re / den), R(-im / den));
@@ -544,11 +546,11 @@ void sys_dinv(Context* c, const System* sys, Coef k, const uint8_t* mask, void* 
   if constexpr (sizeof(R) == 4)
     SUCEM_LAUNCH((k_sys_dinv<float>), grid_for(s->ndofs, TB), TB, 0, c->stream, s->ndofs,
                  s->diag.p, sys->st32.p, has_s0 ? s->s0_diag.p : nullptr, sys->s0_32.p, k.s, k.t,
-                 k.s0, mask, (float2*)dinv);
+                 k.ti, k.s0, mask, (float2*)dinv);
   else
     SUCEM_LAUNCH((k_sys_dinv<double>), grid_for(s->ndofs, TB), TB, 0, c->stream, s->ndofs,
                  s->diag.p, sys->st64.p, has_s0 ? s->s0_diag.p : nullptr, sys->s0_64.p, k.s, k.t,
-                 k.s0, mask, (double2*)dinv);
+                 k.ti, k.s0, mask, (double2*)dinv);
 }
 template void sys_dinv<float>(Context*, const System*, Coef, const uint8_t*, void*);
 template void sys_dinv<double>(Context*, const System*, Coef, const uint8_t*, void*);

@@ -1006,6 +1006,7 @@ void sweep_chunk(Context* c, System* sys, int nb, const double* k0, const double
       const Coef cf = driven_coef(k0[j < nb ? j : 0]);
       Coef cj = cf;
       if (!(sys->which & SUCEM_MAT_S0)) cj.s0 = 0;
+      cj.ti = -std::max(0.0, o.precond_shift) * cj.t;   // the preconditioner's complex shift
       sys_dinv<R>(c, sys, cj, k.mask, tmp);
       SUCEM_LAUNCH((k_sw_put<C>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, NB, j,
                    (const C*)tmp, k.dinv);

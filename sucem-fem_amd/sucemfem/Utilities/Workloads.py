@@ -73,7 +73,8 @@ class DeviceProblem(object):
                              % (dtype, nvec * self.ndofs, t.dtype, t.numel()))
 
     def solve(self, k0, b, x, precision=64, tol=1e-8, max_iter=50000, method=_hip.METHOD_COCG,
-              check_every=16, ru_delta=0.01, hist=None, precond=_hip.PRECOND_AUXILIARY):
+              check_every=16, ru_delta=0.01, hist=None, precond=_hip.PRECOND_AUXILIARY,
+              precond_shift=-1.0):
         import torch
         self._check_vec(b, torch.complex128)
         self._check_vec(x, torch.complex128)
@@ -83,6 +84,7 @@ class DeviceProblem(object):
         _hip.load().sucem_solve_opts_default(ctypes.byref(o))
         o.method, o.precision, o.tol, o.max_iter = method, precision, tol, max_iter
         o.check_every, o.ru_delta, o.precond = check_every, ru_delta, precond
+        o.precond_shift = float(precond_shift)
         st = _hip.SolveStats()
         _hip.call("sucem_solve_driven", self.ctx.handle, self.system.handle.h, float(k0),
                   _hip.ptr(b), _hip.ptr(x), ctypes.byref(o), ctypes.byref(st), _hip.ptr(hist),
@@ -128,7 +130,7 @@ class DeviceProblem(object):
         return [s.as_dict() for s in st]
 
     def solve_sweep(self, k0s, B, X, precision=32, tol=1e-8, max_iter=50000, check_every=16,
-                    width=8, use_graph=True, ru_delta=0.01):
+                    width=8, use_graph=True, ru_delta=0.01, precond_shift=-1.0):
         """Solve len(k0s) driven systems together (``sucem_solve_driven_sweep``): ``width``
         interleaved COCG recursions at a time sharing each operator pass, the iteration
         replayed from a hipGraph.  B, X: (nb, N) complex128 device tensors."""
@@ -140,6 +142,7 @@ class DeviceProblem(object):
         _hip.load().sucem_solve_opts_default(ctypes.byref(o))
         o.method, o.precision, o.tol, o.max_iter = _hip.METHOD_COCG, precision, tol, max_iter
         o.check_every, o.precond, o.ru_delta = check_every, _hip.PRECOND_AUXILIARY, ru_delta
+        o.precond_shift = float(precond_shift)
         st = (_hip.SolveStats * nb)()
         k0 = np.ascontiguousarray(k0s, np.float64)
         _hip.call("sucem_solve_driven_sweep", self.ctx.handle, self.system.handle.h, nb,

@@ -24,7 +24,7 @@ class SolveOpts(ctypes.Structure):
                 ("max_iter", ctypes.c_int), ("check_every", ctypes.c_int),
                 ("eliminate_pec", ctypes.c_int), ("tol", ctypes.c_double),
                 ("ru_delta", ctypes.c_double), ("gmres_restart", ctypes.c_int),
-                ("pad_", ctypes.c_int)]
+                ("pad_", ctypes.c_int), ("precond_shift", ctypes.c_double)]
 
 
 class SolveStats(ctypes.Structure):

@@ -356,7 +356,7 @@ def test_gradient_preconditioner_iterations_match_oracle(dev):
     b = prob.rhs_dipoles(np.array([[0.051, 0.049, 0.05]]), np.array([[0, 1e-3, 1e-3]]), k0)
     x = torch.zeros_like(b)
     st = prob.solve(k0, b, x, precision=64, tol=1e-9, max_iter=20000, check_every=1,
-                    precond=_hip.PRECOND_GRADIENT)
+                    precond=_hip.PRECOND_GRADIENT, precond_shift=0.0)
     co, ce = O.unit_cube(4, 4, 4)
     osp = O.Space(co * 0.1, ce, 2)
     A = O.driven_lhs(O.assemble_fast(osp, eps_r=np.array([1.0, 4.4])[prob.region]), 1e9)
@@ -418,6 +418,42 @@ def test_auxiliary_preconditioner_iterations_match_oracle(dev, order, pec):
             # 1e-12 in A move the oracle's own count by 1 .. 4 of ~295, and the device sums
             # in another order, so the count is compared to 5 %
             assert abs(st["iterations"] - its) <= max(4, its // 20), (st["iterations"], its)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_shifted_driven_preconditioner_follows_oracle(dev, order):
+    """The driven entry point builds the auxiliary-space preconditioner's diagonals from
+    S - (1 - 0.25 i) k0^2 T + i k0 S0 (the default complex shift): the FP64 device solve
+    follows the oracle's auxiliary COCG with that preconditioning matrix (iteration count
+    up to rounding, the same solution), and an explicit shift of 0 follows the unshifted
+    one."""
+    import torch
+    from sucemfem.Utilities.Workloads import DeviceProblem
+    from sucemfem import _hip
+    _say("shifted driven preconditioner order %d" % order)
+    prob = DeviceProblem(4, 0.1, order, _hip.MAT_S | _hip.MAT_T | _hip.MAT_S0, (1.0, 4.4),
+                         seed=1)
+    co, ce = O.unit_cube(4, 4, 4)
+    osp = O.Space(co * 0.1, ce, order)
+    m = O.assemble_fast(osp, eps_r=np.array([1.0, 4.4])[prob.region])
+    k0 = 2 * np.pi * 1e9 / O.c0
+    A = O.driven_lhs(m, 1e9)
+    rng = np.random.default_rng(5)
+    bh = rng.standard_normal(osp.ndofs) + 1j * rng.standard_normal(osp.ndofs)
+    G, Pi = O.discrete_gradient(osp), O.nodal_interpolation(osp)
+    theta = 3.0 if order == 2 else 2.0
+    b = torch.as_tensor(bh, device="cuda")
+    for shift in (-1.0, 0.0):
+        beta = 0.25 if shift < 0 else shift
+        Ap = (m["S"] - (1 - 1j * beta) * k0 ** 2 * m["M"] + 1j * k0 * m["S_0"]).tocsr()
+        xo, its, _ = O.cocg_auxiliary(A, bh, G, Pi, tol=1e-9, maxiter=20000, theta_g=theta,
+                                      A_prec=Ap)
+        x = torch.zeros_like(b)
+        st = prob.solve(k0, b, x, precision=64, tol=1e-9, max_iter=20000, check_every=1,
+                        precond_shift=shift)
+        assert st["status"] == 0 and st["rel_residual"] <= 1e-9, (shift, st)
+        assert abs(st["iterations"] - its) <= max(4, its // 20), (shift, st["iterations"], its)
+        assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) < 1e-6
 
 
 @pytest.mark.parametrize("order", [1, 2])
