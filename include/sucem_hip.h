@@ -286,6 +286,14 @@ int sucem_solve_driven_batch(sucem_ctx* ctx, sucem_system* sys, int64_t nb, cons
 int sucem_solve_driven_sweep(sucem_ctx* ctx, sucem_system* sys, int64_t nb, const double* k0_h,
                              const double* B_d, double* X_d, const sucem_solve_opts* opts,
                              int width, int use_graph, sucem_solve_stats* stats_h);
+/* z = M^-1 r, the auxiliary-space preconditioner of (cs S + ct T + i c0 S0) that COCG applies
+ * (Jacobi + gradient + vector nodal terms, diagonals built with the complex shift `shift`),
+ * once, in precision 32 (r, z complex64) or 64 (complex128); PEC rows zero with
+ * eliminate_pec.  No counterpart in the reference (its preconditioners are SciPy's): for
+ * testing the preconditioner kernels against a restatement of M^-1. */
+int sucem_system_precond_apply(sucem_ctx* ctx, sucem_system* sys, double cs, double ct,
+                               double c0, double shift, int precision, int eliminate_pec,
+                               const void* r_d, void* z_d);
 /* Solve (cs S + ct T + i c0 S0) x = b (e.g. the shifted S - sigma T of the eigen solver). */
 int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, double c0,
                        const double* b_d, double* x_d, const sucem_solve_opts* opts,

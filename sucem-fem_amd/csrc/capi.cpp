@@ -784,6 +784,19 @@ int sucem_solve_system(sucem_ctx* ctx, sucem_system* sys, double cs, double ct, 
   API_END
 }
 
+int sucem_system_precond_apply(sucem_ctx* ctx, sucem_system* sys, double cs, double ct,
+                               double c0, double shift, int precision, int eliminate_pec,
+                               const void* r_d, void* z_d) {
+  API_BEGIN
+  NOT_NULL(ctx); NOT_NULL(sys); NOT_NULL(r_d); NOT_NULL(z_d);
+  Context* c = &ctx->c;
+  c->use();
+  Coef k;
+  k.s = cs; k.t = ct; k.s0 = c0;
+  precond_apply(c, &sys->s, k, shift, precision, eliminate_pec != 0, r_d, z_d);
+  API_END
+}
+
 int sucem_surface_flux(sucem_ctx* ctx, sucem_space* V, const double* x_d, double k0, int use_mu,
                        double* flux) {
   API_BEGIN

@@ -237,6 +237,8 @@ struct SolveStats {
 };
 void solve_system(Context* c, const System* sys, Coef k, const double2* b, double2* x,
                   const SolveOpts& o, SolveStats& st, double* hist_h, int64_t hist_cap);
+void precond_apply(Context* c, const System* sys, Coef k, double shift, int precision,
+                   bool eliminate_pec, const void* r, void* z);
 void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const SolveOpts& o,
                SolveStats& st, double* hist_h, int64_t hist_cap);
 

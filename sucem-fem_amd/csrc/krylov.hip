@@ -1188,6 +1188,41 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
   }
 }
 
+// z = M^-1 r with the auxiliary-space preconditioner of cs S + ct T + i c0 S0 (diagonals
+// built with the complex shift `shift`), in the given precision: the kernels COCG runs,
+// applied once (r and z complex64 for precision 32, complex128 for 64).
+void precond_apply(Context* c, const System* sys, Coef k, double shift, int precision,
+                   bool eliminate_pec, const void* r, void* z) {
+  SUCEM_CHECK(sys->st64.p, SUCEM_ERR_INVALID, "S and T must be assembled");
+  SUCEM_CHECK(precision == 32 || precision == 64, SUCEM_ERR_INVALID,
+              "precision must be 32 or 64");
+  SUCEM_CHECK(shift >= 0, SUCEM_ERR_INVALID, "shift must be >= 0");
+  Coef kp = k;
+  kp.ti = -shift * k.t;
+  auto run = [&](auto tag) {
+    using R = decltype(tag);
+    using C = typename C2<R>::T;
+    SysOp<R> op;
+    op.c = c;
+    op.n = sys->space->ndofs;
+    op.mask = eliminate_pec ? sys->space->pec.p : nullptr;
+    op.sys = sys;
+    op.k = k;
+    op.kp = kp;
+    op.aux = true;
+    const int64_t n = op.n, nl = op.nlag(), nn = op.nnod();
+    DBuf<C> dinv(n), w(nl), winv(nl), u(nn), vinv(nn);
+    op.dinv_lo(dinv.p);
+    op.winv_lo(winv.p);
+    op.vinv_lo(vinv.p);
+    RedSite none{};
+    op.precond_lo(winv.p, dinv.p, r, w.p, z, op_grid(c, n, 8), none, vinv.p, u.p);
+    c->sync();
+  };
+  if (precision == 32) run(float{});
+  else run(double{});
+}
+
 void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const SolveOpts& o,
                SolveStats& st, double* hist_h, int64_t hist_cap) {
   check_opts(o);

@@ -98,6 +98,8 @@ SIGNATURES = {
     "sucem_system_export": (ctypes.c_int, [_P, ctypes.c_uint, _P]),
     "sucem_system_destroy": (ctypes.c_int, [_P]),
     "sucem_system_apply": (ctypes.c_int, [_P, _P, _D, ctypes.c_int, ctypes.c_int, _I64, _P, _P]),
+    "sucem_system_precond_apply": (ctypes.c_int, [_P, _P, _D, _D, _D, _D, ctypes.c_int,
+                                                   ctypes.c_int, _P, _P]),
     "sucem_system_apply_coef": (ctypes.c_int, [_P, _P, _D, _D, _D, ctypes.c_int, ctypes.c_int,
                                                 _I64, _P, _P]),
     "sucem_system_tune": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, _P, _P]),
