@@ -104,11 +104,11 @@ def read_traffic(matrix_free, precision, cb):
 METRIC = "solves/s + p50/p99 latency at rel. residual 1e-8, FP32+IR vs plain FP64 GPU"
 
 
-def sweep_measure(n=40, order=1, nfreq=64, ndip=8, seed=0, precisions=(32, 64), width=8,
+def sweep_measure(n=40, order=1, nfreq=64, ndip=8, seed=0, precisions=(32, 64), width=0,
                   use_graph=True, sequential_sample=0, tol=1e-8):
     """The batched frequency sweep: an n^3-box cube (1 m, ABC, eps_r = 1), nfreq frequencies
     in 0.3-1.5 GHz x ndip seeded unit dipoles, all systems solved by the batched sweep
-    (``width`` systems per group, hipGraph replay); per precision: seconds, solves/s,
+    (``width`` systems per group, 0: autotuned; hipGraph replay); per precision: seconds, solves/s,
     iterations.  ``sequential_sample``: also time that many of the systems (spread over the
     frequencies) with the single-system solver, for the per-solve comparison."""
     import numpy as np
@@ -124,8 +124,9 @@ def sweep_measure(n=40, order=1, nfreq=64, ndip=8, seed=0, precisions=(32, 64), 
                        "unit dipoles (seed %d)" % (n, order, nfreq, ndip, seed),
            "ndofs": N, "systems": nb, "width": width, "graph": bool(use_graph)}
     for prec in precisions:
-        prob.solve_sweep(k0s[:width], B[:width * N], X[:width * N], precision=prec, tol=tol,
-                         width=width, use_graph=use_graph)          # warm-up (first use)
+        w0 = width or 8
+        prob.solve_sweep(k0s[:w0], B[:w0 * N], X[:w0 * N], precision=prec, tol=tol,
+                         width=w0, use_graph=use_graph)             # warm-up (first use)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         st = prob.solve_sweep(k0s, B, X, precision=prec, tol=tol, width=width,
@@ -136,6 +137,8 @@ def sweep_measure(n=40, order=1, nfreq=64, ndip=8, seed=0, precisions=(32, 64), 
         ok = all(q["status"] == 0 and q["rel_residual"] <= tol for q in st)
         key = "fp32ir" if prec == 32 else "fp64"
         out[key] = {"s": dt, "solves_per_s": nb / dt if ok else 0.0, "converged": ok,
+                    "widths": sorted(set(int(q["batch_width"]) for q in st)),
+                    "width_used": int(st[-1]["batch_width"]),
                     "iterations_mean": float(np.mean(its)), "iterations_max": int(max(its)),
                     "fp64_finish": int(sum(q["fp64_finish"] for q in st))}
         log("sweep FP%d: %.2f s, %.1f solves/s, iterations mean %.0f max %d" % (

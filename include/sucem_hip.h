@@ -123,7 +123,7 @@ typedef struct sucem_solve_stats {
   int fp64_finish;      /* precision 32: 1 when the FP32 cycles stopped lowering the true
                            residual (or diverged) and the solve was finished in FP64 from
                            their accepted solution */
-  int pad_;
+  int batch_width;      /* systems solved together in one group (batched sweep; 1 otherwise) */
 } sucem_solve_stats;
 
 const char* sucem_last_error(void);
@@ -275,7 +275,9 @@ int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const doubl
 int sucem_solve_driven_batch(sucem_ctx* ctx, sucem_system* sys, int64_t nb, const double* k0_h,
                              const double* B_d, double* X_d, const sucem_solve_opts* opts,
                              sucem_solve_stats* stats_h);
-/* The batched sweep: the nb systems are solved `width` (1, 2, 4 or 8) at a time, each
+/* The batched sweep: the nb systems are solved `width` (1, 2, 4 or 8; 0: autotuned -- the
+ * first systems are solved in timed groups of 2, 4 and 8 and the width with the least time
+ * per system-iteration takes the rest; stats_h[j].batch_width reports it) at a time, each
  * group as that many COCG recursions (auxiliary-space preconditioner, precision 64 or 32
  * with FP64 residual replacement per system) over vectors interleaved per DOF, whose
  * operator reads each patch's maps and cell coefficients once for the whole group; with

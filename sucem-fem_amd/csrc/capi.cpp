@@ -711,7 +711,7 @@ static void convert_stats(const SolveStats& s, sucem_solve_stats* out) {
   out->op_ms = s.op_ms;
   out->op_samples = s.op_samples;
   out->fp64_finish = s.fp32_failed;
-  out->pad_ = 0;
+  out->batch_width = s.batch_width;
 }
 
 int sucem_solve_driven(sucem_ctx* ctx, sucem_system* sys, double k0, const double* b_d,
@@ -741,7 +741,7 @@ int sucem_solve_driven_batch(sucem_ctx* ctx, sucem_system* sys, int64_t nb, cons
   if (!opts) o.precond = SUCEM_PRECOND_AUXILIARY;
   if (o.method == SUCEM_METHOD_COCG && o.precond == SUCEM_PRECOND_AUXILIARY &&
       sys_supports_sweep(c, &sys->s))
-    solve_system_sweep(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, 8, true,
+    solve_system_sweep(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, 0, true,
                        st.data());
   else
     solve_system_batch(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, st.data());

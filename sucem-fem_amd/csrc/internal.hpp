@@ -234,6 +234,7 @@ struct SolveStats {
   int64_t op_samples = 0;
   int stagnated = 0;        // stopped at the floor of the true residual (status > 0)
   int fp32_failed = 0;      // FP32 cycles stopped making progress; finished in FP64
+  int batch_width = 1;      // systems solved together with this one (batched sweep)
 };
 void solve_system(Context* c, const System* sys, Coef k, const double2* b, double2* x,
                   const SolveOpts& o, SolveStats& st, double* hist_h, int64_t hist_cap);

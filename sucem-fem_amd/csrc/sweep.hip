@@ -1262,8 +1262,8 @@ template <typename R, int ORDER>
 void sweep_dispatch(Context* c, System* sys, int64_t nb, const double* k0, const double2* B,
                     double2* X, const SolveOpts& o, int width, bool use_graph, SolveStats* st) {
   const int64_t n = sys->space->ndofs;
-  for (int64_t off = 0; off < nb; off += width) {
-    const int m = (int)std::min<int64_t>(width, nb - off);
+  // systems off .. off + m - 1 as one group of the next width up from m
+  auto group = [&](int64_t off, int m) {
     const double2* Bc = B + off * n;
     double2* Xc = X + off * n;
     const double* kc = k0 + off;
@@ -1272,7 +1272,34 @@ void sweep_dispatch(Context* c, System* sys, int64_t nb, const double* k0, const
     else if (m > 2) sweep_chunk<R, 4, ORDER>(c, sys, m, kc, Bc, Xc, o, use_graph, sc);
     else if (m > 1) sweep_chunk<R, 2, ORDER>(c, sys, m, kc, Bc, Xc, o, use_graph, sc);
     else sweep_chunk<R, 1, ORDER>(c, sys, m, kc, Bc, Xc, o, use_graph, sc);
+    for (int j = 0; j < m; ++j) sc[j].batch_width = m;
+  };
+  int64_t off = 0;
+  if (width == 0) {
+    // Autotuning: the first systems are solved in groups of 2, then 4, then 8 (real solves,
+    // each group timed on the host around its synchronised chunk), and the width with the
+    // least time per system-iteration -- the frozen lanes of systems that converged early
+    // included -- takes the rest.  The first group also pays the first-use set-up, which
+    // biases against the narrowest width only.
+    width = 8;
+    double best = 1e300;
+    for (int w : {2, 4, 8}) {
+      if (off + w > nb) break;
+      const auto t0 = std::chrono::steady_clock::now();
+      group(off, w);
+      const double sec =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      int64_t its = 0;
+      for (int j = 0; j < w; ++j) its += st[off + j].iterations;
+      const double per = sec / (double)std::max<int64_t>(its, 1);
+      if (per < best) {
+        best = per;
+        width = w;
+      }
+      off += w;
+    }
   }
+  for (; off < nb; off += width) group(off, (int)std::min<int64_t>(width, nb - off));
 }
 
 }  // namespace
@@ -1296,8 +1323,8 @@ void solve_system_sweep(Context* c, System* sys, int64_t nb, const double* k0, c
               "precision must be 32 or 64");
   SUCEM_CHECK(o.max_iter > 0 && o.check_every > 0 && o.check_every <= 256, SUCEM_ERR_INVALID,
               "bad iteration limits (check_every at most 256)");
-  SUCEM_CHECK(width == 1 || width == 2 || width == 4 || width == 8, SUCEM_ERR_INVALID,
-              "batch width must be 1, 2, 4 or 8");
+  SUCEM_CHECK(width == 0 || width == 1 || width == 2 || width == 4 || width == 8,
+              SUCEM_ERR_INVALID, "batch width must be 1, 2, 4 or 8 (0: autotuned)");
   const int order = sys->space->order;
   SUCEM_CHECK(order == 1 || order == 2, SUCEM_ERR_UNSUPPORTED, "orders 1 and 2 only");
   if (o.precision == 32) {

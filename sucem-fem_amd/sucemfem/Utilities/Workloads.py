@@ -130,10 +130,11 @@ class DeviceProblem(object):
         return [s.as_dict() for s in st]
 
     def solve_sweep(self, k0s, B, X, precision=32, tol=1e-8, max_iter=50000, check_every=16,
-                    width=8, use_graph=True, ru_delta=0.01, precond_shift=-1.0):
+                    width=0, use_graph=True, ru_delta=0.01, precond_shift=-1.0):
         """Solve len(k0s) driven systems together (``sucem_solve_driven_sweep``): ``width``
-        interleaved COCG recursions at a time sharing each operator pass, the iteration
-        replayed from a hipGraph.  B, X: (nb, N) complex128 device tensors."""
+        interleaved COCG recursions at a time sharing each operator pass (0: the width is
+        autotuned on the first systems), the iteration replayed from a hipGraph.  B, X:
+        (nb, N) complex128 device tensors."""
         import torch
         nb = len(k0s)
         self._check_vec(B, torch.complex128, nb)

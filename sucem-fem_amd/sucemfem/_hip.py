@@ -33,7 +33,7 @@ class SolveStats(ctypes.Structure):
                 ("rel_residual", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
                 ("spmv_lo", ctypes.c_int64), ("spmv_hi", ctypes.c_int64),
                 ("op_ms", ctypes.c_double), ("op_samples", ctypes.c_int64),
-                ("fp64_finish", ctypes.c_int), ("pad_", ctypes.c_int)]
+                ("fp64_finish", ctypes.c_int), ("batch_width", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}

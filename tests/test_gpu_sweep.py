@@ -72,13 +72,20 @@ def test_sweep_graph_width_and_sequential_agree():
     nb, n = len(k0s), prob.ndofs
     runs = {}
     for key, width, graph in (("g8", 8, True), ("e8", 8, False), ("g4", 4, True),
-                              ("g1", 1, True)):
+                              ("g1", 1, True), ("a0", 0, True)):
         X = torch.zeros_like(B)
         st = prob.solve_sweep(k0s, B, X, precision=64, tol=1e-11, width=width, use_graph=graph)
         assert all(s["status"] == 0 for s in st), (key, _brief(st))
+        if width:
+            assert all(s["batch_width"] == min(width, nb) for s in st), (key, _brief(st))
+        else:
+            # autotuned: timed groups of 2 and 4 (8 do not fit in the 8 systems), then the
+            # faster width for the remaining 2
+            assert [s["batch_width"] for s in st[:6]] == [2, 2, 4, 4, 4, 4], _brief(st)
+            assert st[6]["batch_width"] == st[7]["batch_width"] == 2
         runs[key] = X.reshape(nb, n).cpu().numpy()
     assert np.array_equal(runs["g8"], runs["e8"])
-    for key in ("g4", "g1"):
+    for key in ("g4", "g1", "a0"):
         d = np.linalg.norm(runs[key] - runs["g8"], axis=1) / np.linalg.norm(runs["g8"], axis=1)
         assert d.max() < 1e-9, (key, d)
     for j in range(nb):
