@@ -205,7 +205,7 @@ def eigen_measure(n=(120, 60, 90), nev=20, sigma=1.1, seed=0, precisions=(32, 64
 
 
 def inner_measure(n=(120, 60, 90), sigma=1.1, seed=0, precisions=(64, 32), tol=1e-9,
-                  max_iter=40000):
+                  max_iter=40000, operator=None):
     """One inner solve of the eigen workload: (S - sigma T) x = b on the dielectric-block
     cavity with the PEC DOFs eliminated, auxiliary-space COCG to ``tol``, b a seeded
     normal vector; per precision: seconds, iterations, ms per iteration and per operator
@@ -219,6 +219,10 @@ def inner_measure(n=(120, 60, 90), sigma=1.1, seed=0, precisions=(64, 32), tol=1
                        "(S - %g T) x = b, PEC eliminated, b seeded normal (seed %d), tol %g"
                        % (tuple(n) + (sigma, seed, tol)),
            "ndofs": prob.ndofs, "nnz": prob.nnz, "setup_s": time.perf_counter() - t0}
+    if operator is not None:
+        for prec in (32, 64):
+            prob.set_spmv(prec, *operator)
+        out["operator"] = list(operator)
     g = torch.Generator(device="cuda").manual_seed(seed)
     b = torch.randn(prob.ndofs, dtype=torch.complex128, device="cuda", generator=g)
     for prec in precisions:
@@ -345,7 +349,11 @@ def main():
     if a.workload == "inner":
         nn = tuple(int(v) for v in a.eig_n.split(","))
         precs = tuple(int(v) for v in a.eig_prec.split(","))
-        r = inner_measure(n=nn, precisions=precs)
+        fixed = {"assembled": (0, 16, 4, 2048), "cells": (1, 0, 1, 2048),
+                 "patch128": (2, 0, 128, 4096), "patch192": (2, 0, 192, 4096),
+                 "patch256": (2, 0, 256, 4096)}
+        r = inner_measure(n=nn, precisions=precs,
+                          operator=None if a.operator == "auto" else fixed[a.operator])
         key = "fp32ir" if 32 in precs else "fp64"
         v = 1.0 / r[key]["s"]
         print(json.dumps(workload_line("inner", a, r, v, "solves/s", 1e3 * r[key]["s"],
