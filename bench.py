@@ -162,10 +162,11 @@ def sweep_measure(n=40, order=1, nfreq=64, ndip=8, seed=0, precisions=(32, 64), 
 
 
 def eigen_measure(n=(120, 60, 90), nev=20, sigma=1.1, seed=0, precisions=(32, 64), tol=1e-8,
-                  inner_tol=1e-9):
+                  inner_tol=1e-9, block=0):
     """The dielectric-block cavity: 1.0 x 0.5 x 0.75 m PEC box of n boxes, a seeded eps_r = 6
     block (10-30 % of the volume), order 2, the nev lowest k^2 above sigma; per inner
-    precision: seconds, Lanczos steps, inner iterations, FP64 residuals."""
+    precision: seconds, Lanczos steps, inner iterations, FP64 residuals.  ``block``: Lanczos
+    block size (0: the solver's choice, 2 with paired inner solves on these meshes)."""
     import numpy as np
     import torch
     from sucemfem.Utilities.Workloads import dielectric_cavity
@@ -181,21 +182,24 @@ def eigen_measure(n=(120, 60, 90), nev=20, sigma=1.1, seed=0, precisions=(32, 64
     ks = {}
     for prec in precisions:
         t0 = time.perf_counter()
-        k2, resid, st = prob.eigs(sigma, nev, tol=tol, inner_tol=inner_tol, inner_precision=prec)
+        k2, resid, st = prob.eigs(sigma, nev, tol=tol, inner_tol=inner_tol, inner_precision=prec,
+                                  block=block)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         key = "fp32ir" if prec == 32 else "fp64"
         ks[key] = k2
         out[key] = {"s": dt, "solves_per_s": 1.0 / dt if st["status"] == 0 else 0.0,
-                    "status": st["status"], "lanczos_steps": st["lanczos_steps"],
+                    "status": st["status"], "block": st["block"],
+                    "lanczos_steps": st["lanczos_steps"],
                     "inner_iterations": st["inner_iterations"],
                     "inner_stagnated": st["inner_stagnated"],
                     "inner_failures": st["inner_failures"],
                     "inner_fp64_from": st["inner_fp64_from"],
                     "max_ritz_residual": st["max_ritz_residual"],
                     "max_mode_residual": float(np.max(resid)), "k2": [float(v) for v in k2]}
-        log("eigen FP%d inner: %.2f s, %d Lanczos steps, %d inner iterations, status %d" % (
-            prec, dt, st["lanczos_steps"], st["inner_iterations"], st["status"]))
+        log("eigen FP%d inner, block %d: %.2f s, %d Lanczos vectors, %d inner iterations, "
+            "status %d" % (prec, st["block"], dt, st["lanczos_steps"], st["inner_iterations"],
+                           st["status"]))
     if len(ks) == 2:
         a, b = ks["fp32ir"], ks["fp64"]
         out["max_rel_k2_diff_fp32ir_vs_fp64"] = float(np.max(np.abs(a - b) / np.abs(b)))
@@ -313,6 +317,8 @@ def main():
     ap.add_argument("--eig-n", default="120,60,90", help="eigen workload boxes per axis")
     ap.add_argument("--eig-prec", default="32,64",
                     help="eigen workload inner precisions, comma-separated (32 = FP32+IR)")
+    ap.add_argument("--eig-block", type=int, default=0, choices=[0, 1, 2],
+                    help="eigen workload Lanczos block size (0: the solver's choice)")
     ap.add_argument("--serve-count", type=int, default=16)
     ap.add_argument("--serve-seed", type=int, default=0)
     ap.add_argument("--no-extras", action="store_true",
@@ -340,7 +346,7 @@ def main():
     if a.workload == "eigen":
         nn = tuple(int(v) for v in a.eig_n.split(","))
         precs = tuple(int(v) for v in a.eig_prec.split(","))
-        r = eigen_measure(n=nn, precisions=precs)
+        r = eigen_measure(n=nn, precisions=precs, block=a.eig_block)
         key = "fp32ir" if 32 in precs else "fp64"
         v = r[key]["solves_per_s"]
         print(json.dumps(workload_line("eigen", a, r, v, "solves/s", 1e3 * r[key]["s"],

@@ -69,7 +69,13 @@ typedef struct sucem_solve_opts {
   double tol;         /* relative residual ||b - A x|| / ||b|| (FP64) */
   double ru_delta;    /* FP32: reliable update when the residual fell by this factor */
   int gmres_restart;  /* GMRES: Arnoldi steps per cycle (SciPy's default 20) */
-  int pad_;
+  int paired;         /* 1: b and x hold two real systems, one in the real and one in the
+                         imaginary parts, solved as two independent real COCG recursions that
+                         share every operator and preconditioner pass (sucem_solve_system
+                         with c0 = 0 and precond_shift 0, gradient or auxiliary-space
+                         preconditioner, patch operator; the residual reported is the larger
+                         of the two relative residuals).  No counterpart in the reference:
+                         its real eigen pencils are solved one right-hand side at a time. */
   double precond_shift; /* complex shift beta of the operator the Jacobi / gradient /
                            auxiliary diagonals are built from: cs S + ct (1 - i beta) T +
                            i c0 S0 (the solved operator is unchanged).  < 0 (the default):
@@ -87,6 +93,10 @@ typedef struct sucem_eig_opts {
   double tol;           /* relative Ritz residual of nu = 1/(k^2 - sigma) */
   double inner_tol;     /* relative residual of the inner (S - sigma T) solves */
   uint64_t seed;        /* start vector */
+  int block;            /* Lanczos block size: 1, 2 (each pair of inner solves runs as one
+                           paired solve, see sucem_solve_opts.paired), 0 (default): 2 where the
+                           paired solve is available (patch operator), else 1 */
+  int pad_;
 } sucem_eig_opts;
 
 typedef struct sucem_eig_stats {
@@ -99,7 +109,8 @@ typedef struct sucem_eig_stats {
   int inner_fp64_from;  /* inner_precision 32: the Lanczos step from which the inner solves
                            ran in FP64 because two in a row had their FP32 cycles fail and
                            were finished in FP64 (0: all steps ran FP32 cycles) */
-  int pad_;
+  int block;            /* the Lanczos block size used (lanczos_steps counts basis vectors:
+                           lanczos_steps / block inner solves) */
   int64_t inner_iterations;
   double max_ritz_residual;
   double elapsed_ms;

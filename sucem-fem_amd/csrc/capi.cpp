@@ -662,7 +662,7 @@ void sucem_solve_opts_default(sucem_solve_opts* o) {
   o->tol = 1e-8;
   o->ru_delta = 0.01;
   o->gmres_restart = 20;
-  o->pad_ = 0;
+  o->paired = 0;
   o->precond_shift = -1.0;
 }
 
@@ -687,11 +687,13 @@ static SolveOpts convert(const sucem_solve_opts* o, double auto_shift) {
     s.tol = o->tol;
     s.ru_delta = o->ru_delta;
     s.restart = o->gmres_restart;
+    s.paired = o->paired;
   }
   SUCEM_CHECK(s.tol > 0 && s.ru_delta > 0 && s.ru_delta < 1, SUCEM_ERR_INVALID,
               "tol must be > 0 and ru_delta in (0, 1)");
   SUCEM_CHECK(s.precond_shift >= 0 && s.precond_shift <= 10, SUCEM_ERR_INVALID,
               "precond_shift must be in [0, 10] (or negative for the default)");
+  SUCEM_CHECK(s.paired == 0 || s.paired == 1, SUCEM_ERR_INVALID, "paired must be 0 or 1");
   SUCEM_CHECK(s.precond == SUCEM_PRECOND_NONE || s.precond == SUCEM_PRECOND_JACOBI ||
                   s.precond == SUCEM_PRECOND_GRADIENT || s.precond == SUCEM_PRECOND_AUXILIARY,
               SUCEM_ERR_UNSUPPORTED, "unknown preconditioner");
@@ -739,6 +741,7 @@ int sucem_solve_driven_batch(sucem_ctx* ctx, sucem_system* sys, int64_t nb, cons
   std::vector<SolveStats> st(nb);
   SolveOpts o = convert(opts, kDrivenShift);
   if (!opts) o.precond = SUCEM_PRECOND_AUXILIARY;
+  SUCEM_CHECK(!o.paired, SUCEM_ERR_UNSUPPORTED, "driven systems are complex: no paired solves");
   if (o.method == SUCEM_METHOD_COCG && o.precond == SUCEM_PRECOND_AUXILIARY &&
       sys_supports_sweep(c, &sys->s))
     solve_system_sweep(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, 0, true,
@@ -760,6 +763,7 @@ int sucem_solve_driven_sweep(sucem_ctx* ctx, sucem_system* sys, int64_t nb, cons
   c->use();
   SolveOpts o = convert(opts, kDrivenShift);
   if (!opts) o.precond = SUCEM_PRECOND_AUXILIARY;
+  SUCEM_CHECK(!o.paired, SUCEM_ERR_UNSUPPORTED, "driven systems are complex: no paired solves");
   std::vector<SolveStats> st(nb);
   solve_system_sweep(c, &sys->s, nb, k0_h, (const double2*)B_d, (double2*)X_d, o, width,
                      use_graph != 0, st.data());
@@ -840,6 +844,8 @@ void sucem_eig_opts_default(sucem_eig_opts* o) {
   o->tol = d.tol;
   o->inner_tol = d.inner_tol;
   o->seed = d.seed;
+  o->block = d.block;
+  o->pad_ = 0;
 }
 
 int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,
@@ -860,7 +866,10 @@ int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,
     o.tol = opts->tol;
     o.inner_tol = opts->inner_tol;
     o.seed = opts->seed;
+    o.block = opts->block;
   }
+  SUCEM_CHECK(o.block >= 0 && o.block <= 2, SUCEM_ERR_INVALID,
+              "Lanczos block size must be 0 (automatic), 1 or 2");
   SUCEM_CHECK(o.inner_precision == 32 || o.inner_precision == 64, SUCEM_ERR_INVALID,
               "inner precision must be 32 or 64");
   SUCEM_CHECK(o.tol > 0 && o.inner_tol > 0 && o.max_restarts >= 0 && o.inner_max_iter > 0,
@@ -875,7 +884,7 @@ int sucem_eigs(sucem_ctx* ctx, sucem_system* sys, double sigma, int nev,
     stats->inner_failures = st.inner_failures;
     stats->inner_stagnated = st.inner_stagnated;
     stats->inner_fp64_from = st.inner_fp64_from;
-    stats->pad_ = 0;
+    stats->block = st.block;
     stats->inner_iterations = st.inner_iterations;
     stats->max_ritz_residual = st.max_ritz_residual;
     stats->elapsed_ms = st.elapsed_ms;

@@ -229,9 +229,14 @@ __global__ __launch_bounds__(TB) void k_aux_z(int n, int G, int nv, int nE, R in
     z[i] = zi;
     if (reduce) {
       const double2 rd = to_d(ri), zd = to_d(zi);
-      d1.x += rd.x * zd.x - rd.y * zd.y;
-      d1.y += rd.x * zd.y + rd.y * zd.x;
-      d2.x += rd.x * rd.x + rd.y * rd.y;
+      if (rs.paired) {   // two real systems in the real and imaginary parts
+        dot_acc(d1, rd, zd, true);
+        sq_acc(d2, rd, true);
+      } else {
+        d1.x += rd.x * zd.x - rd.y * zd.y;
+        d1.y += rd.x * zd.y + rd.y * zd.x;
+        d2.x += rd.x * rd.x + rd.y * rd.y;
+      }
     }
   }
   if (reduce) reduce_finalize<TB>(rs, G, d1, d2);
@@ -331,9 +336,16 @@ __global__ __launch_bounds__(TB) void k_aux_z2(int n2, int G, int nv, int nE, R 
     }
     if (reduce) {
       const double2 ra = to_d(r0), za = to_d(z0), rb = to_d(r1), zb = to_d(z1);
-      d1.x += ra.x * za.x - ra.y * za.y + rb.x * zb.x - rb.y * zb.y;
-      d1.y += ra.x * za.y + ra.y * za.x + rb.x * zb.y + rb.y * zb.x;
-      d2.x += ra.x * ra.x + ra.y * ra.y + rb.x * rb.x + rb.y * rb.y;
+      if (rs.paired) {   // two real systems in the real and imaginary parts
+        d1.x += ra.x * za.x + rb.x * zb.x;
+        d1.y += ra.y * za.y + rb.y * zb.y;
+        d2.x += ra.x * ra.x + rb.x * rb.x;
+        d2.y += ra.y * ra.y + rb.y * rb.y;
+      } else {
+        d1.x += ra.x * za.x - ra.y * za.y + rb.x * zb.x - rb.y * zb.y;
+        d1.y += ra.x * za.y + ra.y * za.x + rb.x * zb.y + rb.y * zb.x;
+        d2.x += ra.x * ra.x + ra.y * ra.y + rb.x * rb.x + rb.y * rb.y;
+      }
     }
   }
   if (reduce) reduce_finalize<TB>(rs, G, d1, d2);

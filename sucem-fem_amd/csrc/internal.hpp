@@ -225,6 +225,11 @@ struct SolveOpts {
   // cs S + ct (1 - i beta) T + i c0 S0 (driven Helmholtz systems converge faster);
   // < 0: not resolved yet (the entry points set 0.5 for driven systems, 0 otherwise)
   double precond_shift = -1.0;
+  // b and x hold two real systems, one in the real and one in the imaginary parts, solved
+  // as two independent real COCG recursions sharing each operator and preconditioner pass
+  // (real coefficients only: c0 = 0, no preconditioner shift; the patch operator; the
+  // residuals reported are the larger of the two relative residuals)
+  int paired = 0;
 };
 struct SolveStats {
   int status = 0, iterations = 0, reliable_updates = 0;
@@ -272,14 +277,18 @@ struct EigOpts {
   double tol = 1e-10;        // relative Ritz residual of nu
   double inner_tol = 1e-11;  // inner (S - sigma T) solves
   unsigned long long seed = 12345;
+  int block = 0;             // Lanczos block size 1 or 2 (0: 2 where paired inner solves run)
 };
 struct EigStats {
   int status = 0, nconv = 0, restarts = 0, lanczos_steps = 0, inner_failures = 0;
   int inner_stagnated = 0;
   int inner_fp64_from = 0;   // precision 32: Lanczos step from which inner solves ran in FP64
+  int block = 1;             // the Lanczos block size used
   int64_t inner_iterations = 0;
   double max_ritz_residual = 0, elapsed_ms = 0;
 };
+// whether paired solves (SolveOpts::paired) run on this system in this precision
+bool paired_solves_available(Context* c, const System* sys, int precision);
 void eigs_system(Context* c, const System* sys, double sigma, int nev, const EigOpts& o,
                  double* lam_h, double* resid_h, double2* vecs_d, EigStats& st);
 

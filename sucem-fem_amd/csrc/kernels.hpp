@@ -37,6 +37,39 @@ __device__ __forceinline__ double2 cmuld(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Paired recursions: two real systems packed in one complex vector (system 0 in the real
+// parts, system 1 in the imaginary parts) run as two independent real recursions over the
+// same operator and preconditioner passes (linear with real coefficients, so the two parts
+// never mix).  Their products are taken per part, (sum a.x b.x, sum a.y b.y), and their
+// scalars (a, b) scale the parts separately.
+template <typename T>
+__device__ __forceinline__ T pmul(T a, T b) {
+  return mk(a.x * b.x, a.y * b.y);
+}
+// per part a / b, 0 where b == 0
+__device__ __forceinline__ double2 pdiv0(double2 a, double2 b) {
+  return make_double2(b.x != 0.0 ? a.x / b.x : 0.0, b.y != 0.0 ? a.y / b.y : 0.0);
+}
+// a.b into d: the complex bilinear product, or per part when paired
+__device__ __forceinline__ void dot_acc(double2& d, double2 a, double2 b, bool paired) {
+  if (paired) {
+    d.x += a.x * b.x;
+    d.y += a.y * b.y;
+  } else {
+    d.x += a.x * b.x - a.y * b.y;
+    d.y += a.x * b.y + a.y * b.x;
+  }
+}
+// |a|^2 into d.x, or per part (d.x, d.y) when paired
+__device__ __forceinline__ void sq_acc(double2& d, double2 a, bool paired) {
+  if (paired) {
+    d.x += a.x * a.x;
+    d.y += a.y * a.y;
+  } else {
+    d.x += a.x * a.x + a.y * a.y;
+  }
+}
+
 // Wavefront-then-block sum of a double2; every thread of the block gets the result.
 // Deterministic: fixed reduction tree.
 template <int TB>
@@ -84,7 +117,7 @@ enum ScalarSlot {
   S_OMEGA = 6,    // complex (BiCGStab)
   S_RR = 8,       // |r|^2 of the recursion
   S_TRUE = 9,     // |b - A x|^2 (FP64)
-  S_BNORM = 10,   // |b|^2
+  S_BNORM = 10,   // |b|^2 (paired recursions: the two systems' |b|^2 at 10 and 11)
   S_RHO_NEXT = 12,  // complex: rho for the next alpha (equal to S_RHO except between the
                     // fused p update and the next x/r update of patch-operator COCG)
   S_COUNT = 14
@@ -112,7 +145,15 @@ struct RedSite {
   double* hist;       // residual history (nullable)
   int mode;
   int it;
+  int paired = 0;     // two real systems packed as real and imaginary parts (see pmul)
 };
+
+// Paired recursions: the larger of the two systems' squared residual norms s.x, s.y, each
+// relative to its |b|^2 (a system with b = 0 counts as solved).
+__device__ __forceinline__ double prel2(const double* sc, double2 s) {
+  const double b0 = sc[S_BNORM], b1 = sc[S_BNORM + 1];
+  return fmax(b0 > 0.0 ? s.x / b0 : 0.0, b1 > 0.0 ? s.y / b1 : 0.0);
+}
 
 __device__ __forceinline__ void finalize(const RedSite& rs, double2 s0, double2 s1) {
   double* sc = rs.scal;
@@ -132,13 +173,14 @@ __device__ __forceinline__ void finalize(const RedSite& rs, double2 s0, double2 
       sc[S_RR] = s1.x;
       if (rs.hist) rs.hist[rs.it] = sqrt(s1.x);
       break;
-    case FIN_RESID:
-      sc[S_TRUE] = s0.x;
+    case FIN_RESID:   // paired: S_TRUE holds the larger relative square (see prel2)
+      sc[S_TRUE] = rs.paired ? prel2(sc, s0) : s0.x;
       sc[S_RHO] = s1.x; sc[S_RHO + 1] = s1.y;
       sc[S_RHO_NEXT] = s1.x; sc[S_RHO_NEXT + 1] = s1.y;
       break;
     case FIN_NORM:
       sc[S_BNORM] = s0.x;
+      sc[S_BNORM + 1] = s0.y;   // paired: system 1 (0 otherwise)
       break;
     case FIN_SET_RHO:
       sc[S_RHO] = s0.x; sc[S_RHO + 1] = s0.y;

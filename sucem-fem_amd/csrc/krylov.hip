@@ -71,7 +71,7 @@ __global__ __launch_bounds__(TB) void k_norm2(int n, int G, const double2* b, co
   Range rr = block_rows(n, G);
   double2 s = make_double2(0, 0);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB)
-    if (!(mask && mask[i])) s.x += b[i].x * b[i].x + b[i].y * b[i].y;
+    if (!(mask && mask[i])) sq_acc(s, b[i], rs.paired);
   reduce_finalize<TB>(rs, G, s, make_double2(0, 0));
 }
 
@@ -86,14 +86,14 @@ __global__ __launch_bounds__(TB) void k_residual(int n, int G, const double2* b,
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
     double2 v = make_double2(b[i].x - ax[i].x, b[i].y - ax[i].y);
     if (mask && mask[i]) v = make_double2(0, 0);
-    tr.x += v.x * v.x + v.y * v.y;
+    sq_acc(tr, v, rs.paired);
     const C rl = from_d<C>(v);
     r[i] = rl;
     const double2 rd = to_d(rl);
     if (cocg) {
+      // (paired systems have real diagonals: the complex product scales each part)
       const double2 z = to_d(cmul(dinv[i], rl));
-      rho.x += rd.x * z.x - rd.y * z.y;
-      rho.y += rd.x * z.y + rd.y * z.x;
+      dot_acc(rho, rd, z, rs.paired);
     } else {
       rho.x += rd.x * rd.x + rd.y * rd.y;
     }
@@ -220,7 +220,8 @@ __global__ __launch_bounds__(TB) void k_cocg_xr_def_a(int n, int G, int Gd, RedS
   }
   s0 = block_sum<TB>(s0);
   const double2 rho = scal2(rs.scal, S_RHO_NEXT);
-  const double2 al = cdiv0(rho, s0);
+  const bool pr = rs.paired;
+  const double2 al = pr ? pdiv0(rho, s0) : cdiv0(rho, s0);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     rs.scal[S_ALPHA] = al.x;
     rs.scal[S_ALPHA + 1] = al.y;
@@ -244,7 +245,8 @@ __global__ __launch_bounds__(TB) void k_cocg_xr_def_a(int n, int G, int Gd, RedS
       }
       if (mask && mask[i]) qi = mk(decltype(qi.x)(0), decltype(qi.x)(0));
     }
-    const C ap = cmul(a, p[i]), aq = cmul(a, qi);
+    const C pi = p[i];
+    const C ap = pr ? pmul(a, pi) : cmul(a, pi), aq = pr ? pmul(a, qi) : cmul(a, qi);
     C xi = x[i], ri = r[i];
     xi.x += ap.x; xi.y += ap.y;
     ri.x -= aq.x; ri.y -= aq.y;
@@ -267,7 +269,8 @@ __global__ __launch_bounds__(TB) void k_cocg_pz_b(int n, int G, int Gr, RedSite 
     s0.y += t.y;
   }
   s0 = block_sum<TB>(s0);
-  const double2 be = cdiv0(s0, scal2(rs.scal, S_RHO));
+  const bool pr = rs.paired;
+  const double2 be = pr ? pdiv0(s0, scal2(rs.scal, S_RHO)) : cdiv0(s0, scal2(rs.scal, S_RHO));
   if (blockIdx.x == 0) {
     for (int k = threadIdx.x; k < Gr; k += TB) {
       const double2 t = rs.part[Gr + k];
@@ -279,14 +282,16 @@ __global__ __launch_bounds__(TB) void k_cocg_pz_b(int n, int G, int Gr, RedSite 
       double* sc = rs.scal;
       sc[S_BETA] = be.x; sc[S_BETA + 1] = be.y;
       sc[S_RHO_NEXT] = s0.x; sc[S_RHO_NEXT + 1] = s0.y;
-      sc[S_RR] = s1.x;
-      if (rs.hist) rs.hist[rs.it] = sqrt(s1.x);
+      // paired: the larger of the two relative residuals, squared (see prel2)
+      sc[S_RR] = pr ? prel2(sc, s1) : s1.x;
+      if (rs.hist) rs.hist[rs.it] = sqrt(sc[S_RR]);
     }
   }
   const C b = from_d<C>(be);
   Range rr = block_rows(n, G);
   for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
-    const C bp = cmul(b, p[i]), zi = z[i];
+    const C pi = p[i], zi = z[i];
+    const C bp = pr ? pmul(b, pi) : cmul(b, pi);
     p[i] = mk(zi.x + bp.x, zi.y + bp.y);
   }
 }
@@ -548,6 +553,13 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
               "the gradient and auxiliary-space preconditioners need an assembled Nedelec "
               "system");
   const int64_t nn = aux ? op.nnod() : 0;
+  // COCG with the patch operator: the shared rows' partial sums are added by the x/r update,
+  // which also forms alpha from the operator's Gd block partials
+  const int Gd = (cocg && grad) ? op.deferred_parts() : 0;
+  const bool paired = o.paired != 0;
+  SUCEM_CHECK(!paired || Gd > 0, SUCEM_ERR_UNSUPPORTED,
+              "paired solves run COCG with the gradient or auxiliary-space preconditioner "
+              "and the patch operator");
   auto t0 = std::chrono::steady_clock::now();
   st = SolveStats();
   const char* lg = std::getenv("SUCEM_LOG");   // 3: one line per residual replacement
@@ -569,6 +581,7 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     r.hist = hist.p;
     r.mode = mode;
     r.it = it;
+    r.paired = paired ? 1 : 0;
     return r;
   };
 
@@ -595,7 +608,13 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
   if (lo) SUCEM_LAUNCH((k_zero_c<C>), grid_for(n, TB), TB, 0, c->stream, (int64_t)n, xlo);
   SUCEM_LAUNCH(k_norm2, G, TB, 0, c->stream, n, G, b, op.mask, site(SITE_NORM, FIN_NORM));
   finish_reduction(c, site(SITE_NORM, FIN_NORM), G);
-  const double bnorm = std::sqrt(read_scalar(c, scal.p + S_BNORM));
+  double bnorm = std::sqrt(read_scalar(c, scal.p + S_BNORM));
+  if (paired) {
+    // the kernels report the larger of the two systems' residuals, each relative to its
+    // own |b| (prel2), so the relative figures below divide by 1
+    const double b1 = std::sqrt(read_scalar(c, scal.p + S_BNORM + 1));
+    bnorm = (bnorm == 0.0 && b1 == 0.0) ? 0.0 : 1.0;
+  }
   if (bnorm == 0.0) {
     st.status = 0;
     st.rel_residual = 0;
@@ -653,9 +672,6 @@ void krylov(Op& op, const double2* b, double2* x, const SolveOpts& o, SolveStats
     if (sample) timer.stop(c);
     finish_reduction(c, rs, op.flo.parts());
   };
-  // COCG with the patch operator: the shared rows' partial sums are added by the x/r update,
-  // which also forms alpha from the operator's Gd block partials
-  const int Gd = (cocg && grad) ? op.deferred_parts() : 0;
   // grid of the preconditioner's z kernel in that loop (4 blocks per CU: 23.4 against
   // 24.7 us per FP32 application with the 8 of the other vector kernels, which lose more
   // at 4: the x/r update 34.3 against 28.5 us)
@@ -1122,6 +1138,9 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
   // the preconditioner's diagonals come from cs S + ct (1 - i beta) T + i c0 S0
   Coef kp = k;
   kp.ti = -std::max(0.0, o.precond_shift) * k.t;
+  SUCEM_CHECK(!o.paired || (o.method == SUCEM_METHOD_COCG && k.s0 == 0.0 && kp.ti == 0.0),
+              SUCEM_ERR_UNSUPPORTED,
+              "paired solves run COCG on real coefficients (c0 = 0, no preconditioner shift)");
   const Space* s = sys->space;
   auto fill = [&](Op& op) {
     op.c = c;
@@ -1188,6 +1207,15 @@ void solve_system(Context* c, const System* sys, Coef k, const double2* b, doubl
   }
 }
 
+// Paired solves run the patch operator's COCG form: in the working precision and, for FP32
+// solves, in the FP64 finish too.
+bool paired_solves_available(Context* c, const System* sys, int precision) {
+  if (sys->space->order != 1 && sys->space->order != 2) return false;
+  const bool p64 = pt_deferred_parts(sys, sys_cfg<double>(c, sys)) > 0;
+  if (precision == 64) return p64;
+  return p64 && pt_deferred_parts(sys, sys_cfg<float>(c, sys)) > 0;
+}
+
 // z = M^-1 r with the auxiliary-space preconditioner of cs S + ct T + i c0 S0 (diagonals
 // built with the complex shift `shift`), in the given precision: the kernels COCG runs,
 // applied once (r and z complex64 for precision 32, complex128 for 64).
@@ -1226,6 +1254,7 @@ void precond_apply(Context* c, const System* sys, Coef k, double shift, int prec
 void solve_csr(Context* c, const Csr* A, const double2* b, double2* x, const SolveOpts& o,
                SolveStats& st, double* hist_h, int64_t hist_cap) {
   check_opts(o);
+  SUCEM_CHECK(!o.paired, SUCEM_ERR_UNSUPPORTED, "paired solves run on assembled systems");
   auto fill = [&](Op& op) {
     op.c = c;
     op.n = A->n;

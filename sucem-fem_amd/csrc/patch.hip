@@ -266,10 +266,16 @@ __device__ __forceinline__ void block_partials(const RedSite& rs, int Gt, int at
   }
 }
 
+// EPI 1: w.y (bilinear), EPI 2: conj(w).y, EPI 4: w.y per part (paired systems)
 template <int EPI, typename C>
 __device__ __forceinline__ void epilogue(double2& d1, double2& d2, C v, C wv) {
   const double2 yd = to_d(v);
   double2 wd = to_d(wv);
+  if constexpr (EPI == 4) {
+    d1.x += wd.x * yd.x;
+    d1.y += wd.y * yd.y;
+    return;
+  }
   if (EPI == 2) wd.y = -wd.y;
   d1.x += wd.x * yd.x - wd.y * yd.y;
   d1.y += wd.x * yd.y + wd.y * yd.x;
@@ -278,7 +284,8 @@ __device__ __forceinline__ void epilogue(double2& d1, double2& d2, C v, C wv) {
 
 // One workgroup per patch (XCD-aware order: consecutive patches share x entries and
 // should share an L2); Gp = grid (a multiple of 8), Gt = all block partials of the
-// operator (Gp + the grid of k_mf_psum).
+// operator (Gp + the grid of k_mf_psum).  EPI 3 (COCG, shared rows left as partial sums)
+// reduces p.Ap; EPI 4 is EPI 3 for two paired real systems (per-part products).
 template <typename R, int ORDER, int NFD, int EPI, int CB>
 __global__ __launch_bounds__(CB) void k_mf_patch(
     int64_t np, int Gp, int Gt, int64_t nc, int64_t nbf, int lds_main, const R* __restrict__ coef,
@@ -292,6 +299,8 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
     typename C2<R>::T* __restrict__ part, const typename C2<R>::T* __restrict__ w,
     RedSite rs) {
   using C = typename C2<R>::T;
+  constexpr bool E3 = EPI == 3 || EPI == 4;   // COCG form (EPI 4: paired systems)
+  constexpr bool PR = EPI == 4;
   constexpr int NLOC = MfTab<ORDER>::NLOC;
   constexpr int KK = NFD * NFD;
   extern __shared__ double2 smem_[];
@@ -352,7 +361,7 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
 #pragma unroll
       for (int i = 0; i < NLOC; ++i) acc[i] = mk(R(0), R(0));
       element_product<R, ORDER>(a, xl, acc, std::make_integer_sequence<int, MfTab<ORDER>::NP>{});
-      if constexpr (EPI == 3 && sizeof(R) == 8) {
+      if constexpr (E3 && sizeof(R) == 8) {
         // p.Ap as the sum of the element quadratic forms x_c^T K_c x_c (w = x in COCG;
         // x is zero on masked rows there, so the masked rows add nothing either way).
         // FP64 only: in FP32 the extra registers cost a wave per SIMD, and the sums of
@@ -360,8 +369,13 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
 #pragma unroll
         for (int i = 0; i < NLOC; ++i) {
           const double2 xd = to_d(xl[i]), yd = to_d(acc[i]);
-          d1.x += xd.x * yd.x - xd.y * yd.y;
-          d1.y += xd.x * yd.y + xd.y * yd.x;
+          if constexpr (PR) {
+            d1.x += xd.x * yd.x;
+            d1.y += xd.y * yd.y;
+          } else {
+            d1.x += xd.x * yd.x - xd.y * yd.y;
+            d1.y += xd.x * yd.y + xd.y * yd.x;
+          }
         }
       }
     }
@@ -392,11 +406,16 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
           v.y += m * xb.y;
         }
         stage[id] = mk(-c0 * v.y, c0 * v.x);
-        if constexpr (EPI == 3 && sizeof(R) == 8) {
+        if constexpr (E3 && sizeof(R) == 8) {
           const C xa = sm[lp[(int64_t)face_dofs[lf * NFD + a] * nc + rk] & 0xffffu];
           const double2 xd = to_d(xa), yd = to_d(stage[id]);
-          d1.x += xd.x * yd.x - xd.y * yd.y;
-          d1.y += xd.x * yd.y + xd.y * yd.x;
+          if constexpr (PR) {
+            d1.x += xd.x * yd.x;
+            d1.y += xd.y * yd.y;
+          } else {
+            d1.x += xd.x * yd.x - xd.y * yd.y;
+            d1.y += xd.x * yd.y + xd.y * yd.x;
+          }
         }
       }
     }
@@ -434,12 +453,12 @@ __global__ __launch_bounds__(CB) void k_mf_patch(
           if (out[q] >= 0) {
             part[out[q]] = v;
             // FP32: x.y over the partial sums (linear in y; x is zero on masked rows)
-            if constexpr (EPI == 3 && sizeof(R) == 4) epilogue<1>(d1, d2, v, w[dv[q]]);
+            if constexpr (E3 && sizeof(R) == 4) epilogue<PR ? 4 : 1>(d1, d2, v, w[dv[q]]);
           } else {
             if (mask && mask[dv[q]]) v = mk(R(0), R(0));
             y[dv[q]] = v;
             if constexpr (EPI == 1 || EPI == 2) epilogue<EPI>(d1, d2, v, w[dv[q]]);
-            else if constexpr (EPI == 3 && sizeof(R) == 4) epilogue<1>(d1, d2, v, w[dv[q]]);
+            else if constexpr (E3 && sizeof(R) == 4) epilogue<PR ? 4 : 1>(d1, d2, v, w[dv[q]]);
           }
         }
       }
@@ -770,7 +789,8 @@ void launch_patch(Context* c, const System* sys, const PatchMaps& pm, Coef k,
   if (epi == 0) SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 0, CB>), Gp, CB, lds, c->stream, PT_ARGS);
   else if (epi == 1) SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 1, CB>), Gp, CB, lds, c->stream, PT_ARGS);
   else if (epi == 2) SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 2, CB>), Gp, CB, lds, c->stream, PT_ARGS);
-  else SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 3, CB>), Gp, CB, lds, c->stream, PT_ARGS);
+  else if (epi == 3) SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 3, CB>), Gp, CB, lds, c->stream, PT_ARGS);
+  else SUCEM_LAUNCH((k_mf_patch<R, ORDER, NFD, 4, CB>), Gp, CB, lds, c->stream, PT_ARGS);
 #undef PT_ARGS
 }
 
@@ -888,12 +908,13 @@ void pt_apply_deferred(Context* c, const System* sys, Coef k, const uint8_t* mas
   SUCEM_CHECK(pt_fits<R>(sys->space, f.U), SUCEM_ERR_UNSUPPORTED,
               "the patch operator needs more LDS per workgroup than the device has");
   const PatchMaps& pm = sys->space->pt[cb_slot(f.U)];
+  const int epi = rs.paired ? 4 : 3;
   if (sys->space->order == 2) {
-    if (f.U == 128) launch_patch<R, 2, 8, 128>(c, sys, pm, k, mask, x, y, w, 3, rs, Gp, Gp);
-    else if (f.U == 192) launch_patch<R, 2, 8, 192>(c, sys, pm, k, mask, x, y, w, 3, rs, Gp, Gp);
-    else launch_patch<R, 2, 8, 256>(c, sys, pm, k, mask, x, y, w, 3, rs, Gp, Gp);
+    if (f.U == 128) launch_patch<R, 2, 8, 128>(c, sys, pm, k, mask, x, y, w, epi, rs, Gp, Gp);
+    else if (f.U == 192) launch_patch<R, 2, 8, 192>(c, sys, pm, k, mask, x, y, w, epi, rs, Gp, Gp);
+    else launch_patch<R, 2, 8, 256>(c, sys, pm, k, mask, x, y, w, epi, rs, Gp, Gp);
   } else {
-    launch_patch<R, 1, 3, 256>(c, sys, pm, k, mask, x, y, w, 3, rs, Gp, Gp);
+    launch_patch<R, 1, 3, 256>(c, sys, pm, k, mask, x, y, w, epi, rs, Gp, Gp);
   }
 }
 template void pt_apply_deferred<float>(Context*, const System*, Coef, const uint8_t*,

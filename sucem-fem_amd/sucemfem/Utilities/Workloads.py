@@ -93,9 +93,10 @@ class DeviceProblem(object):
 
     def solve_coef(self, cs, ct, c0, b, x, precision=64, tol=1e-8, max_iter=50000,
                    method=_hip.METHOD_COCG, check_every=16, ru_delta=0.01,
-                   precond=_hip.PRECOND_AUXILIARY, eliminate_pec=False):
+                   precond=_hip.PRECOND_AUXILIARY, eliminate_pec=False, paired=False):
         """Solve (cs S + ct T + i c0 S0) x = b (e.g. the shifted cavity operator with the
-        PEC DOFs eliminated)."""
+        PEC DOFs eliminated).  ``paired``: b and x hold two real systems in their real and
+        imaginary parts, solved as two independent real recursions (c0 = 0)."""
         import torch
         self._check_vec(b, torch.complex128)
         self._check_vec(x, torch.complex128)
@@ -104,6 +105,7 @@ class DeviceProblem(object):
         o.method, o.precision, o.tol, o.max_iter = method, precision, tol, max_iter
         o.check_every, o.ru_delta, o.precond = check_every, ru_delta, precond
         o.eliminate_pec = 1 if eliminate_pec else 0
+        o.paired = 1 if paired else 0
         st = _hip.SolveStats()
         _hip.call("sucem_solve_system", self.ctx.handle, self.system.handle.h, float(cs),
                   float(ct), float(c0), _hip.ptr(b), _hip.ptr(x), ctypes.byref(o),
@@ -152,12 +154,15 @@ class DeviceProblem(object):
         return [s.as_dict() for s in st]
 
     def eigs(self, sigma, nev, ncv=0, tol=1e-10, inner_tol=1e-11, vectors=False,
-             inner_precision=64):
-        """Lowest nev k^2 > sigma of the PEC cavity: (k2, relative residuals, stats[, modes])."""
+             inner_precision=64, block=0):
+        """Lowest nev k^2 > sigma of the PEC cavity: (k2, relative residuals, stats[, modes]).
+        ``block``: Lanczos block size 1 or 2 (paired inner solves); 0 picks 2 where the
+        paired solves run."""
         o = _hip.EigOpts()
         _hip.load().sucem_eig_opts_default(ctypes.byref(o))
         o.ncv, o.tol, o.inner_tol = int(ncv), tol, inner_tol
         o.inner_precision = int(inner_precision)
+        o.block = int(block)
         k2 = np.zeros(nev)
         resid = np.zeros(nev)
         vecs = self.ctx.tensor(nev * self.ndofs) if vectors else None

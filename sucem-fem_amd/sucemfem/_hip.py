@@ -24,7 +24,7 @@ class SolveOpts(ctypes.Structure):
                 ("max_iter", ctypes.c_int), ("check_every", ctypes.c_int),
                 ("eliminate_pec", ctypes.c_int), ("tol", ctypes.c_double),
                 ("ru_delta", ctypes.c_double), ("gmres_restart", ctypes.c_int),
-                ("pad_", ctypes.c_int), ("precond_shift", ctypes.c_double)]
+                ("paired", ctypes.c_int), ("precond_shift", ctypes.c_double)]
 
 
 class SolveStats(ctypes.Structure):
@@ -43,14 +43,14 @@ class EigOpts(ctypes.Structure):
     _fields_ = [("ncv", ctypes.c_int), ("max_restarts", ctypes.c_int),
                 ("inner_precision", ctypes.c_int), ("inner_max_iter", ctypes.c_int),
                 ("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("block", ctypes.c_int), ("pad_", ctypes.c_int)]
 
 
 class EigStats(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("nconv", ctypes.c_int), ("restarts", ctypes.c_int),
                 ("lanczos_steps", ctypes.c_int), ("inner_failures", ctypes.c_int),
                 ("inner_stagnated", ctypes.c_int), ("inner_fp64_from", ctypes.c_int),
-                ("pad_", ctypes.c_int), ("inner_iterations", ctypes.c_int64),
+                ("block", ctypes.c_int), ("inner_iterations", ctypes.c_int64),
                 ("max_ritz_residual", ctypes.c_double), ("elapsed_ms", ctypes.c_double)]
 
     def as_dict(self):

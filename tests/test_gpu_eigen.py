@@ -162,3 +162,29 @@ def test_relaxed_tolerances_keep_eigenvalues_within_1e8(inner_precision):
     eps = np.where(prob.region == 1, 6.0, 1.0)
     ref, _, _ = O.cavity_eigs(O.Space(co, ce, 2), 20, 1.1, eps_r=eps)
     assert np.all(np.abs(k2 - ref) / ref <= 1e-8), (np.abs(k2 - ref) / ref).max()
+
+
+@pytest.mark.parametrize("order,block,prec", [(1, 1, 64), (2, 1, 64), (1, 2, 64), (2, 2, 64),
+                                              (2, 2, 32), (2, 0, 64)])
+def test_lanczos_block_sizes_match_oracle(order, block, prec):
+    """Block size 1 (one inner solve per basis vector) and 2 (two basis vectors per step,
+    their inner solves run as one paired solve) give the oracle pencil's 10 lowest k^2 to
+    1e-8; block 0 picks 2 on these meshes (the patch operator serves the paired solves)."""
+    from sucemfem.ProblemConfigurations.EMVectorWaveEigenproblem import DefaultEigenSolver
+    print("cavity order %d, Lanczos block %d, inner FP%d" % (order, block, prec), flush=True)
+    ep, (co, ce) = _cavity(order)
+    es = DefaultEigenSolver()
+    es.set_eigenproblem(ep)
+    es.set_sigma(1.1)
+    es.set_options(block=block, inner_precision=prec)
+    k2, modes, resid = es.solve_problem_k2(10)
+    st = es.stats
+    assert st["status"] == 0 and st["inner_failures"] == 0, st
+    assert st["block"] == (block or 2), st
+    ref, _, _ = O.cavity_eigs(O.Space(co, ce, order), 10, 1.1)
+    assert np.all(np.abs(k2 - ref) / ref <= 1e-8), (k2, ref)
+    assert np.all(resid < 1e-7)
+    # T-orthonormal modes: modes^T T modes = I (T of the oracle, PEC rows excluded)
+    m = O.assemble_fast(O.Space(co, ce, order), abc=False)
+    gram = modes.real @ (m["M"] @ modes.real.T)
+    assert np.abs(gram - np.eye(10)).max() < 1e-7
