@@ -43,7 +43,9 @@ class DefaultEigenSolver(object):
         self.sigma = sigma
 
     def set_options(self, **options):
-        """ncv, max_restarts, inner_precision (64|32), inner_max_iter, tol, inner_tol, seed."""
+        """ncv, max_restarts, inner_precision (64|32), inner_max_iter, tol, inner_tol, seed,
+        block (Lanczos block size 1 or 2; 0, the default, picks 2 where paired inner solves
+        run)."""
         self.options.update(options)
 
     def solve_problem_k2(self, nev, ncv=None, return_vectors=True):
