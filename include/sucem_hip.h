@@ -85,7 +85,8 @@ typedef struct sucem_solve_opts {
 } sucem_solve_opts;
 
 typedef struct sucem_eig_opts {
-  int ncv;              /* Lanczos basis size (0: max(2 nev + 1, 20)) */
+  int ncv;              /* Lanczos basis size (0: max(2 nev + 1, 20) for block size 1,
+                           max(2 nev + 2, nev + 20) for block size 2) */
   int max_restarts;
   int inner_precision;  /* 64: FP64 inner solves; 32: FP32 refinement cycles (FP32 COCG
                            recursions restarted from the FP64 residual, x in FP64) */

@@ -270,7 +270,7 @@ bool sys_supports_sweep(Context* c, System* sys);
 
 // eigen.hip
 struct EigOpts {
-  int ncv = 0;               // Lanczos basis size (0: max(2 nev + 1, 20))
+  int ncv = 0;               // Lanczos basis size (0: see eigs_system)
   int max_restarts = 100;
   int inner_precision = 64;
   int inner_max_iter = 20000;
