@@ -847,11 +847,11 @@ def nodal_interpolation(space):
 
 
 def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000, gdiag=None, vdiag=None,
-                   theta_g=3.0, A_prec=None):
+                   theta_g=3.0, A_prec=None, theta_v=2.0):
     """COCG with the auxiliary-space preconditioner, FP64 (scipy CSR):
-    M^-1 r = D^-1 r + theta_g G diag(G^T A G)^-1 G^T r + Pi diag(Pi^T A Pi)^-1 Pi^T r
+    M^-1 r = D^-1 r + theta_g G diag(G^T A G)^-1 G^T r + theta_v Pi diag(Pi^T A Pi)^-1 Pi^T r
     (zero diagonals dropped; ``gdiag`` / ``vdiag`` precomputed diagonals; the device uses
-    theta_g = 3 at order 2 and 2 at order 1).  ``A_prec``: the matrix D and the two
+    theta_g = 3 at order 2 and 2 at order 1, theta_v = 2).  ``A_prec``: the matrix D and the two
     auxiliary diagonals are taken from (default A).  Returns ``(x, iterations, history)``."""
     A = A.tocsr()
     P = A if A_prec is None else A_prec.tocsr()
@@ -863,7 +863,7 @@ def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000, gdiag=None, vdiag=None,
         w[nz] = 1.0 / d[nz]
         return w
     W = theta_g * inv(gradient_diag(P, G) if gdiag is None else gdiag)
-    V = inv(gradient_diag(P, Pi) if vdiag is None else vdiag)
+    V = theta_v * inv(gradient_diag(P, Pi) if vdiag is None else vdiag)
     G = G.tocsr().astype(np.complex128)
     Gt = G.T.tocsr()
     Pi = Pi.tocsr().astype(np.complex128)

@@ -72,8 +72,10 @@ def test_paired_solve_matches_two_direct_solves(dev, order, precision):
         assert np.linalg.norm(part[free] - ref) / np.linalg.norm(ref) < 1e-7
     pec = np.setdiff1d(np.arange(ndofs), free)
     assert np.abs(xs[pec]).max() == 0.0
-    # the same two systems one at a time: the paired recursion is each real recursion
-    # (the same iteration counts within the rounding of the packed sums)
+    # the same two systems one at a time: the paired recursion runs each real recursion,
+    # so it stops within the two single solves' iteration counts (FP64 solves confirm the
+    # target with the true residual and restart when it is a rounding above the recursion's,
+    # which moves single counts by tens of iterations: 5 % slack on either side)
     its = []
     for bp in (b0, b1):
         xb = torch.zeros_like(b)
@@ -83,7 +85,8 @@ def test_paired_solve_matches_two_direct_solves(dev, order, precision):
         assert s1["status"] == 0, s1
         its.append(s1["iterations"])
     if precision == 64:
-        assert abs(st["iterations"] - max(its)) <= max(4, max(its) // 20), (st, its)
+        slack = max(4, max(its) // 20)
+        assert min(its) - slack <= st["iterations"] <= max(its) + slack, (st, its)
 
 
 def test_paired_solve_with_a_zero_part(dev):
