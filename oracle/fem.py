@@ -851,7 +851,8 @@ def cocg_auxiliary(A, b, G, Pi, tol=1e-8, maxiter=10000, gdiag=None, vdiag=None,
     """COCG with the auxiliary-space preconditioner, FP64 (scipy CSR):
     M^-1 r = D^-1 r + theta_g G diag(G^T A G)^-1 G^T r + theta_v Pi diag(Pi^T A Pi)^-1 Pi^T r
     (zero diagonals dropped; ``gdiag`` / ``vdiag`` precomputed diagonals; the device uses
-    theta_g = 3 at order 2 and 2 at order 1, theta_v = 2).  ``A_prec``: the matrix D and the two
+    theta_g = 3 at order 2 and 2 at order 1, theta_v = 2 for driven systems and 1 for the
+    real shifted cavity operator).  ``A_prec``: the matrix D and the two
     auxiliary diagonals are taken from (default A).  Returns ``(x, iterations, history)``."""
     A = A.tocsr()
     P = A if A_prec is None else A_prec.tocsr()

@@ -12,7 +12,8 @@
 //
 // The auxiliary-space form adds the vector nodal term of the Hiptmair-Xu splitting:
 //
-//   M^-1 r = D^-1 r + G W G^T r + Pi V Pi^T r,   V = theta_v diag(Pi^T A Pi)^-1,
+//   M^-1 r = D^-1 r + G W G^T r + Pi V Pi^T r,   V = theta_v diag(Pi^T A Pi)^-1
+//   (theta_v: see nodal_weight),
 //
 // Pi the nodal interpolation of continuous piecewise-linear vector fields into the
 // Nedelec space: Pi[i][(v, d)] = lambda_v(x_i) (t_i)_d for the DOF i with dual point x_i
@@ -777,14 +778,17 @@ const double* nodal_diag(Context* c, System* sys, bool masked) {
   return d.p;
 }
 
-// Weight of the vector nodal term, theta_v Pi diag(Pi^T A Pi)^-1 Pi^T.  Measured on the
-// device with the shifted driven preconditioner (beta = 0.25), theta_v = 1 / 1.5 / 2 /
-// 2.5: 40^3 order-2 cube FP32+IR 1,616 / 1,440 / 1,344 / 1,584 COCG iterations, FP64
-// 1,552 / 1,376 / 1,312 / 1,248; 40^3 order-1 sweep mean 888 / 822 / 788 (FP32+IR) and
-// 764 / 708 / 686 (FP64); cavity 60x30x45 order 2 (eigen, FP64 inner) 54,152 / 53,120 /
-// 53,176 inner iterations.  The oracle agrees (16^3 / 20^3 order-2 driven cube, FP64:
-// 609 / 574 / 571 and 773 / 709 / 656 at 1 / 1.5 / 2).
-constexpr double kNodalWeight = 2.0;
+// Weight of the vector nodal term, theta_v Pi diag(Pi^T A Pi)^-1 Pi^T: 2 for driven
+// systems (c0 or the preconditioner's complex shift nonzero), 1 for the real shifted
+// cavity operator S - sigma T.  Measured on the device, theta_v = 1 / 1.5 / 2 / 2.5 with
+// the shifted driven preconditioner (beta = 0.25): 40^3 order-2 cube FP32+IR 1,616 /
+// 1,440 / 1,344 / 1,584 COCG iterations, FP64 1,552 / 1,376 / 1,312 / 1,248; 40^3 order-1
+// sweep mean 888 / 822 / 788 (FP32+IR) and 764 / 708 / 686 (FP64) at 1 / 1.5 / 2.  The
+// oracle agrees (16^3 / 20^3 order-2 driven cube, FP64: 609 / 574 / 571 and 773 / 709 /
+// 656 at 1 / 1.5 / 2).  On the cavity operator 2 gains 2 % (60x30x45 order 2, eigen inner
+// iterations 54,152 / 53,176 at 1 / 2) but loses 12 % of the serving trace's throughput
+// (its eigen requests, sigma at half the lowest mode: 0.48 / 0.42 req/s).
+double nodal_weight(const Coef& k) { return (k.s0 != 0.0 || k.ti != 0.0) ? 2.0 : 1.0; }
 
 template <typename R>
 void nodal_winv(Context* c, System* sys, Coef k, bool masked, void* vinv) {
@@ -792,7 +796,7 @@ void nodal_winv(Context* c, System* sys, Coef k, bool masked, void* vinv) {
   using C = typename C2<R>::T;
   const int64_t nn = 3 * sys->space->mesh->nv;
   SUCEM_LAUNCH((k_grad_winv<C>), grid_for(nn, TB), TB, 0, c->stream, nn, d9, k.s, k.t, k.ti, k.s0,
-               kNodalWeight, (C*)vinv);
+               nodal_weight(k), (C*)vinv);
 }
 template void nodal_winv<float>(Context*, System*, Coef, bool, void*);
 template void nodal_winv<double>(Context*, System*, Coef, bool, void*);

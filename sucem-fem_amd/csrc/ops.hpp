@@ -61,6 +61,8 @@ void finish_reduction(Context* c, const RedSite& rs, int G);
 template <typename R>
 void grad_winv(Context* c, System* sys, Coef k, bool masked, bool aux, void* winv);
 double gradient_weight(int order, bool aux);
+// weight of the auxiliary-space preconditioner's vector nodal term for coefficients k
+double nodal_weight(const Coef& k);
 // vinv (nullable): the auxiliary-space form, vinv = diag(Pi^T A Pi)^-1 (3 nv) from
 // nodal_winv, u a 3 nv work vector
 template <typename R>

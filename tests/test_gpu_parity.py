@@ -403,7 +403,8 @@ def test_auxiliary_preconditioner_iterations_match_oracle(dev, order, pec):
         free = np.arange(osp.ndofs)
         cs, ct, c0 = 1.0, -k0 * k0, k0
     xo, its, _ = O.cocg_auxiliary(A, bh[free], G, Pi, tol=1e-9, maxiter=20000,
-                                  theta_g=3.0 if order == 2 else 2.0)
+                                  theta_g=3.0 if order == 2 else 2.0,
+                                  theta_v=1.0 if pec else 2.0)
     b = torch.as_tensor(bh, device="cuda")
     for precision in (64, 32):
         x = torch.zeros_like(b)
@@ -423,7 +424,8 @@ def test_auxiliary_preconditioner_iterations_match_oracle(dev, order, pec):
 @pytest.mark.parametrize("order,pec", [(2, False), (1, False), (2, True), (1, True)])
 def test_auxiliary_preconditioner_application_matches_oracle(dev, order, pec):
     """z = M^-1 r of the device's auxiliary-space preconditioner (Jacobi + theta G
-    diag(G^T A G)^-1 G^T + 2 Pi diag(Pi^T A Pi)^-1 Pi^T, G and Pi applied from entity ids and
+    diag(G^T A G)^-1 G^T + theta_v Pi diag(Pi^T A Pi)^-1 Pi^T, theta_v 2 driven and 1 on the
+    cavity operator, G and Pi applied from entity ids and
     coordinates) equals the oracle's restatement: FP64 to 1e-12, FP32 to 1e-5; driven
     systems with the complex shift of their diagonals, and the shifted cavity operator
     with the PEC DOFs eliminated."""
@@ -457,7 +459,8 @@ def test_auxiliary_preconditioner_application_matches_oracle(dev, order, pec):
         w[nz] = 1.0 / d[nz]
         return w
     W = theta * inv(O.gradient_diag(P, G))
-    V = 2.0 * inv(O.gradient_diag(P, Pi))   # the nodal term's weight theta_v = 2
+    # the nodal term's weight: 2 for driven systems, 1 for the cavity operator
+    V = (1.0 if pec else 2.0) * inv(O.gradient_diag(P, Pi))
     rng = np.random.default_rng(7)
     r = rng.standard_normal(osp.ndofs) + 1j * rng.standard_normal(osp.ndofs)
     if pec:
