@@ -216,7 +216,9 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
   // 30 / 40, 10 modes 53 / 40 at 22 / 30, 4 modes 26 / 27 at 20 / 24)
   auto basis_size = [&](int b) {
     int mb = o.ncv > 0 ? o.ncv : std::max(2 * nev + b, b == 2 ? nev + 20 : 20);
-    mb = (int)std::min<int64_t>(mb, nfree - (b - 1));
+    // block 2 holds m + 2 T-orthonormal vectors (block 1's last vector ends the cycle as an
+    // invariant subspace when m = nfree)
+    mb = (int)std::min<int64_t>(mb, b == 1 ? nfree : nfree - b);
     return mb - mb % b;
   };
   int m = basis_size(bs);
@@ -310,8 +312,15 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
   auto random_vector = [&](int top, unsigned long long salt) {
     SUCEM_LAUNCH(k_start_vector, grid_for(n, TB), TB, 0, c->stream, (int64_t)n,
                  (unsigned long long)o.seed + salt, mask, y.p);
+    double n0 = 1.0;
+    if (top > 0) {   // its T-norm before the projection, to judge what is left of it
+      apply(Tk, y.p, ty.p);
+      std::vector<double> nn;
+      multidot(y.p, 1, ty.p, nn);
+      n0 = std::sqrt(std::max(nn[0], 0.0));
+    }
     const double b = orth_normalize(y.p, top, -1);
-    SUCEM_CHECK(std::isfinite(b) && b > 0.0, SUCEM_ERR_INVALID,
+    SUCEM_CHECK(std::isfinite(b) && b > (top > 0 ? 1e-8 * n0 : 0.0), SUCEM_ERR_INVALID,
                 "no start vector independent of the basis (too few free DOFs)");
   };
   SolveOpts so;

@@ -188,3 +188,20 @@ def test_lanczos_block_sizes_match_oracle(order, block, prec):
     m = O.assemble_fast(O.Space(co, ce, order), abc=False)
     gram = modes.real @ (m["M"] @ modes.real.T)
     assert np.abs(gram - np.eye(10)).max() < 1e-7
+
+
+@pytest.mark.parametrize("nev", [6, 20])
+def test_tiny_cavity_basis_capped_by_free_dofs(nev):
+    """2x2x2 boxes, order 1: 26 free DOFs (25 nonzero modes), so the default basis is cut
+    to the free DOFs (block 2 holds m + 2 orthonormal vectors); the k^2 still match the
+    oracle pencil to 1e-8."""
+    from sucemfem.ProblemConfigurations.EMVectorWaveEigenproblem import DefaultEigenSolver
+    print("tiny cavity, %d modes" % nev, flush=True)
+    ep, (co, ce) = _cavity(1, n=2)
+    es = DefaultEigenSolver()
+    es.set_eigenproblem(ep)
+    es.set_sigma(1.1)
+    k2, modes, resid = es.solve_problem_k2(nev)
+    assert es.stats["status"] == 0, es.stats
+    ref, _, _ = O.cavity_eigs(O.Space(co, ce, 1), nev, 1.1)
+    assert np.all(np.abs(k2 - ref) / ref <= 1e-8), (k2, ref, es.stats)
