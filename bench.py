@@ -317,6 +317,8 @@ def main():
     ap.add_argument("--eig-n", default="120,60,90", help="eigen workload boxes per axis")
     ap.add_argument("--eig-prec", default="32,64",
                     help="eigen workload inner precisions, comma-separated (32 = FP32+IR)")
+    ap.add_argument("--eig-inner-tol", type=float, default=1e-9,
+                    help="eigen workload: relative residual of the inner solves")
     ap.add_argument("--eig-block", type=int, default=0, choices=[0, 1, 2],
                     help="eigen workload Lanczos block size (0: the solver's choice)")
     ap.add_argument("--serve-count", type=int, default=16)
@@ -346,7 +348,7 @@ def main():
     if a.workload == "eigen":
         nn = tuple(int(v) for v in a.eig_n.split(","))
         precs = tuple(int(v) for v in a.eig_prec.split(","))
-        r = eigen_measure(n=nn, precisions=precs, block=a.eig_block)
+        r = eigen_measure(n=nn, precisions=precs, block=a.eig_block, inner_tol=a.eig_inner_tol)
         key = "fp32ir" if 32 in precs else "fp64"
         v = r[key]["solves_per_s"]
         print(json.dumps(workload_line("eigen", a, r, v, "solves/s", 1e3 * r[key]["s"],
