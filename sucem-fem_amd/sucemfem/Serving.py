@@ -98,8 +98,10 @@ def footprint_bytes(req):
     build = contrib * (8 + 8 + 4 + 4) + contrib * 4               # pattern sort peak
     resident = nnz * (4 + 16 + 8) + contrib * 4 + nnz * 4 + N * 64
     if req.kind == 'eigen':
-        m = max(2 * req.nev + 1, 20)
-        solver = 3 * (m + 1) * N * 16 + 12 * N * 16
+        # block-2 Lanczos: basis of max(2 nev + 2, nev + 20) plus the two residual vectors
+        # (V and T V), the restart copy, five work vectors, and the inner solve's workspace
+        m = max(2 * req.nev + 2, req.nev + 20)
+        solver = (3 * m + 10) * N * 16 + 12 * N * 16
     else:
         solver = 14 * N * 16
     return int(max(build, resident) + solver + (64 << 20))
