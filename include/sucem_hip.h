@@ -104,15 +104,17 @@ typedef struct sucem_eig_stats {
   int status;           /* 0 converged, 1 restart limit reached */
   int nconv;
   int restarts;
-  int lanczos_steps;
+  int lanczos_steps;    /* Lanczos basis vectors generated (inner solves x block size) */
   int inner_failures;   /* inner solves that hit inner_max_iter or broke down */
   int inner_stagnated;  /* inner solves stopped at the floor of their true residual */
-  int inner_fp64_from;  /* inner_precision 32: the Lanczos step from which the inner solves
-                           ran in FP64 because two in a row had their FP32 cycles fail and
-                           were finished in FP64 (0: all steps ran FP32 cycles) */
+  int inner_fp64_from;  /* inner_precision 32: the Lanczos vector (counted as in
+                           lanczos_steps) from which the inner solves ran in FP64 because two
+                           solves in a row had their FP32 cycles fail and were finished in FP64
+                           (0: all inner solves ran FP32 cycles) */
   int block;            /* the Lanczos block size used (lanczos_steps counts basis vectors:
                            lanczos_steps / block inner solves) */
-  int64_t inner_iterations;
+  int64_t inner_iterations;  /* COCG iterations over all inner solves (a paired solve's
+                                once) */
   double max_ritz_residual;
   double elapsed_ms;
 } sucem_eig_stats;
