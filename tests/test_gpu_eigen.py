@@ -205,3 +205,22 @@ def test_tiny_cavity_basis_capped_by_free_dofs(nev):
     assert es.stats["status"] == 0, es.stats
     ref, _, _ = O.cavity_eigs(O.Space(co, ce, 1), nev, 1.1)
     assert np.all(np.abs(k2 - ref) / ref <= 1e-8), (k2, ref, es.stats)
+
+
+@pytest.mark.parametrize("block", [1, 2])
+def test_cube_cavity_double_modes(block):
+    """The unit cube cavity (4^3 boxes, order 2): its discrete spectrum holds exact double
+    eigenvalues (mesh symmetry); block sizes 1 and 2 both return the 14 lowest k^2 with
+    their multiplicities, to 1e-8 of the oracle pencil."""
+    from sucemfem.ProblemConfigurations.EMVectorWaveEigenproblem import DefaultEigenSolver
+    print("cube cavity, block %d" % block, flush=True)
+    ep, (co, ce) = _cavity(2, n=4, dims=[1.0, 1.0, 1.0])
+    es = DefaultEigenSolver()
+    es.set_eigenproblem(ep)
+    es.set_sigma(1.1)
+    es.set_options(block=block)
+    k2, modes, resid = es.solve_problem_k2(14)
+    assert es.stats["status"] == 0 and es.stats["block"] == block, es.stats
+    ref, _, _ = O.cavity_eigs(O.Space(co, ce, 2), 14, 1.1)
+    assert np.sum(np.abs(np.diff(ref)) < 1e-9 * ref[1:]) >= 4   # double modes present
+    assert np.all(np.abs(k2 - ref) / ref <= 1e-8), (k2, ref)
