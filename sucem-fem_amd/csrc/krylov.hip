@@ -168,42 +168,14 @@ __global__ void k_cocg_xr_plain(int n, int G, const double* scal, C* x, C* r, co
   }
 }
 
-// The same update when the operator left the sums of DOFs shared by several patches
-// unsummed: for those rows (qcode >= 0) q is the sum of their partial sums, in patch order
-// (zero on masked rows).
-template <typename C>
-__global__ void k_cocg_xr_def(int n, int G, const double* scal, C* x, C* r, const C* p,
-                              const C* q, const int32_t* __restrict__ qcode,
-                              const C* __restrict__ part, const uint8_t* __restrict__ mask) {
-  const C a = from_d<C>(scal2(scal, S_ALPHA));
-  Range rr = block_rows(n, G);
-  for (int i = rr.start + threadIdx.x; i < rr.end; i += TB) {
-    const int code = qcode[i];
-    C qi;
-    if (code < 0) {
-      qi = q[i];
-    } else {
-      qi = mk(decltype(qi.x)(0), decltype(qi.x)(0));
-      const int q0 = code >> 4, q1 = q0 + (code & 15);
-      for (int j = q0; j < q1; ++j) {
-        const C t = part[j];
-        qi.x += t.x;
-        qi.y += t.y;
-      }
-      if (mask && mask[i]) qi = mk(decltype(qi.x)(0), decltype(qi.x)(0));
-    }
-    const C ap = cmul(a, p[i]), aq = cmul(a, qi);
-    C xi = x[i], ri = r[i];
-    xi.x += ap.x; xi.y += ap.y;
-    ri.x -= aq.x; ri.y -= aq.y;
-    x[i] = xi;
-    r[i] = ri;
-  }
-}
-
-// The same update with alpha = rho / (p.Ap) formed by every block from the operator's Gd
-// block partials (its p.Ap epilogue): all blocks sum them in one fixed order, so they agree
-// bit for bit, and the one-block finalising launch between the two kernels is saved.
+// x += alpha p ; r -= alpha q after the patch operator left the sums of DOFs shared by
+// several patches unsummed: for those rows (qcode >= 0) q is the sum of their partial sums,
+// in patch order (zero on masked rows).  alpha = rho / (p.Ap) is formed by every block from
+// the operator's Gd block partials (its p.Ap epilogue): all blocks sum them in one fixed
+// order, so they agree bit for bit, and the one-block finalising launch between the two
+// kernels is saved.  (Moving x += alpha p into the p update, which reads p anyway, was
+// measured: this kernel 30.3 -> 23.4 us, the p update 12.2 -> 20.9 us per FP32 application
+// at 40^3 order 2, so not kept.)
 // rho is read from S_RHO_NEXT, which no block of this kernel writes; block 0 stores alpha
 // and copies that rho to S_RHO, which the next p update reads as the previous rho.
 template <typename C>
