@@ -365,11 +365,11 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
           st.inner_fp64_from = st.lanczos_steps + 1;
           if (log_level >= 1)
             std::fprintf(stderr, "[sucem eigs] FP32 inner cycles failed on two solves in a "
-                                 "row: FP64 inner solves from step %d\n", st.inner_fp64_from);
+                                 "row: FP64 inner solves from vector %d\n", st.inner_fp64_from);
         }
       }
       if (log_level >= 2)
-        std::fprintf(stderr, "[sucem eigs] step %d: inner %d iterations, rel %.2e%s, %.1f s\n",
+        std::fprintf(stderr, "[sucem eigs] vectors %d: inner %d iterations, rel %.2e%s, %.1f s\n",
                      st.lanczos_steps, ist.iterations, ist.rel_residual,
                      ist.stagnated ? " (floor)" : "", elapsed_s());
       bool invariant = false;
@@ -430,7 +430,7 @@ void eigs_system(Context* c, const System* sys, double sigma, int nev, const Eig
     st.max_ritz_residual = worst;
     st.restarts = cycle;
     if (log_level >= 1)
-      std::fprintf(stderr, "[sucem eigs] cycle %d: %d Lanczos steps, %lld inner iterations, "
+      std::fprintf(stderr, "[sucem eigs] cycle %d: %d Lanczos vectors, %lld inner iterations, "
                    "worst Ritz residual %.2e, %.1f s\n", cycle, st.lanczos_steps,
                    (long long)st.inner_iterations, worst, elapsed_s());
     if (conv || mm < m || cycle == o.max_restarts) {
