@@ -305,9 +305,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=200)
     ap.add_argument("--operator", default="auto",
-                    help="auto (autotuned), assembled, cells (matrix-free per cell) or "
-                         "patch128 / patch192 / patch256 (matrix-free per patch of cells), "
-                         "for both precisions")
+                    help="auto (autotuned), assembled, cells (matrix-free per cell), "
+                         "cellsmfma (per cell, order-2 element products on the matrix cores "
+                         "in FP32; FP64 runs cells) or patch128 / patch192 / patch256 "
+                         "(matrix-free per patch of cells), for both precisions")
     ap.add_argument("--workload", default="driven",
                     choices=["driven", "sweep", "eigen", "inner", "serve"],
                     help="driven (the headline line), sweep (64 frequencies x 8 dipoles, "
@@ -382,6 +383,7 @@ def main():
     log("setup %.2fs: N=%d nnz=%d nnz_abc=%d" % (setup_s, prob.ndofs, prob.nnz, prob.nnz0))
     tuned = {}
     fixed = {"assembled": (0, 16, 4, 2048), "cells": (1, 0, 1, 2048),
+             "cellsmfma": (1, 0, 3, 2048),
              "patch128": (2, 0, 128, 4096), "patch192": (2, 0, 192, 4096),
              "patch256": (2, 0, 256, 4096)}
     for prec in (32, 64):
@@ -389,6 +391,8 @@ def main():
             cfg, ms = prob.tune(prec)
         else:
             mf, lanes, loads, blocks = fixed[a.operator]
+            if loads == 3 and prec == 64:
+                loads = 1   # the matrix-core element products are FP32 only
             prob.set_spmv(prec, mf, lanes, loads, blocks)
             cfg, ms = dict(matrix_free=mf, lanes=lanes, loads=loads, blocks=blocks), None
         tuned[prec] = dict(cfg, ms=ms)

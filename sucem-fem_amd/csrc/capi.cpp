@@ -498,8 +498,11 @@ int sucem_system_set_spmv(sucem_ctx* ctx, sucem_system* sys, int precision, cons
                 "matrix-free operator: orders 1 and 2 only");
     SUCEM_CHECK(f.L == 0 || f.L == 1 || f.L == 4, SUCEM_ERR_INVALID,
                 "matrix-free lanes per row must be 0 (LDS-staged rows), 1 or 4");
-    SUCEM_CHECK(f.U == 1 || f.U == 2, SUCEM_ERR_INVALID,
-                "matrix-free loads must be 1 (whole element) or 2 (half the rows at a time)");
+    SUCEM_CHECK(f.U == 1 || f.U == 2 || f.U == 3, SUCEM_ERR_INVALID,
+                "matrix-free loads must be 1 (whole element), 2 (half the rows at a time) or "
+                "3 (order-2 element products on the matrix cores, FP32)");
+    SUCEM_CHECK(f.U != 3 || (sys->s.space->order == 2 && precision == 32),
+                SUCEM_ERR_UNSUPPORTED, "the MFMA element products are order 2, FP32 only");
     mf_prepare(c, &sys->s);
   } else {
     SUCEM_CHECK(f.mf == 0, SUCEM_ERR_INVALID, "matrix_free must be 0, 1 or 2");

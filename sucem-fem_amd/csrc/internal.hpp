@@ -100,6 +100,7 @@ struct Space {
   // cells (cell * nloc + local) and boundary faces (face * nfd + face DOF) sorted by DOF:
   // segment pointers per DOF and each product's position in that order
   DBuf<int32_t> mf_cdT, mf_dptr, mf_posT, mf_posF;
+  DBuf<uint16_t> mf_afrag;        // order 2: fp16 operand fragments of the MFMA element kernel
   // patch operator (built on first use): cells in Morton order of their centroids
   // (pt_cell: rank -> cell) and the boundary faces in the order of their cells' ranks
   // (pt_face: original id, pt_fmeta: (cell rank << 2) | local face); per patch size, the

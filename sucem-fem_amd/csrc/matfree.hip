@@ -22,7 +22,9 @@
 // order-2 cell.  The order of every sum is
 // fixed, so results are deterministic.  The element buffer belongs to the System:
 // concurrent applications of one System on different streams are not supported.
+#include <algorithm>
 #include <cstring>
+#include <vector>
 #include <utility>
 #include <rocprim/rocprim.hpp>
 
@@ -221,6 +223,134 @@ __global__ __launch_bounds__(TB, SPLIT ? (sizeof(R) == 8 ? 3 : 5) : 1) void k_mf
   }
 }
 
+// ---- order-2 element products on the matrix cores (FP32 operator) ---------------------
+// Z = M X with M the 12 reference tensors' integer numerators stacked as 240 rows (tensor
+// k, row i) x 20 columns j (K padded to 32), exact in fp16 (|v| <= 2268, at most 11
+// significant bits), and X 8 cells' x, real and imaginary parts as 16 columns.  Each column
+// is scaled by a power of two s (its largest |x| into [1, 2)) and split into fp16 pieces
+// hi + 2^-11 lo (22 significant bits); v_mfma_f32_16x16x32_f16 forms M lo, which is
+// scaled by 2^-11, then accumulates M hi onto it, per 16-row tile.  The contraction
+// y_i = sum_k a_k Z[(k, i)] / s runs on the vector units: M's rows are ordered so that the
+// 4 output rows a lane holds per tile (rows 4 (lane / 16) .. + 3), over the 15 tiles, are
+// 5 rows i of all 12 tensors k, so no sums cross lanes.  MFMA operand layout (checked on
+// the device): A[row = lane % 16][k = 8 (lane / 16) + e], B[k = 8 (lane / 16) + e][col =
+// lane % 16], D[4 (lane / 16) + q][lane % 16].
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+constexpr int kMfmaTiles = 15;   // 240 rows / 16
+
+// (i, k) held at slot = 4 t + q by lane group g: i = 5 g + slot / 12, k = slot % 12
+__host__ __device__ constexpr int mfma_row_i(int g, int slot) { return 5 * g + slot / 12; }
+__host__ __device__ constexpr int mfma_row_k(int slot) { return slot % 12; }
+
+__global__ __launch_bounds__(TB) void k_mf_cells_mfma(int64_t nc, const float* __restrict__ coef,
+                                                      const int32_t* __restrict__ cdT,
+                                                      const int32_t* __restrict__ posT, float cs,
+                                                      float ct, const half8_t* __restrict__ afrag,
+                                                      const float2* __restrict__ x,
+                                                      float2* __restrict__ yl) {
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, col = lane & 15, part = col & 1;
+  half8_t A[kMfmaTiles];
+#pragma unroll
+  for (int t = 0; t < kMfmaTiles; ++t) A[t] = afrag[t * 64 + lane];
+  const int64_t ntile = (nc + 7) / 8;
+  const int64_t nwave = (int64_t)gridDim.x * (TB / 64);
+  float* ylf = reinterpret_cast<float*>(yl);
+  const float inv_den = (float)(1.0 / MfTab<2>::den);
+  // tiles of 8 cells, one wave per tile (the loop condition is uniform over the wave)
+  for (int64_t tile = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6); tile < ntile;
+       tile += nwave) {
+    const int64_t cell = tile * 8 + (col >> 1);
+    const bool valid = cell < nc;
+    float a[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+      a[k] = valid ? coef[k * nc + cell] * (k < 6 ? ct : cs) * inv_den : 0.f;
+    float xv[8];
+    float m = 0.f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int j = 8 * g + e;
+      float v = 0.f;
+      if (valid && j < 20) {
+        const float2 xd = x[cdT[(int64_t)j * nc + cell]];
+        v = part ? xd.y : xd.x;
+      }
+      xv[e] = v;
+      m = fmaxf(m, fabsf(v));
+    }
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    int ex = 0;
+    (void)frexpf(m, &ex);                          // m = f 2^ex, f in [0.5, 1)
+    const float sc = m > 0.f ? ldexpf(1.f, 1 - ex) : 1.f;
+    half8_t hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float v = xv[e] * sc;
+      const _Float16 h = (_Float16)v;
+      hi[e] = h;
+      lo[e] = (_Float16)((v - (float)h) * 2048.f);
+    }
+    float y[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < kMfmaTiles; ++t) {
+      float4_t acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[t], lo, acc, 0, 0, 0);
+      acc = acc * (1.f / 2048.f);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[t], hi, acc, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int slot = 4 * t + q;
+        y[slot / 12] += a[mfma_row_k(slot)] * acc[q];
+      }
+    }
+    if (valid) {
+      const float inv = 1.f / sc;
+#pragma unroll
+      for (int ii = 0; ii < 5; ++ii)
+        ylf[2 * (int64_t)posT[(int64_t)mfma_row_i(g, 12 * ii) * nc + cell] + part] = y[ii] * inv;
+    }
+  }
+}
+
+// fp16 bits of an integer with at most 11 significant bits (exact)
+uint16_t fp16_of_int(int v) {
+  if (v == 0) return 0;
+  const uint16_t sign = v < 0 ? 0x8000 : 0;
+  unsigned a = (unsigned)(v < 0 ? -v : v);
+  int e = 0;
+  while ((a >> (e + 1)) != 0) ++e;                  // a in [2^e, 2^(e+1))
+  SUCEM_CHECK(e <= 15 && (e <= 10 || (a & ((1u << (e - 10)) - 1)) == 0), SUCEM_ERR_INVALID,
+              "reference tensor entry not exact in fp16");
+  const unsigned mant = e >= 10 ? (a >> (e - 10)) & 0x3FF : (a << (10 - e)) & 0x3FF;
+  return (uint16_t)(sign | ((unsigned)(e + 15) << 10) | mant);
+}
+
+// the MFMA A operand of every tile and lane (see k_mf_cells_mfma), built once per space
+void mfma_fragments(Context* c, Space* s) {
+  if (s->mf_afrag.p) return;
+  constexpr int N = 20;
+  auto num = [](int k, int i, int j) {
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    return kMfNum2[k][lo * N - lo * (lo - 1) / 2 + (hi - lo)];
+  };
+  std::vector<uint16_t> h((size_t)kMfmaTiles * 64 * 8, 0);
+  for (int t = 0; t < kMfmaTiles; ++t)
+    for (int lane = 0; lane < 64; ++lane) {
+      const int r = lane % 16, gr = r / 4, slot = 4 * t + r % 4;
+      const int i = mfma_row_i(gr, slot), k = mfma_row_k(slot);
+      for (int e = 0; e < 8; ++e) {
+        const int j = 8 * (lane / 16) + e;
+        h[((size_t)t * 64 + lane) * 8 + e] = j < N ? fp16_of_int(num(k, i, j)) : 0;
+      }
+    }
+  s->mf_afrag.alloc(h.size());
+  h2d(c, s->mf_afrag.p, h.data(), h.size());
+  c->sync();
+}
+
 // Row sums of the DOF-sorted products: L lanes per row, each summing every L-th
 // product of the row's contiguous segment (a wave covers one contiguous stretch of the
 // buffer), then the PEC row mask and the epilogue.  The summation order is fixed per
@@ -358,6 +488,7 @@ void mf_prepare(Context* c, System* sys) {
                  s->mf_posT.p, s->mf_posF.p);
     c->sync();   // temporaries are released on return
   }
+  if (s->order == 2) mfma_fragments(c, s);
   sys->mf_coef64.alloc((size_t)12 * nc);
   sys->mf_coef32.alloc((size_t)12 * nc);
   SUCEM_LAUNCH((k_mf_coefs<double>), grid_for(nc, TB), TB, 0, c->stream, nc, m->xyz.p, m->cells.p,
@@ -393,7 +524,20 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
   }
   C* yl = (C*)sys->mf_yl.p;
   const int Gc = (int)((nc + 8 * TB - 1) / (8 * TB) * 8);   // TB cells per block, multiple of 8
-  if (s->order == 2 && f.U == 2)
+  if (s->order == 2 && f.U == 3) {
+    // element products on the matrix cores (FP32 only; 8 cells per wave and tile)
+    if constexpr (sizeof(R) == 4) {
+      SUCEM_CHECK(s->mf_afrag.p, SUCEM_ERR_INVALID, "MFMA element tables not built");
+      const int64_t ntile = (nc + 7) / 8;
+      const int Gm = (int)std::max<int64_t>(8, std::min<int64_t>((ntile + 3) / 4,
+                                                                   (int64_t)c->num_cus * 8));
+      SUCEM_LAUNCH(k_mf_cells_mfma, Gm, TB, 0, c->stream, nc, coef, s->mf_cdT.p, s->mf_posT.p,
+                   (float)k.s, (float)k.t, (const half8_t*)s->mf_afrag.p, (const float2*)x,
+                   (float2*)yl);
+    } else {
+      SUCEM_CHECK(false, SUCEM_ERR_UNSUPPORTED, "the MFMA element products are FP32 only");
+    }
+  } else if (s->order == 2 && f.U == 2)
     SUCEM_LAUNCH((k_mf_cells<R, 2, true>), Gc, TB, 0, c->stream, nc, Gc, coef, s->mf_cdT.p,
                  s->mf_posT.p, (R)k.s, (R)k.t, (const C*)x, yl);
   else if (s->order == 2)

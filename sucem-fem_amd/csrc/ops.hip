@@ -510,9 +510,12 @@ SpmvCfg tune_sys(Context* c, System* sys, int reps) {
   // matrix-free candidates: element products + row gather
   if (s->nloc == 6 || s->nloc == 20) {
     mf_prepare(c, sys);
-    for (int split : {1, 2})
+    // element products whole (1), half the rows at a time (2), on the matrix cores (3:
+    // order 2, FP32)
+    for (int split : {1, 2, 3})
       for (int lanes : {0, 1, 4})
         for (int pc : per_cu) {
+          if (split == 3 && !(sizeof(R) == 4 && s->order == 2)) continue;
           SpmvCfg f;
           f.mf = 1; f.L = lanes; f.U = split; f.G = op_grid(c, s->ndofs, pc);
           time_cfg(f);
