@@ -243,38 +243,49 @@ constexpr int kMfmaTiles = 15;   // 240 rows / 16
 __host__ __device__ constexpr int mfma_row_i(int g, int slot) { return 5 * g + slot / 12; }
 __host__ __device__ constexpr int mfma_row_k(int slot) { return slot % 12; }
 
+// A block takes 32 cells at a time (4 waves, one tile of 8 cells each): their 12
+// coefficients, 20 x values, places and products go through LDS with coalesced,
+// once-per-cell loads and stores; the waves read their MFMA operands from there.
+constexpr int kMfmaCells = 32;   // cells per block and step (TB / 64 tiles of 8)
 __global__ __launch_bounds__(TB) void k_mf_cells_mfma(int64_t nc, const float* __restrict__ coef,
                                                       const int32_t* __restrict__ cdT,
                                                       const int32_t* __restrict__ posT, float cs,
                                                       float ct, const half8_t* __restrict__ afrag,
                                                       const float2* __restrict__ x,
                                                       float2* __restrict__ yl) {
-  const int lane = threadIdx.x & 63;
+  static_assert(TB == 256 && kMfmaCells == 8 * (TB / 64), "one tile of 8 cells per wave");
+  __shared__ float s_a[12][kMfmaCells];
+  __shared__ float2 s_x[20][kMfmaCells];
+  __shared__ float2 s_y[20][kMfmaCells];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, col = lane & 15, part = col & 1;
+  const int cl = 8 * wave + (col >> 1);            // this lane's cell in the block's 32
   half8_t A[kMfmaTiles];
 #pragma unroll
   for (int t = 0; t < kMfmaTiles; ++t) A[t] = afrag[t * 64 + lane];
-  const int64_t ntile = (nc + 7) / 8;
-  const int64_t nwave = (int64_t)gridDim.x * (TB / 64);
-  float* ylf = reinterpret_cast<float*>(yl);
   const float inv_den = (float)(1.0 / MfTab<2>::den);
-  // tiles of 8 cells, one wave per tile (the loop condition is uniform over the wave)
-  for (int64_t tile = (int64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6); tile < ntile;
-       tile += nwave) {
-    const int64_t cell = tile * 8 + (col >> 1);
-    const bool valid = cell < nc;
-    float a[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k)
-      a[k] = valid ? coef[k * nc + cell] * (k < 6 ? ct : cs) * inv_den : 0.f;
+  const int64_t ngroup = (nc + kMfmaCells - 1) / kMfmaCells;
+  for (int64_t grp = blockIdx.x; grp < ngroup; grp += gridDim.x) {   // uniform per block
+    const int64_t c0 = grp * kMfmaCells;
+    for (int e = threadIdx.x; e < 12 * kMfmaCells; e += TB) {
+      const int k = e / kMfmaCells, ci = e % kMfmaCells;
+      const int64_t c = c0 + ci;
+      s_a[k][ci] = c < nc ? coef[k * nc + c] * (k < 6 ? ct : cs) * inv_den : 0.f;
+    }
+    for (int e = threadIdx.x; e < 20 * kMfmaCells; e += TB) {
+      const int j = e / kMfmaCells, ci = e % kMfmaCells;
+      const int64_t c = c0 + ci;
+      s_x[j][ci] = c < nc ? x[cdT[(int64_t)j * nc + c]] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
     float xv[8];
     float m = 0.f;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int j = 8 * g + e;
       float v = 0.f;
-      if (valid && j < 20) {
-        const float2 xd = x[cdT[(int64_t)j * nc + cell]];
+      if (j < 20) {
+        const float2 xd = s_x[j][cl];
         v = part ? xd.y : xd.x;
       }
       xv[e] = v;
@@ -293,6 +304,9 @@ __global__ __launch_bounds__(TB) void k_mf_cells_mfma(int64_t nc, const float* _
       hi[e] = h;
       lo[e] = (_Float16)((v - (float)h) * 2048.f);
     }
+    float a[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) a[k] = s_a[k][cl];
     float y[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < kMfmaTiles; ++t) {
@@ -306,12 +320,18 @@ __global__ __launch_bounds__(TB) void k_mf_cells_mfma(int64_t nc, const float* _
         y[slot / 12] += a[mfma_row_k(slot)] * acc[q];
       }
     }
-    if (valid) {
-      const float inv = 1.f / sc;
+    const float inv = 1.f / sc;
+    float* sy = reinterpret_cast<float*>(&s_y[0][0]);
 #pragma unroll
-      for (int ii = 0; ii < 5; ++ii)
-        ylf[2 * (int64_t)posT[(int64_t)mfma_row_i(g, 12 * ii) * nc + cell] + part] = y[ii] * inv;
+    for (int ii = 0; ii < 5; ++ii)
+      sy[2 * (mfma_row_i(g, 12 * ii) * kMfmaCells + cl) + part] = y[ii] * inv;
+    __syncthreads();
+    for (int e = threadIdx.x; e < 20 * kMfmaCells; e += TB) {
+      const int i = e / kMfmaCells, ci = e % kMfmaCells;
+      const int64_t c = c0 + ci;
+      if (c < nc) yl[posT[(int64_t)i * nc + c]] = s_y[i][ci];
     }
+    __syncthreads();   // s_a, s_x and s_y are refilled by the next step
   }
 }
 
@@ -525,11 +545,11 @@ void mf_apply(Context* c, const System* sys, Coef k, const uint8_t* mask, const 
   C* yl = (C*)sys->mf_yl.p;
   const int Gc = (int)((nc + 8 * TB - 1) / (8 * TB) * 8);   // TB cells per block, multiple of 8
   if (s->order == 2 && f.U == 3) {
-    // element products on the matrix cores (FP32 only; 8 cells per wave and tile)
+    // element products on the matrix cores (FP32 only; 32 cells per block step)
     if constexpr (sizeof(R) == 4) {
       SUCEM_CHECK(s->mf_afrag.p, SUCEM_ERR_INVALID, "MFMA element tables not built");
-      const int64_t ntile = (nc + 7) / 8;
-      const int Gm = (int)std::max<int64_t>(8, std::min<int64_t>((ntile + 3) / 4,
+      const int64_t ngroup = (nc + kMfmaCells - 1) / kMfmaCells;   // 32 cells per block step
+      const int Gm = (int)std::max<int64_t>(1, std::min<int64_t>(ngroup,
                                                                    (int64_t)c->num_cus * 8));
       SUCEM_LAUNCH(k_mf_cells_mfma, Gm, TB, 0, c->stream, nc, coef, s->mf_cdT.p, s->mf_posT.p,
                    (float)k.s, (float)k.t, (const half8_t*)s->mf_afrag.p, (const float2*)x,
